@@ -1,0 +1,145 @@
+"""Benchmark: cell-steps/s of the batched Rosenbrock FP64 solver on one MI355X.
+
+Workload (the 48-species / 150-reaction synthetic mechanism, ~18 % fill after
+the symbolic LU): 65,536 cells, pH 2-7, rtol 1e-6 atol 1e-12, Rodas3, and ten
+operator-split advance calls of 1 s each carrying h. One bench "step" is one
+1-s advance of all cells (one kernel launch); the default K = 10 steps is the
+whole 10-s integration. Inputs are resident in HBM before timing starts.
+
+value = accepted cell-steps in the timed steps / wall seconds (the reference's
+own definition, src/tests/test_kinetics.cpp:1037-1042).
+
+Prints one JSON line with the roofline of the chemistry kernel (algorithmic
+FP64 flops per launch from the mechanism tables / kernel time from HIP events
+on the module stream) and the reference's CPU path timed on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_VALU_TFLOPS = 78.6     # MI355X FP64 vector, dense
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(mech, meth, ctl, sp, cells, duration):
+    """The reference's own FP64 AVX2 path (oracle/_ref, f64x4 lanes, threads as
+    dbd_advance_threads) on the first `cells` cells for one advance; falls back
+    to the oracle's scalar restatement if the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi
+    from simstack_amd import problem
+    threads = min(os.cpu_count() or 1, 64)
+    z = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
+    t0 = time.perf_counter()
+    if oracle_ffi.have_ref():
+        mb = mech.rebuild(oracle_ffi.ref_caps()).bytes
+        oracle_ffi.ref_advance(mb, oracle_ffi.ref_method_bytes(2), ctl, z, duration, z.padded, width=4,
+                               threads=threads)
+        kind = "reference"
+    else:
+        oracle_ffi.oracle_advance(mech, meth, ctl, z, duration, threads=threads)
+        kind = "port"
+    wall = time.perf_counter() - t0
+    steps = float(z.steps[:cells].sum())
+    return {"value": steps / wall, "unit": "cell-steps/s", "cores": threads, "kind": kind,
+            "sample": f"first {cells} cells of the workload, one {duration:g}-s advance "
+                      f"({int(steps)} accepted steps, {wall:.1f} s wall)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=65536)
+    ap.add_argument("--cpu-cells", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generated, 2 table walk")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (cells shard with no "
+              "communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from simstack_amd import gpu, problem
+    sp = problem.spec("c2")
+    mech = problem.Mechanism.synthetic(sp)
+    meth, ctl = problem.method(2), problem.control()
+    flops_attempt = mech.flops_per_attempt()
+    z0 = problem.Zones(mech.n, mech.R, args.cells).fill_synthetic(sp, mech.params)
+
+    gm = gpu.GpuModule(arena_bytes=z0.block.nbytes + (64 << 20))
+    gm.set_kernel(args.kernel)
+    gm.set_mech(mech, meth, ctl)
+    dz = gpu.DeviceZones(gm, z0.copy())
+
+    # Warmup: advance from the initial state, then restore it on the device.
+    dz.upload()
+    for _ in range(args.warmup):
+        dz.advance(1.0)
+    dz.z.block[:] = z0.block
+    dz.upload()
+
+    steps0 = z0.steps[:args.cells].copy()
+    rej0 = z0.rejects[:args.cells].copy()
+    kernel_ms = []
+    gm.lib.gpu_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(dz.advance(1.0))
+    gm.lib.gpu_sync()
+    wall = time.perf_counter() - t0
+    dz.download()
+    z = dz.z
+    acc = float((z.steps[:args.cells] - steps0).sum())
+    att = acc + float((z.rejects[:args.cells] - rej0).sum())
+    failed = int(z.failed[:args.cells].sum())
+    info = gm.kin_info()
+
+    avg_ms = float(np.mean(kernel_ms))
+    flops_per_launch = att * flops_attempt / args.steps
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    out = {
+        "metric": "cell-steps/s at rtol 1e-6 (FP64, 1 GPU)",
+        "value": acc / wall,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mechanism and cells)",
+        "config": {"workload": "48 species / 150 reactions, 65,536 cells, 10 x 1 s operator-split, Rodas3, "
+                               "rtol 1e-6 atol 1e-12", "cells": args.cells, "species": mech.n,
+                   "reactions": mech.R, "nnz": mech.nnz, "lu_div": mech.lu_div, "lu_sub": mech.lu_sub,
+                   "kernel": info.kernel_name.decode(), "parallelism": "1 GPU, one cell per lane"},
+        "attempts_per_s": att / wall,
+        "accepted_steps": acc,
+        "failed_cells": failed,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": traffic,
+                     "kernel_ms_avg": avg_ms, "flops_per_attempt": flops_attempt},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+/* simk_gpu_api.h — C ABI of libsimk_gpu.so, the MI355X GPU module.
+ *
+ * The first block is the reference's GPU module interface, same names, same
+ * signatures, same meaning (src/gpu/gpu_api.h:47-115), so the reference's
+ * loader (src/platform/linux_gpu.cpp:7-31) can dlopen this library in place of
+ * its CUDA module. Return values are hipError_t cast to s32 (0 = success),
+ * decoded by gpu_error_string. The module owns one device bump arena, one
+ * non-blocking HIP stream and its events; calls are not re-entrant (one
+ * thread at a time), as in the reference.
+ *
+ * The second block extends it for what the reference lacks: mechanisms larger
+ * than its compile-time capacities, the kernel choice, mixed precision, the
+ * operator-split advection stencil and per-call statistics.
+ */
+#ifndef SIMK_GPU_API_H
+#define SIMK_GPU_API_H
+
+#include "simk_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPU_API __attribute__((visibility("default")))
+
+/* ---- reference interface (src/gpu/gpu_api.h) ---------------------------- */
+
+/* gpu_api.h:47 — selects the device, reserves arena_bytes of device memory
+ * for gpu_alloc, creates the stream and events, fills *info. */
+GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info);
+/* gpu_api.h:50 */
+GPU_API const char *gpu_error_string(s32 error);
+/* gpu_api.h:54 — bump allocation from the arena, 256-byte aligned; 0 when full. */
+GPU_API void *gpu_alloc(u64 size);
+/* gpu_api.h:57 */
+GPU_API u64 gpu_arena_used(void);
+/* gpu_api.h:62 — waits for the device, then rolls the arena back to empty. */
+GPU_API void gpu_arena_reset(void);
+/* gpu_api.h:65 — copy on the module stream, then wait for it. */
+GPU_API s32 gpu_copy_to_device(void *dst_device, const void *src_host, u64 size);
+/* gpu_api.h:68 */
+GPU_API s32 gpu_copy_to_host(void *dst_host, const void *src_device, u64 size);
+/* gpu_api.h:71 — waits for the module stream. */
+GPU_API s32 gpu_sync(void);
+/* gpu_api.h:75 — the reference's point-particle demo step (kept so the
+ * reference loader finds every export). */
+GPU_API s32 gpu_step_points(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy, f32 *vz, u32 count, f32 dt);
+/* gpu_api.h:80 — mechanism (KinMech at the default capacities), method and
+ * control for the next advances. Generates and JIT-compiles the mechanism's
+ * straight-line kernel (cached on disk) and uploads the table-walk tables. */
+GPU_API s32 gpu_kin_set_mech(const KinMech *mech, const KinMethod *method, const KinControl *control);
+/* gpu_api.h:85 — advances lanes [0, lane_count or padded) of the block over
+ * [0, duration] and waits; *kernel_ms = kernel time from stream events.
+ * Results are bit-identical to the reference's FP64 CPU path (precision 0). */
+GPU_API s32 gpu_kin_advance(const GpuKinLaunch *launch, f32 *kernel_ms);
+/* gpu_api.h:89 — records the range; the platform's pinned-memory path is
+ * not used (copies from pageable memory are staged by the runtime). */
+GPU_API s32 gpu_host_register(void *host, u64 size);
+/* gpu_api.h:91 */
+GPU_API s32 gpu_host_unregister(void *host);
+/* gpu_api.h:99 — uploads lanes [0, lane_count) of every row of the
+ * [rows][padded] block, launches, downloads the first mutable_rows rows, all
+ * on the module stream; returns at once. One advance may be outstanding. */
+GPU_API s32 gpu_kin_advance_async(const GpuKinLaunch *launch, void *host_block, void *device_block, u64 rows,
+                                  u64 mutable_rows);
+/* gpu_api.h:105 — waits for the async advance; kernel and total milliseconds. */
+GPU_API s32 gpu_kin_wait(f32 *kernel_ms, f32 *total_ms);
+/* gpu_api.h:109 — 0: one cell per lane (default); 1: a team of lanes per cell. */
+GPU_API s32 gpu_kin_set_mode(u32 mode);
+
+/* ---- extensions (no counterpart in the reference) ----------------------- */
+
+/* gpu_kin_set_mech for a KinMech compiled with capacities caps (mechanisms
+ * beyond the default 32 species / 96 reactions / 512 non-zeros). */
+GPU_API s32 simk_kin_set_mech_caps(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                   const KinControl *control);
+
+/* Kernel used by the advances: 0 auto (generated when it compiled, else
+ * table), 1 generated straight-line, 2 table walk. */
+GPU_API s32 simk_kin_set_kernel(u32 kind);
+
+/* Generator knobs (speed only; results unchanged): see SimkGenOptions. */
+typedef struct SimkGenOptions {
+    u32 fence_rows;
+    u32 waves_per_eu;
+    u32 unroll_stages;
+    u32 pad_;
+} SimkGenOptions;
+GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
+
+/* Statistics of the current mechanism and the last advance. */
+typedef struct SimkKinInfo {
+    u32 kernel;          /* 1 generated, 2 table */
+    u32 ws_slots;        /* workspace f64 slots per lane */
+    u32 species, reactions, nnz, lu_div, lu_sub;
+    u32 jit_cached;      /* 1: the generated kernel came from the disk cache */
+    f64 jit_ms;          /* generate + compile (or cache load) time */
+    f64 last_kernel_ms;
+    u64 signature;       /* FNV-1a of the table content */
+    char kernel_name[64];
+} SimkKinInfo;
+GPU_API s32 simk_kin_info(SimkKinInfo *out);
+
+/* Generates and compiles the straight-line kernel for a mechanism into the
+ * on-disk cache without touching a device (build step). 0 on success; the
+ * compile log is written to log (may be null). */
+GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                const SimkGenOptions *options, char *log, u32 log_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMK_GPU_API_H */

@@ -1,0 +1,222 @@
+// Straight-line kernel generator (see codegen.hpp).
+//
+// Emitted per mechanism:
+//   fun         f = src; per reaction: rate = k_r * y_a [* y_b]; f_s = fma(coef, rate, f_s)
+//               in reaction order (kin_codegen.cpp:58-72 order)
+//   jac_factor  row by row: the row's -J entries accumulated in (reaction,
+//               reactant, net entry) order starting from +0.0 (kin_solver.h:75-93),
+//               then the row's eliminations against finished rows and its
+//               reciprocal pivot (kin_solver.h:95-113), then the row is stored.
+//               Row i of -J depends only on y, and row i of the factor reads only
+//               rows k < i, so fusing the two per row gives the table walk's bits.
+//   solve       forward over lower (ascending col), backward over upper
+//               (descending col) then the reciprocal pivot (kin_solver.h:122-139)
+// y, f and the stage vector being solved live in registers; the LU and earlier
+// stage vectors live in the lane's workspace slots ([wave][slot][64] f64, so
+// each access is one coalesced 512-byte wave transaction).
+#include "codegen.hpp"
+
+#include <cstdarg>
+#include <cstdio>
+#include <vector>
+
+namespace simk {
+
+namespace {
+
+struct Out {
+    std::string s;
+    void operator()(const char *fmt, ...) __attribute__((format(printf, 2, 3))) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        s += buf;
+    }
+};
+
+std::string lit(f64 v) {
+    char buf[64];
+    snprintf(buf, sizeof(buf), "%a", v);   // exact hexadecimal literal
+    return buf;
+}
+
+}  // namespace
+
+void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
+    // f holds f(point of stage `holder`); at_y: that point is y itself.
+    u32 holder = 0;
+    bool at_y = true;
+    for (u32 st = 0; st < 4; ++st) eval[st] = save_f[st] = 0;
+    for (u32 st = 1; st < me.stages && st < 4; ++st) {
+        bool reuse = !me.new_f[st];
+        if (!reuse) {
+            const u32 packed = st * (st - 1) / 2;
+            bool all_zero = true;
+            for (u32 j = 0; j < st; ++j)
+                if (me.a[packed + j] != 0.0) all_zero = false;
+            // With every a_ij zero the stage point is y: if f(y) is the f held,
+            // recomputing it would reproduce the same bits.
+            reuse = all_zero && at_y;
+            if (!reuse) {
+                eval[st] = 1;
+                holder = st;
+                at_y = all_zero;
+            }
+        }
+        if (reuse) save_f[holder] = 1;
+    }
+}
+
+GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, const char *device_header) {
+    GenResult g;
+    const u32 n = m.n;
+    u64 sig = signature(m);
+    char name[64];
+    snprintf(name, sizeof(name), "simk_gen_%016llx", (unsigned long long)sig);
+    g.kernel = name;
+    const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;   // the last stage stays in registers
+    const u32 foff = m.nnz + stored_stages * n;                       // the reused f
+    g.ws_slots = foff + n;
+
+    Out w;
+    w("// generated by simstack_amd codegen: %u species, %u reactions, %u non-zeros\n", n, m.R, m.nnz);
+    w.s += device_header;
+    w.s += "\n";
+    w("struct GenOps {\n");
+    w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
+    w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, STAGES = %u;\n", m.nnz, foff, me.stages);
+    w("    using Vec = SimkRegVec<N>;\n");
+    w("    SimkRows kr, sr, pr, ws;\n");
+    w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
+    w("        const simk_u32 w0 = wave * SIMK_WAVE, stride = A.padded * 8u, tail = (A.padded - w0) * 8u;\n");
+    w("        kr.init(A.k + w0, lane, stride, (R - 1) * stride + tail);\n");
+    w("        sr.init(A.src + w0, lane, stride, (N - 1) * stride + tail);\n");
+    w("        pr.init(A.pulse + w0, lane, stride, (N - 1) * stride + tail);\n");
+    w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    w("    }\n");
+    w("    static SIMK_DEV constexpr simk_u32 n() { return N; }\n");
+    w("    SIMK_DEV Vec vec(int) const { return Vec(); }\n");
+    w("    SIMK_DEV double K(simk_u32 r) const { return kr.ld(r); }\n");
+    w("    SIMK_DEV double S(simk_u32 s) const { return sr.ld(s); }\n");
+    w("    SIMK_DEV double pulse(simk_u32 s) const { return pr.ld(s); }\n");
+    w("    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }\n");
+    w("    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }\n");
+    w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(KOFF + st * N + s); }\n");
+    w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
+    w("        if (st + 1 >= STAGES) return;\n");
+    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(KOFF + st * N + s, x.v[s]);\n    }\n");
+    w("    SIMK_DEV double kfinal(simk_u32 st, simk_u32 s, const Vec &x) const {\n");
+    w("        return st + 1 == STAGES ? x.v[s] : kget(st, s);\n    }\n");
+    w("    SIMK_DEV void fsave(const Vec &x) const {\n");
+    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(FOFF + s, x.v[s]);\n    }\n");
+    w("    SIMK_DEV void fload(Vec &x) const {\n");
+    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
+
+    // fun
+    w("    SIMK_DEV void fun(const Vec &y, Vec &f) const {\n");
+    for (u32 s = 0; s < n; ++s) w("        f.v[%u] = S(%u);\n", s, s);
+    for (u32 r = 0; r < m.R; ++r) {
+        const Reaction &x = m.rx[r];
+        w("        {\n            const double rate = K(%u)", r);
+        for (u32 p = 0; p < x.reactant_count; ++p) w(" * y.v[%u]", x.reactant[p]);
+        w(";\n");
+        for (u32 e = 0; e < x.net_count; ++e)
+            w("            f.v[%u] = simk_fma(%d.0, rate, f.v[%u]);\n", x.net_species[e], (int)x.net_coef[e],
+              x.net_species[e]);
+        w("        }\n");
+    }
+    w("    }\n");
+
+    // jac_factor: rows in order.
+    w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
+    for (u32 i = 0; i < n; ++i) {
+        std::vector<u32> row;   // compact indices of row i
+        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) row.push_back(m.lower[e].index);
+        row.push_back(m.diag[i]);
+        for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e) row.push_back(m.upper[e].index);
+        w("        { // row %u\n", i);
+        for (u32 idx : row) w("            double a%u = 0.0;\n", idx);
+        // -J contributions, per entry in (r, p, e) order.
+        for (u32 r = 0; r < m.R; ++r) {
+            const Reaction &x = m.rx[r];
+            for (u32 p = 0; p < x.reactant_count; ++p) {
+                bool any = false;
+                for (u32 e = 0; e < x.net_count; ++e)
+                    if (x.net_species[e] == i) any = true;
+                if (!any) continue;
+                w("            { const double d = K(%u)", r);
+                for (u32 q = 0; q < x.reactant_count; ++q)
+                    if (q != p) w(" * y.v[%u]", x.reactant[q]);
+                w(";");
+                for (u32 e = 0; e < x.net_count; ++e) {
+                    if (x.net_species[e] != i) continue;
+                    u32 idx = x.jac_index[p][e];
+                    w(" a%u = simk_fma(%d.0, d, a%u);", idx, -(int)x.net_coef[e], idx);
+                }
+                w(" }\n");
+            }
+        }
+        // Eliminations against finished rows k < i.
+        for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
+            const KinLuDiv &dv = m.lu_div[d];
+            w("            { const double l = a%u * W(%u); a%u = l;\n", dv.ik, dv.kk, dv.ik);
+            for (u32 s = dv.sub_begin; s < dv.sub_end; ++s)
+                w("              a%u = simk_fnma(l, W(%u), a%u);\n", m.lu_sub[s].ij, m.lu_sub[s].kj, m.lu_sub[s].ij);
+            w("            }\n");
+        }
+        w("            a%u = 1.0 / (a%u + g);\n", m.diag[i], m.diag[i]);
+        for (u32 idx : row) w("            WS(%u, a%u);\n", idx, idx);
+        if (o.fence_rows) w("            SIMK_MEMORY_FENCE();\n");
+        w("        }\n");
+    }
+    w("        SIMK_MEMORY_FENCE();\n    }\n");
+
+    // solve
+    w("    SIMK_DEV void solve(Vec &x) const {\n        SIMK_MEMORY_FENCE();\n");
+    for (u32 i = 0; i < n; ++i)
+        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e)
+            w("        x.v[%u] = simk_fnma(W(%u), x.v[%u], x.v[%u]);\n", i, m.lower[e].index, m.lower[e].col, i);
+    for (u32 ii = n; ii-- > 0;) {
+        for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e)
+            w("        x.v[%u] = simk_fnma(W(%u), x.v[%u], x.v[%u]);\n", ii, m.upper[e].index, m.upper[e].col, ii);
+        w("        x.v[%u] = x.v[%u] * W(%u);\n", ii, ii, m.diag[ii]);
+    }
+    w("    }\n};\n\n");
+
+    // Method with compile-time coefficients.
+    u32 eval[4], save_f[4];
+    stage_plan(me, eval, save_f);
+    w("struct GenMethod {\n");
+    w("    static constexpr bool kUnrollStages = %s;\n", o.unroll_stages ? "true" : "false");
+    w("    SIMK_DEV static constexpr simk_u32 stages() { return %u; }\n", me.stages);
+    w("    SIMK_DEV static constexpr simk_u32 order() { return %u; }\n", me.order);
+    w("    SIMK_DEV static constexpr double gamma0() { return %s; }\n", lit(me.gamma0).c_str());
+    auto table = [&](const char *fn, const f64 *v, int count) {
+        w("    SIMK_DEV static constexpr double %s(simk_u32 i) {\n", fn);
+        for (int q = 0; q < count; ++q) w("        if (i == %d) return %s;\n", q, lit(v[q]).c_str());
+        w("        return 0.0;\n    }\n");
+    };
+    table("a", me.a, 6);
+    table("c", me.c, 6);
+    table("m", me.m, 4);
+    table("e", me.e, 4);
+    w("    SIMK_DEV static constexpr bool eval(simk_u32 i) {\n");
+    for (int q = 0; q < 4; ++q) w("        if (i == %d) return %s;\n", q, eval[q] ? "true" : "false");
+    w("        return false;\n    }\n");
+    w("    SIMK_DEV static constexpr bool save_f(simk_u32 i) {\n");
+    for (int q = 0; q < 4; ++q) w("        if (i == %d) return %s;\n", q, save_f[q] ? "true" : "false");
+    w("        return false;\n    }\n};\n\n");
+
+    if (o.waves_per_eu)
+        w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) __attribute__((amdgpu_waves_per_eu(%u))) %s(SimkKinArgs A) {\n",
+          o.waves_per_eu, name);
+    else
+        w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) %s(SimkKinArgs A) {\n", name);
+    w("    GenMethod me;\n    simk_lane_body<GenOps, GenMethod>(A, me);\n}\n");
+    g.source = std::move(w.s);
+    return g;
+}
+
+}  // namespace simk

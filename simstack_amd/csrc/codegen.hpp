@@ -1,0 +1,31 @@
+// Kernel generator: emits HIP source for one mechanism with the four Ops
+// kernels unrolled into straight-line code from the symbolic pattern — the
+// role of the reference's build-time kin_codegen (src/tools/kin_codegen.cpp:
+// 32-138), here run when a mechanism is set and compiled for gfx950 by hipRTC.
+#pragma once
+
+#include <string>
+
+#include "mech.hpp"
+
+namespace simk {
+
+// Knobs the autotuner sweeps. Each changes speed only, never results.
+struct GenOptions {
+    u32 fence_rows = 1;      // compiler memory fence after every factored row (else only after the factor)
+    u32 waves_per_eu = 0;    // amdgpu_waves_per_eu lower bound (0: compiler default)
+    u32 unroll_stages = 0;   // unroll the stage loop (bigger code, constant stage offsets)
+};
+
+struct GenResult {
+    std::string source;   // complete translation unit (device header included)
+    std::string kernel;   // extern "C" kernel name
+    u32 ws_slots = 0;     // workspace slots per lane the kernel uses
+};
+
+// Stage plan shared with the host launch code: which stages evaluate f.
+void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);
+
+GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, const char *device_header);
+
+}  // namespace simk

@@ -1,0 +1,367 @@
+// Device side of the batched Rosenbrock solver for gfx950: one cell per lane,
+// 64-lane wavefronts, per-lane adaptive step control.
+//
+// This header is compiled twice: into libsimk_gpu.so (the table-walk kernel,
+// any mechanism) and, as embedded source text, into every kernel the code
+// generator emits for one mechanism through hipRTC. It includes no system or
+// HIP headers so hipRTC compiles it as is.
+//
+// Arithmetic contract (the reference's lane rules, src/base/base_lane.h:13-21):
+// every fused multiply-add is an explicit __builtin_fma (files are compiled
+// with -ffp-contract=off); division and sqrt are IEEE correctly rounded f64;
+// min/max are the select forms "a < b ? a : b"; |a| keeps -0; finiteness is
+// tested as (x - x) == (x - x). So each lane performs exactly the operation
+// sequence of the reference's scalar kin_advance on its cell.
+//
+// Memory: every [row][lanes] f64 array (the zone block rows and the per-lane
+// workspace) is addressed through a buffer resource whose base is the wave's
+// first lane: the lane offset is one VGPR, the row offset a scalar register,
+// so no per-lane 64-bit address is ever kept live, and every access is one
+// coalesced 512-byte wave transaction.
+#pragma once
+
+typedef unsigned int simk_u32;
+typedef unsigned long long simk_u64;
+typedef unsigned int simk_v2u __attribute__((ext_vector_type(2)));
+
+#define SIMK_WAVE 64
+#define SIMK_DEV __device__ __forceinline__
+
+struct SimkDevCtl {
+    double rtol, atol, h_min, h_max, h_start;
+    double inv_n;            // 1.0 / n, rounded once on the host (kin_solver.h:197)
+    simk_u32 max_steps, fixed_step, clamp_negative, pad_;
+};
+
+// Method coefficients with the host-derived stage plan.
+struct SimkDevMethod {
+    simk_u32 stages, order;
+    double a[6], c[6], m[4], e[4], gamma0;
+    simk_u32 new_f[4];
+    simk_u32 eval[4];        // stage st evaluates f at its own point (else it reuses the saved f)
+    simk_u32 save_f[4];      // stage st's f is reused by a later stage
+};
+
+// Mechanism tables of the table-walk kernel, in device memory (uniform reads).
+struct SimkDevTables {
+    simk_u32 n, R, nnz, pad_;
+    const unsigned char *reactant;      // [R][3]
+    const unsigned char *reactant_count;// [R]
+    const unsigned char *net_count;     // [R]
+    const unsigned char *net_species;   // [R][6]
+    const double *net_coef;             // [R][6]
+    const unsigned short *jac_index;    // [R][3][6]
+    const unsigned short *diag;         // [n]
+    const unsigned short *div_begin;    // [n + 1]
+    const unsigned short *lu_div;       // [D][4]: ik, kk, sub_begin, sub_end
+    const unsigned short *lu_sub;       // [S][2]: ij, kj
+    const unsigned short *lower_begin;  // [n + 1]
+    const unsigned short *upper_begin;  // [n + 1]
+    const unsigned short *lower;        // [L][2]: index, col
+    const unsigned short *upper;        // [U][2]
+};
+
+// Kernel argument: the zone block (GpuKinLaunch fields) plus workspace,
+// control and method, passed by value.
+struct SimkKinArgs {
+    double *y, *h, *steps, *rejects, *pulses, *failed;
+    const double *k, *src, *pulse, *period, *pulse_first, *pulse_count;
+    simk_u32 padded;         // row stride of the block (lanes)
+    simk_u32 lanes;          // cells [0, lanes) are integrated
+    double duration;
+    double *ws;              // workspace: [wave][slot][64] f64
+    simk_u32 ws_slots;       // slots per lane
+    simk_u32 n_species, n_reactions, pad_;
+    const SimkDevTables *tables;
+    SimkDevCtl ctl;
+    SimkDevMethod method;
+};
+
+// ---- lane vocabulary ------------------------------------------------------
+SIMK_DEV double simk_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+SIMK_DEV double simk_fnma(double a, double b, double c) { return __builtin_fma(-a, b, c); }
+SIMK_DEV double simk_min(double a, double b) { return a < b ? a : b; }
+SIMK_DEV double simk_max(double a, double b) { return a > b ? a : b; }
+SIMK_DEV double simk_abs(double a) { return a < 0.0 ? -a : a; }
+SIMK_DEV double simk_sqrt(double a) { return __builtin_sqrt(a); }
+SIMK_DEV bool simk_is_finite(double a) {
+    double d = a - a;
+    return d == d;
+}
+
+// x^(1/3) on [1e-4, 1e4] from arithmetic only (base_lane.h:123-135).
+SIMK_DEV double simk_cbrt_clamped(double x) {
+    x = simk_min(simk_max(x, 1e-4), 1e4);
+    double r4 = simk_sqrt(simk_sqrt(x));
+    double r16 = simk_sqrt(simk_sqrt(r4));
+    double r64 = simk_sqrt(simk_sqrt(r16));
+    double g = r4 * r16 * r64;
+    const double third = 1.0 / 3.0;
+    g = (2.0 * g + x / (g * g)) * third;
+    g = (2.0 * g + x / (g * g)) * third;
+    return g;
+}
+
+// Step-size factor (kin_solver.h:169-180).
+SIMK_DEV double simk_step_factor(double err, simk_u32 order) {
+    double e = simk_min(simk_max(err, 1e-4), 1e4);
+    double root;
+    if (order == 2) root = simk_sqrt(e);
+    else if (order == 4) root = simk_sqrt(simk_sqrt(e));
+    else root = simk_cbrt_clamped(e);
+    double fac = 0.9 / root;
+    return simk_min(simk_max(fac, 0.2), 6.0);
+}
+
+// Compiler-level memory fence: values stored to the workspace are re-read from
+// memory after it instead of being kept live in registers (which would spill).
+#define SIMK_MEMORY_FENCE() asm volatile("" ::: "memory")
+
+// ---- [row][lanes] f64 arrays through a buffer resource ----------------------
+// Raw buffer, 32-bit data format word (gfx9 family descriptor word 3).
+#define SIMK_RSRC_FLAGS 0x00027000
+
+struct SimkRows {
+    __amdgpu_buffer_rsrc_t rs;
+    simk_u32 voff;           // lane * 8
+    simk_u32 stride;         // bytes between rows
+    SIMK_DEV void init(const double *wave_base, simk_u32 lane, simk_u32 row_stride_bytes, simk_u32 bytes) {
+        rs = __builtin_amdgcn_make_buffer_rsrc((void *)wave_base, 0, (int)bytes, SIMK_RSRC_FLAGS);
+        voff = lane * 8u;
+        stride = row_stride_bytes;
+    }
+    SIMK_DEV double ld(simk_u32 row) const {
+        simk_v2u v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)(row * stride), 0);
+        return __builtin_bit_cast(double, v);
+    }
+    SIMK_DEV void st(simk_u32 row, double x) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(simk_v2u, x), rs, (int)voff, (int)(row * stride), 0);
+    }
+};
+
+// A species vector in registers (generated kernels: N is a compile-time constant).
+template <simk_u32 N>
+struct SimkRegVec {
+    double v[N];
+    SIMK_DEV double get(simk_u32 s) const { return v[s]; }
+    SIMK_DEV void set(simk_u32 s, double x) { v[s] = x; }
+};
+
+// A species vector in workspace slots [base, base + n) (table-walk kernel).
+struct SimkWsVec {
+    SimkRows w;
+    simk_u32 base;
+    SIMK_DEV double get(simk_u32 s) const { return w.ld(base + s); }
+    SIMK_DEV void set(simk_u32 s, double x) const { w.st(base + s, x); }
+};
+
+// ---------------------------------------------------------------------------
+// The integrator, written once over an Ops policy P:
+//   P::Vec                 per-lane species vector (get/set)
+//   p.n()                  species count (constexpr in generated code)
+//   p.vec(i)               storage for vector i (0 y, 2 stage point / y_new, 3 stage solution)
+//   p.fun(y, f)            f = src + sum_r net_r k_r prod y         (kin_solver.h:39-53)
+//   p.jac_factor(y, g)     LU of (I g - J(y)) into the workspace    (:75-113)
+//   p.solve(x)             x = (LU)^-1 x                             (:122-139)
+//   p.kget(j, s), p.kstore(st, x), p.kfinal(st, s, x)   stage vectors K_st
+//   p.fsave(x), p.fload(x) the f a later stage reuses
+//   p.pulse(s)             the cell's pulse increment
+// and a method policy M (compile-time constants in generated code).
+//
+// Order of work inside one step attempt differs from the reference only where
+// results cannot depend on it: -J/LU is formed before f(y) (the two read y
+// only), and a stage whose point equals the previous evaluation point reuses
+// the saved f instead of recomputing the same bits (Rodas3 stage 2).
+
+template <class P, class M>
+SIMK_DEV void simk_integrate(P &p, const M &me, const SimkDevCtl &ctl, typename P::Vec &y, double &t,
+                             const double t_end, double &h, double &steps, double &rejects, double &failed) {
+    const simk_u32 n = p.n();
+    bool active = (t < t_end) && !(failed > 0.0);
+    bool reject_last = false, reject_more = false;
+    h = h > 0.0 ? h : ctl.h_start;
+    simk_u32 iterations = 0;
+    while (active) {
+        if (++iterations > ctl.max_steps) {
+            failed = 1.0;
+            break;
+        }
+        const double remaining = t_end - t;
+        double h_cap = remaining;
+        if (ctl.h_max > 0.0) h_cap = simk_min(h_cap, ctl.h_max);
+        const double h_wanted = simk_max(h, ctl.h_min);
+        const bool last = h_wanted >= remaining;
+        const double hs = last ? remaining : simk_min(h_wanted, h_cap);
+
+        const double ghinv = 1.0 / (hs * me.gamma0());
+        p.jac_factor(y, ghinv);
+        const double hinv = 1.0 / hs;
+
+        typename P::Vec x = p.vec(3);
+        // One stage: its point, f there (or the saved f), the c-combination
+        // of earlier stages, the solve. Rolled by default so the solve and f
+        // are emitted once; the generator may unroll it.
+        auto stage = [&](const simk_u32 st) {
+            const simk_u32 packed = st * (st - 1) / 2;
+            if (st == 0 || me.eval(st)) {
+                typename P::Vec pt = p.vec(2);
+#pragma unroll
+                for (simk_u32 s = 0; s < n; ++s) pt.set(s, y.get(s));
+                for (simk_u32 j = 0; j < st; ++j) {
+                    const double a = me.a(packed + j);
+                    if (a == 0.0) continue;
+#pragma unroll
+                    for (simk_u32 s = 0; s < n; ++s) pt.set(s, simk_fma(a, p.kget(j, s), pt.get(s)));
+                }
+                p.fun(pt, x);
+                if (me.save_f(st)) p.fsave(x);
+            } else {
+                p.fload(x);
+            }
+            for (simk_u32 j = 0; j < st; ++j) {
+                const double c = me.c(packed + j);
+                if (c == 0.0) continue;
+                const double hc = c * hinv;
+#pragma unroll
+                for (simk_u32 s = 0; s < n; ++s) x.set(s, simk_fma(hc, p.kget(j, s), x.get(s)));
+            }
+            p.solve(x);
+            p.kstore(st, x);
+        };
+        if constexpr (M::kUnrollStages) {
+#pragma unroll
+            for (simk_u32 st = 0; st < 4; ++st)
+                if (st < me.stages()) stage(st);
+        } else {
+#pragma unroll 1
+            for (simk_u32 st = 0; st < me.stages(); ++st) stage(st);
+        }
+
+        typename P::Vec ynew = p.vec(2);
+        double err = 0.0;
+#pragma unroll
+        for (simk_u32 s = 0; s < n; ++s) {
+            double yn = y.get(s), yerr = 0.0;
+#pragma unroll
+            for (simk_u32 st = 0; st < 4; ++st) {
+                if (st >= me.stages()) break;
+                const double mc = me.m(st), ec = me.e(st);
+                if (mc == 0.0 && ec == 0.0) continue;
+                const double kv = p.kfinal(st, s, x);
+                if (mc != 0.0) yn = simk_fma(mc, kv, yn);
+                if (ec != 0.0) yerr = simk_fma(ec, kv, yerr);
+            }
+            const double scale = simk_fma(ctl.rtol, simk_max(simk_abs(y.get(s)), simk_abs(yn)), ctl.atol);
+            const double q = yerr / scale;
+            err = simk_fma(q, q, err);
+            ynew.set(s, yn);
+        }
+        err = simk_sqrt(err * ctl.inv_n);
+        if (!simk_is_finite(err)) err = 1e10;
+
+        double fac = simk_step_factor(err, me.order());
+        bool ok = (err <= 1.0) || (hs <= ctl.h_min);
+        if (ctl.fixed_step) {
+            ok = true;
+            fac = 1.0;
+        }
+        if (ok) {
+#pragma unroll
+            for (simk_u32 s = 0; s < n; ++s) {
+                double v = ynew.get(s);
+                if (ctl.clamp_negative) v = simk_max(v, 0.0);
+                y.set(s, v);
+            }
+            t = last ? t_end : t + hs;
+        }
+        const double hfac = hs * fac;
+        double h_acc = hfac;
+        if (reject_last) h_acc = simk_min(h_acc, hs);
+        if (last && fac >= 1.0) h_acc = simk_max(h_acc, h_wanted);
+        double h_rej = reject_more ? hs * 0.1 : simk_min(hfac, hs);
+        if (ctl.fixed_step) {
+            h_acc = h_wanted;
+            h_rej = h_wanted;
+        }
+        h = ok ? h_acc : h_rej;
+        reject_more = !ok && reject_last;
+        reject_last = !ok;
+        steps = steps + (ok ? 1.0 : 0.0);
+        rejects = rejects + (ok ? 0.0 : 1.0);
+        active = t < t_end;
+    }
+}
+
+// kin_advance (kin_solver.h:322-347) for one lane: pulse segments over local
+// time [0, duration], h restarts at h_start after each pulse.
+template <class P, class M>
+SIMK_DEV void simk_advance(P &p, const M &me, const SimkDevCtl &ctl, typename P::Vec &y, const double duration,
+                           double &h, const double period, const double first, const double count, double &steps,
+                           double &rejects, double &pulses, double &failed) {
+    double t = 0.0, applied = 0.0;
+    for (;;) {
+        const bool pending = applied < count;
+        const double next_pulse = simk_fma(applied, period, first);
+        const double seg_end = pending ? simk_min(next_pulse, duration) : duration;
+        simk_integrate(p, me, ctl, y, t, seg_end, h, steps, rejects, failed);
+        const bool alive = !(failed > 0.0);
+        const bool hit = pending && alive && (t >= seg_end);
+        if (hit) {
+            const simk_u32 n = p.n();
+#pragma unroll
+            for (simk_u32 s = 0; s < n; ++s) y.set(s, y.get(s) + p.pulse(s));
+            applied = applied + 1.0;
+            pulses = pulses + 1.0;
+            h = ctl.h_start;
+        }
+        // The reference's exit test; the extra "!pending" form is equivalent for
+        // every finite count and ends the loop where a NaN count never would.
+        const bool done = ((t >= duration) && (applied >= count || !pending)) || !alive;
+        if (done) break;
+    }
+}
+
+// The per-lane body of a launch (dbd_advance_group, src/sim/dbd_lanes.h:28-78):
+// load the cell's state, advance, store state and add the counters.
+template <class P, class M>
+SIMK_DEV void simk_lane_body(const SimkKinArgs &A, const M &me) {
+    const simk_u32 wave = blockIdx.x;
+    const simk_u32 lane = threadIdx.x;
+    const simk_u32 cell = wave * SIMK_WAVE + lane;
+    if (cell >= A.lanes) return;
+    const simk_u32 w0 = wave * SIMK_WAVE;
+    const simk_u32 stride = A.padded * 8u;
+    const simk_u32 tail = (A.padded - w0) * 8u;   // bytes of one row from this wave on
+    SimkRows ys, one;
+    ys.init(A.y + w0, lane, stride, (A.n_species - 1) * stride + tail);
+    P p;
+    p.init(A, wave, lane);
+    typename P::Vec y = p.vec(0);
+    const simk_u32 n = p.n();
+#pragma unroll
+    for (simk_u32 s = 0; s < n; ++s) y.set(s, ys.ld(s));
+    one.init(A.h + w0, lane, 0, tail);
+    double h = one.ld(0);
+    one.init(A.period + w0, lane, 0, tail);
+    const double period = one.ld(0);
+    one.init(A.pulse_first + w0, lane, 0, tail);
+    const double first = one.ld(0);
+    one.init(A.pulse_count + w0, lane, 0, tail);
+    const double count = one.ld(0);
+    SimkRows fs;
+    fs.init(A.failed + w0, lane, 0, tail);
+    double failed = fs.ld(0);
+    double steps = 0.0, rejects = 0.0, pulses = 0.0;
+    simk_advance(p, me, A.ctl, y, A.duration, h, period, first, count, steps, rejects, pulses, failed);
+#pragma unroll
+    for (simk_u32 s = 0; s < n; ++s) ys.st(s, y.get(s));
+    one.init(A.h + w0, lane, 0, tail);
+    one.st(0, h);
+    fs.st(0, failed);
+    one.init(A.steps + w0, lane, 0, tail);
+    one.st(0, one.ld(0) + steps);
+    one.init(A.rejects + w0, lane, 0, tail);
+    one.st(0, one.ld(0) + rejects);
+    one.init(A.pulses + w0, lane, 0, tail);
+    one.st(0, one.ld(0) + pulses);
+}

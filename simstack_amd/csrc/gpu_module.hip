@@ -1,0 +1,765 @@
+// libsimk_gpu.so — the GPU module: C ABI (simk_gpu_api.h), device arena,
+// stream, the table-walk kernel, and the JIT pipeline for generated kernels.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "codegen.hpp"
+#include "device_kin.h"
+#include "mech.hpp"
+#include "simk_gpu_api.h"
+
+#include "device_kin_src.inc"   // kDeviceKinSource: device_kin.h as text, for hipRTC
+
+using namespace simk;
+
+// ---------------------------------------------------------------------------
+// Table-walk policy: any mechanism, tables read uniformly from device memory
+// (scalar loads), every per-lane vector in workspace slots. The behaviour of
+// the reference's KinTableOps (src/sim/kinetics/kin_solver.h:152-161).
+
+struct TableOps {
+    using Vec = SimkWsVec;
+    const SimkDevTables *T;
+    simk_u32 n_, R_, nnz_;
+    SimkRows kr, sr, pr, ws;
+
+    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {
+        T = A.tables;
+        n_ = T->n;
+        R_ = T->R;
+        nnz_ = T->nnz;
+        const simk_u32 w0 = wave * SIMK_WAVE, stride = A.padded * 8u, tail = (A.padded - w0) * 8u;
+        kr.init(A.k + w0, lane, stride, (R_ - 1) * stride + tail);
+        sr.init(A.src + w0, lane, stride, (n_ - 1) * stride + tail);
+        pr.init(A.pulse + w0, lane, stride, (n_ - 1) * stride + tail);
+        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);
+    }
+    // Workspace slots: [0, nnz) LU; vectors i = 0..3 at nnz + i n; K_st at
+    // nnz + (4 + st) n; the reused f at nnz + 8 n.
+    SIMK_DEV simk_u32 n() const { return n_; }
+    SIMK_DEV Vec vec(int i) const { return Vec{ws, nnz_ + (simk_u32)i * n_}; }
+    SIMK_DEV double K(simk_u32 r) const { return kr.ld(r); }
+    SIMK_DEV double S(simk_u32 s) const { return sr.ld(s); }
+    SIMK_DEV double pulse(simk_u32 s) const { return pr.ld(s); }
+    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }
+    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }
+    SIMK_DEV simk_u32 koff(simk_u32 st) const { return nnz_ + (4 + st) * n_; }
+    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(koff(st) + s); }
+    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {
+        for (simk_u32 s = 0; s < n_; ++s) WS(koff(st) + s, x.get(s));
+    }
+    SIMK_DEV double kfinal(simk_u32 st, simk_u32 s, const Vec &) const { return kget(st, s); }
+    SIMK_DEV void fsave(const Vec &x) const {
+        for (simk_u32 s = 0; s < n_; ++s) WS(nnz_ + 8 * n_ + s, x.get(s));
+    }
+    SIMK_DEV void fload(const Vec &x) const {
+        for (simk_u32 s = 0; s < n_; ++s) x.set(s, W(nnz_ + 8 * n_ + s));
+    }
+
+    // kin_fun (kin_solver.h:39-53)
+    SIMK_DEV void fun(const Vec &y, const Vec &f) const {
+        for (simk_u32 s = 0; s < n_; ++s) f.set(s, S(s));
+        for (simk_u32 r = 0; r < R_; ++r) {
+            double rate = K(r);
+            const simk_u32 rc = T->reactant_count[r];
+            for (simk_u32 p = 0; p < rc; ++p) rate = rate * y.get(T->reactant[r * 3 + p]);
+            const simk_u32 nc = T->net_count[r];
+            for (simk_u32 e = 0; e < nc; ++e) {
+                const simk_u32 s = T->net_species[r * 6 + e];
+                f.set(s, simk_fma(T->net_coef[r * 6 + e], rate, f.get(s)));
+            }
+        }
+    }
+    // kin_neg_jac then kin_factor (kin_solver.h:75-113), in place in the workspace.
+    SIMK_DEV void jac_factor(const Vec &y, const double g) const {
+        for (simk_u32 i = 0; i < nnz_; ++i) WS(i, 0.0);
+        for (simk_u32 r = 0; r < R_; ++r) {
+            const simk_u32 rc = T->reactant_count[r], nc = T->net_count[r];
+            for (simk_u32 p = 0; p < rc; ++p) {
+                double d = K(r);
+                for (simk_u32 q = 0; q < rc; ++q)
+                    if (q != p) d = d * y.get(T->reactant[r * 3 + q]);
+                for (simk_u32 e = 0; e < nc; ++e) {
+                    const simk_u32 idx = T->jac_index[(r * 3 + p) * 6 + e];
+                    WS(idx, simk_fma(-T->net_coef[r * 6 + e], d, W(idx)));
+                }
+            }
+        }
+        for (simk_u32 i = 0; i < n_; ++i) {
+            const simk_u32 d0 = T->div_begin[i], d1 = T->div_begin[i + 1];
+            for (simk_u32 d = d0; d < d1; ++d) {
+                const unsigned short *dv = T->lu_div + 4 * d;
+                const double l = W(dv[0]) * W(dv[1]);
+                WS(dv[0], l);
+                for (simk_u32 s = dv[2]; s < dv[3]; ++s) {
+                    const simk_u32 ij = T->lu_sub[2 * s], kj = T->lu_sub[2 * s + 1];
+                    WS(ij, simk_fnma(l, W(kj), W(ij)));
+                }
+            }
+            const simk_u32 ii = T->diag[i];
+            WS(ii, 1.0 / (W(ii) + g));
+        }
+    }
+    // kin_solve (kin_solver.h:122-139)
+    SIMK_DEV void solve(const Vec &x) const {
+        for (simk_u32 i = 0; i < n_; ++i)
+            for (simk_u32 e = T->lower_begin[i]; e < T->lower_begin[i + 1]; ++e)
+                x.set(i, simk_fnma(W(T->lower[2 * e]), x.get(T->lower[2 * e + 1]), x.get(i)));
+        for (simk_u32 ii = n_; ii-- > 0;) {
+            for (simk_u32 e = T->upper_begin[ii]; e < T->upper_begin[ii + 1]; ++e)
+                x.set(ii, simk_fnma(W(T->upper[2 * e]), x.get(T->upper[2 * e + 1]), x.get(ii)));
+            x.set(ii, x.get(ii) * W(T->diag[ii]));
+        }
+    }
+};
+
+struct ValueMethod {   // method copied by value from the kernel argument
+    static constexpr bool kUnrollStages = false;
+    SimkDevMethod mm;
+    SIMK_DEV simk_u32 stages() const { return mm.stages; }
+    SIMK_DEV simk_u32 order() const { return mm.order; }
+    SIMK_DEV double gamma0() const { return mm.gamma0; }
+    SIMK_DEV double a(simk_u32 i) const { return mm.a[i]; }
+    SIMK_DEV double c(simk_u32 i) const { return mm.c[i]; }
+    SIMK_DEV double m(simk_u32 i) const { return mm.m[i]; }
+    SIMK_DEV double e(simk_u32 i) const { return mm.e[i]; }
+    SIMK_DEV bool eval(simk_u32 i) const { return mm.eval[i] != 0; }
+    SIMK_DEV bool save_f(simk_u32 i) const { return mm.save_f[i] != 0; }
+};
+
+extern "C" __global__ void __launch_bounds__(SIMK_WAVE) simk_table_kernel(SimkKinArgs A) {
+    ValueMethod me{A.method};
+    simk_lane_body<TableOps, ValueMethod>(A, me);
+}
+
+// The reference's M1 point demo (a spring/swirl particle step), kept so every
+// reference export exists.
+__global__ void simk_step_points_kernel(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy, f32 *vz, u32 count, f32 dt) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const f32 spring = 4.0f, swirl = 0.6f, damp = 1.0f - 0.02f * dt;
+    const f32 px = x[i], py = y[i], pz = z[i];
+    const f32 ax = -spring * px + swirl * pz, ay = -spring * py, az = -spring * pz - swirl * px;
+    const f32 nvx = __builtin_fmaf(ax, dt, vx[i]) * damp;
+    const f32 nvy = __builtin_fmaf(ay, dt, vy[i]) * damp;
+    const f32 nvz = __builtin_fmaf(az, dt, vz[i]) * damp;
+    vx[i] = nvx;
+    vy[i] = nvy;
+    vz[i] = nvz;
+    x[i] = __builtin_fmaf(nvx, dt, px);
+    y[i] = __builtin_fmaf(nvy, dt, py);
+    z[i] = __builtin_fmaf(nvz, dt, pz);
+}
+
+// ---------------------------------------------------------------------------
+// Module state (one device, one stream; calls come from one thread at a time).
+
+namespace {
+
+struct Arena {
+    u8 *base = nullptr;
+    u64 size = 0, used = 0;
+};
+
+struct JitKernel {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    u32 ws_slots = 0;
+    std::string name;
+    bool cached = false;
+    f64 ms = 0.0;
+};
+
+struct State {
+    bool ready = false;
+    int device = 0;
+    Arena arena;
+    hipStream_t stream = nullptr;
+    hipEvent_t kin_start = nullptr, kin_stop = nullptr, async_begin = nullptr, async_end = nullptr;
+    u32 mode = 0;
+    u32 kernel_kind = 0;   // 0 auto, 1 generated, 2 table
+    GenOptions gen;
+    // current mechanism
+    bool have_mech = false;
+    Mech mech;
+    u64 sig = 0;
+    KinMethod method{};
+    SimkDevCtl ctl{};
+    SimkDevMethod dev_method{};
+    void *d_tables = nullptr;   // SimkDevTables followed by its arrays
+    JitKernel jit;
+    bool jit_ok = false;
+    std::string jit_log;
+    // workspace
+    double *ws = nullptr;
+    u64 ws_bytes = 0;
+    // last call
+    u32 last_kernel = 0;
+    u32 last_slots = 0;
+    f64 last_ms = 0.0;
+};
+
+State g;
+
+s32 err_code(hipError_t e) { return (s32)e; }
+
+std::string module_dir() {
+    Dl_info info;
+    if (dladdr((void *)&gpu_init, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        size_t slash = p.rfind('/');
+        if (slash != std::string::npos) return p.substr(0, slash);
+    }
+    return ".";
+}
+
+std::string cache_dir() {
+    const char *env = getenv("SIMK_KCACHE");
+    if (env && *env) return env;
+    return module_dir() + "/kcache";
+}
+
+const char *kJitOptions[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17",
+                             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+const int kJitOptionCount = sizeof(kJitOptions) / sizeof(kJitOptions[0]);
+
+std::string cache_path(const std::string &src) {
+    u64 h = fnv1a64(src.data(), src.size());
+    for (int i = 0; i < kJitOptionCount; ++i) h = fnv1a64(kJitOptions[i], strlen(kJitOptions[i]), h);
+    int rtc_major = 0, rtc_minor = 0;
+    hiprtcVersion(&rtc_major, &rtc_minor);
+    h = fnv1a64(&rtc_major, sizeof(int), h);
+    h = fnv1a64(&rtc_minor, sizeof(int), h);
+    char name[64];
+    snprintf(name, sizeof(name), "/%016llx.hsaco", (unsigned long long)h);
+    return cache_dir() + name;
+}
+
+bool read_file(const std::string &path, std::vector<char> &out) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    fseek(f, 0, SEEK_END);
+    long size = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    out.resize(size > 0 ? (size_t)size : 0);
+    bool ok = size > 0 && fread(out.data(), 1, (size_t)size, f) == (size_t)size;
+    fclose(f);
+    return ok;
+}
+
+void write_file_atomic(const std::string &path, const std::vector<char> &data) {
+    mkdir(cache_dir().c_str(), 0755);
+    std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    bool ok = fwrite(data.data(), 1, data.size(), f) == data.size();
+    fclose(f);
+    if (ok) rename(tmp.c_str(), path.c_str());
+    else unlink(tmp.c_str());
+}
+
+// Compiles (or loads from the cache) the code object of a generated source.
+bool jit_code(const GenResult &gr, std::vector<char> &code, bool &cached, std::string &log) {
+    std::string path = cache_path(gr.source);
+    if (const char *dump = getenv("SIMK_DUMP_SOURCE")) {   // tooling: keep the generated source
+        FILE *f = fopen(dump, "wb");
+        if (f) {
+            fwrite(gr.source.data(), 1, gr.source.size(), f);
+            fclose(f);
+        }
+    }
+    if (read_file(path, code)) {
+        cached = true;
+        return true;
+    }
+    cached = false;
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, gr.source.c_str(), "simk_gen.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    hiprtcResult rc = hiprtcCompileProgram(prog, kJitOptionCount, kJitOptions);
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    if (log_size > 1) {
+        log.resize(log_size);
+        hiprtcGetProgramLog(prog, &log[0]);
+    }
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t size = 0;
+    hiprtcGetCodeSize(prog, &size);
+    code.resize(size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    write_file_atomic(path, code);
+    return true;
+}
+
+void free_mech_resources() {
+    if (g.d_tables) hipFree(g.d_tables);
+    g.d_tables = nullptr;
+    if (g.jit.mod) hipModuleUnload(g.jit.mod);
+    g.jit = JitKernel();
+    g.jit_ok = false;
+}
+
+template <class T>
+void put(std::vector<u8> &blob, u64 &off, const T *data, u64 count) {
+    off = simk_align_up(off, 16);
+    if (blob.size() < off + sizeof(T) * count) blob.resize(off + sizeof(T) * count);
+    if (count) memcpy(blob.data() + off, data, sizeof(T) * count);
+    off += sizeof(T) * count;
+}
+
+// Flattens the tables for the table-walk kernel into one device allocation.
+hipError_t upload_tables(const Mech &m) {
+    std::vector<u8> rc(m.R), nc(m.R), reac(m.R * 3), ns(m.R * 6);
+    std::vector<f64> coef(m.R * 6, 0.0);
+    std::vector<u16> jac(m.R * 18, 0), div(m.lu_div.size() * 4), sub(m.lu_sub.size() * 2), lo(m.lower.size() * 2),
+        up(m.upper.size() * 2);
+    for (u32 r = 0; r < m.R; ++r) {
+        const Reaction &x = m.rx[r];
+        rc[r] = x.reactant_count;
+        nc[r] = x.net_count;
+        for (int p = 0; p < 3; ++p) reac[r * 3 + p] = x.reactant[p];
+        for (int e = 0; e < 6; ++e) {
+            ns[r * 6 + e] = x.net_species[e];
+            coef[r * 6 + e] = (f64)x.net_coef[e];
+        }
+        for (int p = 0; p < 3; ++p)
+            for (int e = 0; e < 6; ++e) jac[(r * 3 + p) * 6 + e] = x.jac_index[p][e];
+    }
+    for (size_t d = 0; d < m.lu_div.size(); ++d) {
+        div[4 * d] = m.lu_div[d].ik;
+        div[4 * d + 1] = m.lu_div[d].kk;
+        div[4 * d + 2] = m.lu_div[d].sub_begin;
+        div[4 * d + 3] = m.lu_div[d].sub_end;
+    }
+    for (size_t s = 0; s < m.lu_sub.size(); ++s) {
+        sub[2 * s] = m.lu_sub[s].ij;
+        sub[2 * s + 1] = m.lu_sub[s].kj;
+    }
+    for (size_t e = 0; e < m.lower.size(); ++e) {
+        lo[2 * e] = m.lower[e].index;
+        lo[2 * e + 1] = m.lower[e].col;
+    }
+    for (size_t e = 0; e < m.upper.size(); ++e) {
+        up[2 * e] = m.upper[e].index;
+        up[2 * e + 1] = m.upper[e].col;
+    }
+    std::vector<u8> blob(sizeof(SimkDevTables), 0);
+    u64 off = sizeof(SimkDevTables);
+    u64 o_reac, o_rc, o_nc, o_ns, o_coef, o_jac, o_diag, o_db, o_div, o_sub, o_lb, o_ub, o_lo, o_up;
+    put(blob, off, reac.data(), reac.size()); o_reac = off - reac.size();
+    put(blob, off, rc.data(), rc.size()); o_rc = off - rc.size();
+    put(blob, off, nc.data(), nc.size()); o_nc = off - nc.size();
+    put(blob, off, ns.data(), ns.size()); o_ns = off - ns.size();
+    put(blob, off, coef.data(), coef.size()); o_coef = off - 8 * coef.size();
+    put(blob, off, jac.data(), jac.size()); o_jac = off - 2 * jac.size();
+    put(blob, off, m.diag.data(), m.diag.size()); o_diag = off - 2 * m.diag.size();
+    put(blob, off, m.div_begin.data(), m.div_begin.size()); o_db = off - 2 * m.div_begin.size();
+    put(blob, off, div.data(), div.size()); o_div = off - 2 * div.size();
+    put(blob, off, sub.data(), sub.size()); o_sub = off - 2 * sub.size();
+    put(blob, off, m.lower_begin.data(), m.lower_begin.size()); o_lb = off - 2 * m.lower_begin.size();
+    put(blob, off, m.upper_begin.data(), m.upper_begin.size()); o_ub = off - 2 * m.upper_begin.size();
+    put(blob, off, lo.data(), lo.size()); o_lo = off - 2 * lo.size();
+    put(blob, off, up.data(), up.size()); o_up = off - 2 * up.size();
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, blob.size());
+    if (e != hipSuccess) return e;
+    u8 *base = (u8 *)d;
+    SimkDevTables t;
+    memset(&t, 0, sizeof(t));
+    t.n = m.n;
+    t.R = m.R;
+    t.nnz = m.nnz;
+    t.reactant = base + o_reac;
+    t.reactant_count = base + o_rc;
+    t.net_count = base + o_nc;
+    t.net_species = base + o_ns;
+    t.net_coef = (const double *)(base + o_coef);
+    t.jac_index = (const unsigned short *)(base + o_jac);
+    t.diag = (const unsigned short *)(base + o_diag);
+    t.div_begin = (const unsigned short *)(base + o_db);
+    t.lu_div = (const unsigned short *)(base + o_div);
+    t.lu_sub = (const unsigned short *)(base + o_sub);
+    t.lower_begin = (const unsigned short *)(base + o_lb);
+    t.upper_begin = (const unsigned short *)(base + o_ub);
+    t.lower = (const unsigned short *)(base + o_lo);
+    t.upper = (const unsigned short *)(base + o_up);
+    memcpy(blob.data(), &t, sizeof(t));
+    e = hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return e;
+    }
+    g.d_tables = d;
+    return hipSuccess;
+}
+
+void fill_dev_method(const KinMethod &me, SimkDevMethod &dm) {
+    memset(&dm, 0, sizeof(dm));
+    dm.stages = me.stages;
+    dm.order = me.order;
+    for (int i = 0; i < 6; ++i) {
+        dm.a[i] = me.a[i];
+        dm.c[i] = me.c[i];
+    }
+    for (int i = 0; i < 4; ++i) {
+        dm.m[i] = me.m[i];
+        dm.e[i] = me.e[i];
+        dm.new_f[i] = me.new_f[i] ? 1 : 0;
+    }
+    dm.gamma0 = me.gamma0;
+    stage_plan(me, dm.eval, dm.save_f);
+}
+
+hipError_t ensure_ws(u64 bytes) {
+    if (bytes <= g.ws_bytes) return hipSuccess;
+    if (g.ws) {
+        hipError_t e = hipStreamSynchronize(g.stream);
+        if (e != hipSuccess) return e;
+        hipFree(g.ws);
+        g.ws = nullptr;
+        g.ws_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&g.ws, bytes);
+    if (e != hipSuccess) {
+        g.ws = nullptr;
+        return e;
+    }
+    g.ws_bytes = bytes;
+    return hipSuccess;
+}
+
+s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *control) {
+    if (!g.ready) return err_code(hipErrorNotInitialized);
+    if (!method || !control) return err_code(hipErrorInvalidValue);
+    if (method->stages < 1 || method->stages > 4) return err_code(hipErrorInvalidValue);
+    std::string v = validate(m);
+    if (!v.empty()) return err_code(hipErrorInvalidValue);
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return err_code(e);
+    free_mech_resources();
+    g.mech = m;
+    g.sig = signature(m);
+    g.method = *method;
+    g.method.name = nullptr;
+    fill_dev_method(g.method, g.dev_method);
+    g.ctl.rtol = control->rtol;
+    g.ctl.atol = control->atol;
+    g.ctl.h_min = control->h_min;
+    g.ctl.h_max = control->h_max;
+    g.ctl.h_start = control->h_start;
+    g.ctl.inv_n = 1.0 / (f64)m.n;
+    g.ctl.max_steps = control->max_steps;
+    g.ctl.fixed_step = control->fixed_step ? 1 : 0;
+    g.ctl.clamp_negative = control->clamp_negative ? 1 : 0;
+    e = upload_tables(m);
+    if (e != hipSuccess) return err_code(e);
+    g.have_mech = true;
+
+    // Generated straight-line kernel (skipped when the table walk is forced).
+    if (g.kernel_kind != 2) {
+        auto t0 = std::chrono::steady_clock::now();
+        GenResult gr = generate(m, g.method, g.gen, kDeviceKinSource);
+        std::vector<char> code;
+        bool cached = false;
+        if (jit_code(gr, code, cached, g.jit_log)) {
+            hipModule_t mod;
+            if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
+                hipFunction_t fn;
+                if (hipModuleGetFunction(&fn, mod, gr.kernel.c_str()) == hipSuccess) {
+                    g.jit.mod = mod;
+                    g.jit.fn = fn;
+                    g.jit.ws_slots = gr.ws_slots;
+                    g.jit.name = gr.kernel;
+                    g.jit.cached = cached;
+                    g.jit_ok = true;
+                } else {
+                    hipModuleUnload(mod);
+                }
+            }
+        }
+        g.jit.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (!g.jit_ok && g.kernel_kind == 1) return err_code(hipErrorInvalidImage);
+    }
+    return 0;
+}
+
+// Launches one advance of lanes [0, lanes) on the module stream.
+hipError_t launch(const GpuKinLaunch *L) {
+    const u32 lanes = L->lane_count ? L->lane_count : L->padded;
+    if (lanes == 0) return hipSuccess;
+    if (lanes > L->padded) return hipErrorInvalidValue;
+    if (!L->y || !L->h || !L->steps || !L->rejects || !L->pulses || !L->failed || !L->k || !L->src || !L->pulse ||
+        !L->period || !L->pulse_first || !L->pulse_count)
+        return hipErrorInvalidValue;
+    if (L->precision != 0) return hipErrorNotSupported;
+    if (g.mode != 0) return hipErrorNotSupported;
+    const bool use_gen = g.jit_ok && g.kernel_kind != 2;
+    const u32 waves = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
+    const u32 slots = use_gen ? g.jit.ws_slots : g.mech.nnz + 9 * g.mech.n;
+    // Buffer resources address each [row][padded] array with 32-bit offsets.
+    const u64 rows = g.mech.R > g.mech.n ? g.mech.R : g.mech.n;
+    if (rows * L->padded * 8ull >= (1ull << 31) || (u64)slots * SIMK_WAVE * 8ull >= (1ull << 31))
+        return hipErrorInvalidValue;
+    hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(double));
+    if (e != hipSuccess) return e;
+    SimkKinArgs A;
+    memset(&A, 0, sizeof(A));
+    A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
+    A.k = L->k; A.src = L->src; A.pulse = L->pulse; A.period = L->period; A.pulse_first = L->pulse_first;
+    A.pulse_count = L->pulse_count;
+    A.padded = L->padded;
+    A.lanes = lanes;
+    A.duration = L->duration;
+    A.ws = g.ws;
+    A.ws_slots = slots;
+    A.n_species = g.mech.n;
+    A.n_reactions = g.mech.R;
+    A.tables = (const SimkDevTables *)g.d_tables;
+    A.ctl = g.ctl;
+    A.method = g.dev_method;
+    g.last_kernel = use_gen ? 1 : 2;
+    g.last_slots = slots;
+    if (use_gen) {
+        size_t size = sizeof(A);
+        void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+        return hipModuleLaunchKernel(g.jit.fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+    }
+    hipLaunchKernelGGL(simk_table_kernel, dim3(waves), dim3(SIMK_WAVE), 0, g.stream, A);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exports
+
+extern "C" {
+
+GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info) {
+    if (info) memset(info, 0, sizeof(*info));
+    hipError_t e = hipSetDevice((int)device_index);
+    if (e != hipSuccess) return err_code(e);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, (int)device_index);
+    if (e != hipSuccess) return err_code(e);
+    size_t free_b = 0, total_b = 0;
+    hipMemGetInfo(&free_b, &total_b);
+    if (info) {
+        snprintf(info->name, sizeof(info->name), "%s (%s)", prop.name, prop.gcnArchName);
+        info->sm_major = (u32)prop.major;
+        info->sm_minor = (u32)prop.minor;
+        info->multiprocessors = (u32)prop.multiProcessorCount;
+        info->free_bytes = free_b;
+        info->total_bytes = total_b;
+        info->arena_bytes = arena_bytes;
+    }
+    if (g.ready) {   // re-init: keep the stream and events, replace the arena
+        hipStreamSynchronize(g.stream);
+        if (g.arena.base) hipFree(g.arena.base);
+        g.arena = Arena();
+    } else {
+        e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return err_code(e);
+        hipEventCreate(&g.kin_start);
+        hipEventCreate(&g.kin_stop);
+        hipEventCreate(&g.async_begin);
+        hipEventCreate(&g.async_end);
+    }
+    void *base = nullptr;
+    if (arena_bytes) {
+        e = hipMalloc(&base, arena_bytes);
+        if (e != hipSuccess) return err_code(e);
+    }
+    g.arena.base = (u8 *)base;
+    g.arena.size = arena_bytes;
+    g.arena.used = 0;
+    g.device = (int)device_index;
+    g.ready = true;
+    return 0;
+}
+
+GPU_API const char *gpu_error_string(s32 error) { return hipGetErrorString((hipError_t)error); }
+
+GPU_API void *gpu_alloc(u64 size) {
+    u64 aligned = simk_align_up(size, 256);
+    if (!g.arena.base || g.arena.used + aligned > g.arena.size) return nullptr;
+    void *p = g.arena.base + g.arena.used;
+    g.arena.used += aligned;
+    return p;
+}
+
+GPU_API u64 gpu_arena_used(void) { return g.arena.used; }
+
+GPU_API void gpu_arena_reset(void) {
+    if (g.ready) hipDeviceSynchronize();
+    g.arena.used = 0;
+}
+
+GPU_API s32 gpu_copy_to_device(void *dst_device, const void *src_host, u64 size) {
+    hipError_t e = hipMemcpyAsync(dst_device, src_host, size, hipMemcpyHostToDevice, g.stream);
+    if (e != hipSuccess) return err_code(e);
+    return err_code(hipStreamSynchronize(g.stream));
+}
+
+GPU_API s32 gpu_copy_to_host(void *dst_host, const void *src_device, u64 size) {
+    hipError_t e = hipMemcpyAsync(dst_host, src_device, size, hipMemcpyDeviceToHost, g.stream);
+    if (e != hipSuccess) return err_code(e);
+    return err_code(hipStreamSynchronize(g.stream));
+}
+
+GPU_API s32 gpu_sync(void) { return err_code(hipStreamSynchronize(g.stream)); }
+
+GPU_API s32 gpu_step_points(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy, f32 *vz, u32 count, f32 dt) {
+    if (count == 0) return 0;
+    const u32 block = 256, grid = (count + block - 1) / block;
+    hipLaunchKernelGGL(simk_step_points_kernel, dim3(grid), dim3(block), 0, g.stream, x, y, z, vx, vy, vz, count, dt);
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_kin_set_mech_caps(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                   const KinControl *control) {
+    if (!mech || !caps) return err_code(hipErrorInvalidValue);
+    Mech m;
+    if (!from_bytes(mech, *caps, m).empty()) return err_code(hipErrorInvalidValue);
+    return set_mech_impl(m, method, control);
+}
+
+GPU_API s32 gpu_kin_set_mech(const KinMech *mech, const KinMethod *method, const KinControl *control) {
+    SimkCaps c = {SIMK_DEFAULT_SPECIES, SIMK_DEFAULT_REACTIONS, SIMK_DEFAULT_NNZ, SIMK_DEFAULT_LU_DIV,
+                  SIMK_DEFAULT_LU_SUB};
+    return simk_kin_set_mech_caps(mech, &c, method, control);
+}
+
+GPU_API s32 gpu_kin_advance(const GpuKinLaunch *L, f32 *kernel_ms) {
+    if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
+    hipEventRecord(g.kin_start, g.stream);
+    hipError_t e = launch(L);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_stop, g.stream);
+    e = hipEventSynchronize(g.kin_stop);
+    if (e != hipSuccess) return err_code(e);
+    f32 ms = 0.0f;
+    hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    g.last_ms = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 gpu_host_register(void *host, u64 size) { return (host && size) ? 0 : err_code(hipErrorInvalidValue); }
+
+GPU_API s32 gpu_host_unregister(void *host) {
+    hipStreamSynchronize(g.stream);
+    return host ? 0 : err_code(hipErrorInvalidValue);
+}
+
+GPU_API s32 gpu_kin_advance_async(const GpuKinLaunch *L, void *host_block, void *device_block, u64 rows,
+                                  u64 mutable_rows) {
+    if (!g.have_mech || !L || !host_block || !device_block || mutable_rows > rows) return err_code(hipErrorInvalidValue);
+    const u64 pitch = (u64)L->padded * sizeof(f64);
+    const u64 width = (u64)(L->lane_count ? L->lane_count : L->padded) * sizeof(f64);
+    hipEventRecord(g.async_begin, g.stream);
+    hipError_t e = hipMemcpy2DAsync(device_block, pitch, host_block, pitch, width, rows, hipMemcpyHostToDevice, g.stream);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_start, g.stream);
+    e = launch(L);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_stop, g.stream);
+    e = hipMemcpy2DAsync(host_block, pitch, device_block, pitch, width, mutable_rows, hipMemcpyDeviceToHost, g.stream);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.async_end, g.stream);
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 gpu_kin_wait(f32 *kernel_ms, f32 *total_ms) {
+    hipError_t e = hipEventSynchronize(g.async_end);
+    if (e != hipSuccess) return err_code(e);
+    f32 k = 0.0f, t = 0.0f;
+    hipEventElapsedTime(&k, g.kin_start, g.kin_stop);
+    hipEventElapsedTime(&t, g.async_begin, g.async_end);
+    g.last_ms = k;
+    if (kernel_ms) *kernel_ms = k;
+    if (total_ms) *total_ms = t;
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 gpu_kin_set_mode(u32 mode) {
+    if (mode > 1) return err_code(hipErrorInvalidValue);
+    g.mode = mode;
+    return 0;
+}
+
+GPU_API s32 simk_kin_set_kernel(u32 kind) {
+    if (kind > 2) return err_code(hipErrorInvalidValue);
+    g.kernel_kind = kind;
+    if (kind == 1 && g.have_mech && !g.jit_ok) return err_code(hipErrorInvalidImage);
+    return 0;
+}
+
+GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
+    if (!o) return err_code(hipErrorInvalidValue);
+    g.gen.fence_rows = o->fence_rows;
+    g.gen.waves_per_eu = o->waves_per_eu;
+    g.gen.unroll_stages = o->unroll_stages;
+    return 0;
+}
+
+GPU_API s32 simk_kin_info(SimkKinInfo *out) {
+    if (!out) return err_code(hipErrorInvalidValue);
+    memset(out, 0, sizeof(*out));
+    out->kernel = g.last_kernel;
+    out->ws_slots = g.last_slots;
+    out->species = g.mech.n;
+    out->reactions = g.mech.R;
+    out->nnz = g.mech.nnz;
+    out->lu_div = (u32)g.mech.lu_div.size();
+    out->lu_sub = (u32)g.mech.lu_sub.size();
+    out->jit_cached = g.jit.cached ? 1 : 0;
+    out->jit_ms = g.jit.ms;
+    out->last_kernel_ms = g.last_ms;
+    out->signature = g.sig;
+    snprintf(out->kernel_name, sizeof(out->kernel_name), "%s",
+             g.last_kernel == 2 ? "simk_table_kernel" : g.jit.name.c_str());
+    return 0;
+}
+
+GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                const SimkGenOptions *options, char *log, u32 log_size) {
+    if (!mech || !caps || !method) return err_code(hipErrorInvalidValue);
+    Mech m;
+    if (!from_bytes(mech, *caps, m).empty()) return err_code(hipErrorInvalidValue);
+    GenOptions o;
+    if (options) {
+        o.fence_rows = options->fence_rows;
+        o.waves_per_eu = options->waves_per_eu;
+        o.unroll_stages = options->unroll_stages;
+    }
+    KinMethod me = *method;
+    me.name = nullptr;
+    GenResult gr = generate(m, me, o, kDeviceKinSource);
+    std::vector<char> code;
+    bool cached = false;
+    std::string text;
+    bool ok = jit_code(gr, code, cached, text);
+    if (log && log_size) snprintf(log, log_size, "%s", text.c_str());
+    return ok ? 0 : err_code(hipErrorInvalidImage);
+}
+
+}  // extern "C"

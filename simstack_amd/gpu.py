@@ -1,0 +1,168 @@
+"""The GPU module (libsimk_gpu.so) from Python: the reference's GPU API
+(src/gpu/gpu_api.h) plus the extensions in include/simk_gpu_api.h.
+
+There is deliberately no CPU fallback: GpuModule() raises if the library is
+not built or no device is present, and every solver call goes to the device.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _ffi
+
+V = C.c_void_p
+
+
+class SimkGenOptions(C.Structure):
+    _fields_ = [("fence_rows", C.c_uint32), ("waves_per_eu", C.c_uint32), ("unroll_stages", C.c_uint32),
+                ("pad_", C.c_uint32)]
+
+
+class SimkKinInfo(C.Structure):
+    _fields_ = [("kernel", C.c_uint32), ("ws_slots", C.c_uint32), ("species", C.c_uint32),
+                ("reactions", C.c_uint32), ("nnz", C.c_uint32), ("lu_div", C.c_uint32), ("lu_sub", C.c_uint32),
+                ("jit_cached", C.c_uint32), ("jit_ms", C.c_double), ("last_kernel_ms", C.c_double),
+                ("signature", C.c_uint64), ("kernel_name", C.c_char * 64)]
+
+
+# Every export the reference's loader resolves (src/gpu/gpu_api.h:112-115).
+REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_used", "gpu_arena_reset",
+                     "gpu_copy_to_device", "gpu_copy_to_host", "gpu_sync", "gpu_step_points", "gpu_kin_set_mech",
+                     "gpu_kin_advance", "gpu_host_register", "gpu_host_unregister", "gpu_kin_advance_async",
+                     "gpu_kin_wait", "gpu_kin_set_mode")
+EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
+                     "simk_jit_precompile")
+
+
+def _sig(lib, name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+
+
+def load_library(path=_ffi.GPU_LIB):
+    """dlopen + typed symbols, without touching a device (safe on a CPU host)."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    _sig(lib, "gpu_init", C.c_int32, C.c_uint32, C.c_uint64, P(_ffi.GpuInfo))
+    _sig(lib, "gpu_error_string", C.c_char_p, C.c_int32)
+    _sig(lib, "gpu_alloc", V, C.c_uint64)
+    _sig(lib, "gpu_arena_used", C.c_uint64)
+    _sig(lib, "gpu_arena_reset", None)
+    _sig(lib, "gpu_copy_to_device", C.c_int32, V, V, C.c_uint64)
+    _sig(lib, "gpu_copy_to_host", C.c_int32, V, V, C.c_uint64)
+    _sig(lib, "gpu_sync", C.c_int32)
+    _sig(lib, "gpu_step_points", C.c_int32, V, V, V, V, V, V, C.c_uint32, C.c_float)
+    _sig(lib, "gpu_kin_set_mech", C.c_int32, V, P(_ffi.KinMethod), P(_ffi.KinControl))
+    _sig(lib, "gpu_kin_advance", C.c_int32, P(_ffi.GpuKinLaunch), P(C.c_float))
+    _sig(lib, "gpu_host_register", C.c_int32, V, C.c_uint64)
+    _sig(lib, "gpu_host_unregister", C.c_int32, V)
+    _sig(lib, "gpu_kin_advance_async", C.c_int32, P(_ffi.GpuKinLaunch), V, V, C.c_uint64, C.c_uint64)
+    _sig(lib, "gpu_kin_wait", C.c_int32, P(C.c_float), P(C.c_float))
+    _sig(lib, "gpu_kin_set_mode", C.c_int32, C.c_uint32)
+    _sig(lib, "simk_kin_set_mech_caps", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl))
+    _sig(lib, "simk_kin_set_kernel", C.c_int32, C.c_uint32)
+    _sig(lib, "simk_kin_set_gen_options", C.c_int32, P(SimkGenOptions))
+    _sig(lib, "simk_kin_info", C.c_int32, P(SimkKinInfo))
+    _sig(lib, "simk_jit_precompile", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(SimkGenOptions),
+         C.c_char_p, C.c_uint32)
+    return lib
+
+
+def precompile(mech, meth, options=None, lib=None):
+    """Generates and compiles a mechanism's kernel into the on-disk cache (no device needed)."""
+    lib = lib or load_library()
+    log = C.create_string_buffer(1 << 16)
+    opt = options or SimkGenOptions(1, 0, 0, 0)
+    rc = lib.simk_jit_precompile(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(opt), log,
+                                 1 << 16)
+    if rc != 0:
+        raise RuntimeError(f"kernel compile failed ({rc}):\n{log.value.decode(errors='replace')[-4000:]}")
+    return log.value.decode(errors="replace")
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+class GpuModule:
+    """One initialised GPU module: arena, stream, current mechanism."""
+
+    def __init__(self, arena_bytes=1 << 30, device=0, path=_ffi.GPU_LIB):
+        self.lib = load_library(path)
+        self.info = _ffi.GpuInfo()
+        self._check(self.lib.gpu_init(device, arena_bytes, C.byref(self.info)), "gpu_init")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise GpuError(f"{what}: {self.lib.gpu_error_string(rc).decode()} ({rc})")
+
+    def set_mech(self, mech, meth, ctl):
+        self._check(self.lib.simk_kin_set_mech_caps(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth),
+                                                    C.byref(ctl)), "simk_kin_set_mech_caps")
+
+    def set_kernel(self, kind):
+        self._check(self.lib.simk_kin_set_kernel(kind), "simk_kin_set_kernel")
+
+    def set_gen_options(self, fence_rows=1, waves_per_eu=0):
+        self._check(self.lib.simk_kin_set_gen_options(C.byref(SimkGenOptions(fence_rows, waves_per_eu, 0, 0))),
+                    "simk_kin_set_gen_options")
+
+    def kin_info(self):
+        info = SimkKinInfo()
+        self._check(self.lib.simk_kin_info(C.byref(info)), "simk_kin_info")
+        return info
+
+    def alloc(self, nbytes):
+        p = self.lib.gpu_alloc(nbytes)
+        if not p:
+            raise GpuError(f"gpu_alloc({nbytes}): arena full")
+        return p
+
+    def arena_reset(self):
+        self.lib.gpu_arena_reset()
+
+    def to_device(self, dev, host_array):
+        a = np.ascontiguousarray(host_array)
+        self._check(self.lib.gpu_copy_to_device(dev, a.ctypes.data, a.nbytes), "gpu_copy_to_device")
+
+    def to_host(self, host_array, dev, nbytes=None):
+        self._check(self.lib.gpu_copy_to_host(host_array.ctypes.data, dev, nbytes or host_array.nbytes),
+                    "gpu_copy_to_host")
+
+
+class DeviceZones:
+    """A zone block mirrored in the module's arena (dbd_advance_gpu's DbdGpu,
+    src/sim/dbd.cpp:263-295): device pointer = device base + host row offset."""
+
+    def __init__(self, gm, zones):
+        self.gm, self.z = gm, zones
+        self.d_block = gm.alloc(zones.block.nbytes)
+
+    def upload(self):
+        self.gm.to_device(self.d_block, self.z.block)
+
+    def download(self, mutable_only=True):
+        n = self.z.mutable * 8 if mutable_only else self.z.block.nbytes
+        self.gm.to_host(self.z.block, self.d_block, n)
+
+    def launch_desc(self, duration, lane_count=0, precision=0):
+        L = _ffi.GpuKinLaunch()
+        for name, off in zip(("y", "h", "steps", "rejects", "pulses", "failed", "k", "src", "pulse", "period",
+                              "pulse_first", "pulse_count"), self.z.offsets):
+            setattr(L, name, self.d_block + off * 8)
+        L.padded = self.z.padded
+        L.lane_count = lane_count or self.z.count
+        L.precision = precision
+        L.duration = duration
+        return L
+
+    def advance(self, duration, lane_count=0, precision=0):
+        """gpu_kin_advance on the device block; returns kernel milliseconds."""
+        L = self.launch_desc(duration, lane_count, precision)
+        ms = C.c_float(0.0)
+        self.gm._check(self.gm.lib.gpu_kin_advance(C.byref(L), C.byref(ms)), "gpu_kin_advance")
+        return ms.value

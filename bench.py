@@ -63,6 +63,12 @@ def main():
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generated, 2 table walk")
+    ap.add_argument("--separate", action="store_true",
+                    help="one gpu_kin_advance launch per step instead of one fused simk_kin_advance_n launch")
+    ap.add_argument("--grid", type=int, default=0, help="persistent waves (0: resident waves)")
+    ap.add_argument("--reg-window", type=int, default=0, help="generator register window (0: default)")
+    ap.add_argument("--waves-per-eu", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (cells shard with no "
@@ -78,13 +84,15 @@ def main():
 
     gm = gpu.GpuModule(arena_bytes=z0.block.nbytes + (64 << 20))
     gm.set_kernel(args.kernel)
+    gm.set_gen_options(reg_window=args.reg_window, waves_per_eu=args.waves_per_eu, batch=args.batch)
+    gm.set_grid(args.grid)
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z0.copy())
 
     # Warmup: advance from the initial state, then restore it on the device.
     dz.upload()
-    for _ in range(args.warmup):
-        dz.advance(1.0)
+    if args.warmup:
+        dz.advance(1.0, repeats=args.warmup)
     dz.z.block[:] = z0.block
     dz.upload()
 
@@ -93,8 +101,11 @@ def main():
     kernel_ms = []
     gm.lib.gpu_sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kernel_ms.append(dz.advance(1.0))
+    if args.separate:
+        for _ in range(args.steps):
+            kernel_ms.append(dz.advance(1.0))
+    else:
+        kernel_ms.append(dz.advance(1.0, repeats=args.steps))
     gm.lib.gpu_sync()
     wall = time.perf_counter() - t0
     dz.download()
@@ -105,7 +116,7 @@ def main():
     info = gm.kin_info()
 
     avg_ms = float(np.mean(kernel_ms))
-    flops_per_launch = att * flops_attempt / args.steps
+    flops_per_launch = att * flops_attempt / len(kernel_ms)
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
@@ -128,7 +139,8 @@ def main():
         "config": {"workload": "48 species / 150 reactions, 65,536 cells, 10 x 1 s operator-split, Rodas3, "
                                "rtol 1e-6 atol 1e-12", "cells": args.cells, "species": mech.n,
                    "reactions": mech.R, "nnz": mech.nnz, "lu_div": mech.lu_div, "lu_sub": mech.lu_sub,
-                   "kernel": info.kernel_name.decode(), "parallelism": "1 GPU, one cell per lane"},
+                   "kernel": info.kernel_name.decode(), "waves": info.waves, "launches": len(kernel_ms),
+                   "parallelism": "1 GPU, one cell per lane, persistent lane-refilling kernel"},
         "attempts_per_s": att / wall,
         "accepted_steps": acc,
         "failed_cells": failed,

@@ -75,16 +75,32 @@ GPU_API s32 gpu_kin_set_mode(u32 mode);
 GPU_API s32 simk_kin_set_mech_caps(const void *mech, const SimkCaps *caps, const KinMethod *method,
                                    const KinControl *control);
 
+/* `repeats` consecutive advances of the launch's duration in ONE persistent
+ * launch, carrying h — the result of `repeats` gpu_kin_advance calls with the
+ * block's inputs unchanged between them (for operator-split steps with no
+ * host work in between). */
+GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *launch, u32 repeats, f32 *kernel_ms);
+
+/* Persistent grid size in waves (0: as many as are resident on the device). */
+GPU_API s32 simk_kin_set_grid(u32 waves);
+
+/* Queue order (device array of lane_count cell indices, a permutation) in
+ * which lanes take cells; null restores the natural order. Results per cell
+ * do not depend on it. The array must stay valid while advances run. */
+GPU_API s32 simk_kin_set_order(const u32 *order_device);
+
 /* Kernel used by the advances: 0 auto (generated when it compiled, else
  * table), 1 generated straight-line, 2 table walk. */
 GPU_API s32 simk_kin_set_kernel(u32 kind);
 
 /* Generator knobs (speed only; results unchanged): see SimkGenOptions. */
 typedef struct SimkGenOptions {
-    u32 fence_rows;
-    u32 waves_per_eu;
-    u32 unroll_stages;
-    u32 pad_;
+    u32 fence_rows;      /* 1: factored rows re-read from the workspace; 0: kept in registers */
+    u32 waves_per_eu;    /* amdgpu_waves_per_eu hint (0: none) */
+    u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
+    u32 reg_window;      /* rows back a finished row stays in registers (0: 8, ~0u: all) */
+    u32 batch;           /* loads issued this many statements ahead (0: 16) */
+    u32 reserved[3];
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
@@ -92,6 +108,8 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 typedef struct SimkKinInfo {
     u32 kernel;          /* 1 generated, 2 table */
     u32 ws_slots;        /* workspace f64 slots per lane */
+    u32 waves;           /* persistent waves of the last launch */
+    u32 pad_;
     u32 species, reactions, nnz, lu_div, lu_sub;
     u32 jit_cached;      /* 1: the generated kernel came from the disk cache */
     f64 jit_ms;          /* generate + compile (or cache load) time */
@@ -106,6 +124,11 @@ GPU_API s32 simk_kin_info(SimkKinInfo *out);
  * compile log is written to log (may be null). */
 GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const KinMethod *method,
                                 const SimkGenOptions *options, char *log, u32 log_size);
+
+/* Measurement aid: device-to-device copy of `bytes` (16-byte multiple and
+ * alignment) by a plain streaming kernel, timed with stream events. HBM
+ * byte counters of this kernel calibrate the solver kernels' counters. */
+GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms);
 
 #ifdef __cplusplus
 }

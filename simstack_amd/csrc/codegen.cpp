@@ -42,6 +42,50 @@ std::string lit(f64 v) {
     return buf;
 }
 
+// One generated statement that may consume a loaded value. The load is
+// emitted ahead of the statement by the pipeline below so that many loads are
+// in flight before the first use (a wave otherwise waits for each load).
+struct Op {
+    std::string load;      // "" when the statement needs no load
+    std::string stmt;      // "%s" stands for the loaded value
+    bool barrier = false;  // loads never move across it (fences, stores they depend on)
+};
+
+// Emits ops with their loads software-pipelined `batch` ops ahead: the loads
+// of ops [b + batch, b + 2 batch) are issued before the statements of
+// [b, b + batch) run. Every loaded value is its own SSA name.
+void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent, u32 &uid) {
+    if (batch == 0) batch = 1;
+    size_t seg = 0;
+    while (seg < ops.size()) {
+        size_t end = seg;
+        while (end < ops.size() && !ops[end].barrier) ++end;
+        // ops [seg, end) form one pipelined segment
+        std::vector<std::string> name(end - seg);
+        size_t issued = seg;
+        auto issue_until = [&](size_t limit) {
+            for (; issued < limit && issued < end; ++issued) {
+                if (ops[issued].load.empty()) continue;
+                char buf[32];
+                snprintf(buf, sizeof(buf), "q%u", uid++);
+                name[issued - seg] = buf;
+                w("%sconst double %s = %s;\n", indent, buf, ops[issued].load.c_str());
+            }
+        };
+        issue_until(seg + 2 * batch);
+        for (size_t i = seg; i < end; ++i) {
+            if ((i - seg) % batch == 0) issue_until(i + 2 * batch);
+            const Op &o = ops[i];
+            std::string st = o.stmt;
+            size_t at = st.find("%s");
+            if (at != std::string::npos) st.replace(at, 2, name[i - seg]);
+            w("%s%s\n", indent, st.c_str());
+        }
+        if (end < ops.size()) w("%s%s\n", indent, ops[end].stmt.c_str());   // the barrier itself
+        seg = end + 1;
+    }
+}
+
 }  // namespace
 
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
@@ -90,12 +134,13 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
     w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
-    w("        const simk_u32 w0 = wave * SIMK_WAVE, stride = A.padded * 8u, tail = (A.padded - w0) * 8u;\n");
-    w("        kr.init(A.k + w0, lane, stride, (R - 1) * stride + tail);\n");
-    w("        sr.init(A.src + w0, lane, stride, (N - 1) * stride + tail);\n");
-    w("        pr.init(A.pulse + w0, lane, stride, (N - 1) * stride + tail);\n");
-    w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    w("        const simk_u32 stride = A.padded * 8u;\n");
+    w("        kr.init(A.k, 0, stride, R * stride);\n");
+    w("        sr.init(A.src, 0, stride, N * stride);\n");
+    w("        pr.init(A.pulse, 0, stride, N * stride);\n");
+    w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
     w("    }\n");
+    w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) { kr.voff = sr.voff = pr.voff = cell * 8u; }\n");
     w("    static SIMK_DEV constexpr simk_u32 n() { return N; }\n");
     w("    SIMK_DEV Vec vec(int) const { return Vec(); }\n");
     w("    SIMK_DEV double K(simk_u32 r) const { return kr.ld(r); }\n");
@@ -115,75 +160,202 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
 
     // fun
-    w("    SIMK_DEV void fun(const Vec &y, Vec &f) const {\n");
-    for (u32 s = 0; s < n; ++s) w("        f.v[%u] = S(%u);\n", s, s);
-    for (u32 r = 0; r < m.R; ++r) {
-        const Reaction &x = m.rx[r];
-        w("        {\n            const double rate = K(%u)", r);
-        for (u32 p = 0; p < x.reactant_count; ++p) w(" * y.v[%u]", x.reactant[p]);
-        w(";\n");
-        for (u32 e = 0; e < x.net_count; ++e)
-            w("            f.v[%u] = simk_fma(%d.0, rate, f.v[%u]);\n", x.net_species[e], (int)x.net_coef[e],
-              x.net_species[e]);
-        w("        }\n");
-    }
-    w("    }\n");
-
-    // jac_factor: rows in order.
-    w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
-    for (u32 i = 0; i < n; ++i) {
-        std::vector<u32> row;   // compact indices of row i
-        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) row.push_back(m.lower[e].index);
-        row.push_back(m.diag[i]);
-        for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e) row.push_back(m.upper[e].index);
-        w("        { // row %u\n", i);
-        for (u32 idx : row) w("            double a%u = 0.0;\n", idx);
-        // -J contributions, per entry in (r, p, e) order.
+    {
+        std::vector<Op> ops;
+        char buf[96];
+        for (u32 s = 0; s < n; ++s) {
+            Op op;
+            snprintf(buf, sizeof(buf), "S(%u)", s);
+            op.load = buf;
+            snprintf(buf, sizeof(buf), "f.v[%u] = %%s;", s);
+            op.stmt = buf;
+            ops.push_back(op);
+        }
         for (u32 r = 0; r < m.R; ++r) {
             const Reaction &x = m.rx[r];
+            Op op;
+            snprintf(buf, sizeof(buf), "K(%u)", r);
+            op.load = buf;
+            std::string st = "{ const double rate = %s";
             for (u32 p = 0; p < x.reactant_count; ++p) {
-                bool any = false;
-                for (u32 e = 0; e < x.net_count; ++e)
-                    if (x.net_species[e] == i) any = true;
-                if (!any) continue;
-                w("            { const double d = K(%u)", r);
-                for (u32 q = 0; q < x.reactant_count; ++q)
-                    if (q != p) w(" * y.v[%u]", x.reactant[q]);
-                w(";");
-                for (u32 e = 0; e < x.net_count; ++e) {
-                    if (x.net_species[e] != i) continue;
-                    u32 idx = x.jac_index[p][e];
-                    w(" a%u = simk_fma(%d.0, d, a%u);", idx, -(int)x.net_coef[e], idx);
+                snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[p]);
+                st += buf;
+            }
+            st += ";";
+            for (u32 e = 0; e < x.net_count; ++e) {
+                snprintf(buf, sizeof(buf), " f.v[%u] = simk_fma(%d.0, rate, f.v[%u]);", x.net_species[e],
+                         (int)x.net_coef[e], x.net_species[e]);
+                st += buf;
+            }
+            st += " }";
+            op.stmt = st;
+            ops.push_back(op);
+        }
+        w("    SIMK_DEV void fun(const Vec &y, Vec &f) const {\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid);
+        w("    }\n");
+    }
+
+    // jac_factor: rows in order. Register mode (fence_rows = 0): every entry is
+    // one SSA value, later rows read finished rows' values directly (no reload),
+    // and each row is stored as soon as it is final. Fenced mode re-reads
+    // finished rows from the workspace.
+    const bool regs = !o.fence_rows;
+    // Row of every compact index (to decide register use vs reload).
+    std::vector<u32> row_of(m.nnz, 0);
+    for (u32 i = 0; i < n; ++i) {
+        row_of[m.diag[i]] = i;
+        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) row_of[m.lower[e].index] = i;
+        for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e) row_of[m.upper[e].index] = i;
+    }
+    // In register mode a finished entry is read from its register when its
+    // row is at most reg_window rows back (0: always), else reloaded from the
+    // workspace after a fence, which bounds live ranges on dense mechanisms.
+    auto near = [&](u32 idx, u32 i) { return o.reg_window == 0 || i - row_of[idx] <= o.reg_window; };
+    auto fin = [&](u32 idx, u32 i) {
+        char buf[24];
+        snprintf(buf, sizeof(buf), (regs && near(idx, i)) ? "a%u" : "W(%u)", idx);
+        return std::string(buf);
+    };
+    {
+        std::vector<Op> ops;
+        auto stmt = [&](const std::string &t) {
+            Op op;
+            op.stmt = t;
+            ops.push_back(op);
+        };
+        auto barrier = [&]() {
+            Op op;
+            op.stmt = "SIMK_MEMORY_FENCE();";
+            op.barrier = true;
+            ops.push_back(op);
+        };
+        char buf[160];
+        for (u32 i = 0; i < n; ++i) {
+            std::vector<u32> row;   // compact indices of row i
+            for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) row.push_back(m.lower[e].index);
+            row.push_back(m.diag[i]);
+            for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e) row.push_back(m.upper[e].index);
+            bool far = false;
+            for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
+                if (!near(m.lu_div[d].kk, i)) far = true;
+                for (u32 s = m.lu_div[d].sub_begin; s < m.lu_div[d].sub_end; ++s)
+                    if (!near(m.lu_sub[s].kj, i)) far = true;
+            }
+            if (!regs || far) barrier();
+            snprintf(buf, sizeof(buf), "// row %u", i);
+            stmt(buf);
+            for (u32 idx : row) {
+                snprintf(buf, sizeof(buf), "double a%u = 0.0;", idx);
+                stmt(buf);
+            }
+            // -J contributions, per entry in (r, p, e) order.
+            for (u32 r = 0; r < m.R; ++r) {
+                const Reaction &x = m.rx[r];
+                for (u32 p = 0; p < x.reactant_count; ++p) {
+                    bool any = false;
+                    for (u32 e = 0; e < x.net_count; ++e)
+                        if (x.net_species[e] == i) any = true;
+                    if (!any) continue;
+                    Op op;
+                    snprintf(buf, sizeof(buf), "K(%u)", r);
+                    op.load = buf;
+                    std::string st = "{ const double d = %s";
+                    for (u32 q = 0; q < x.reactant_count; ++q)
+                        if (q != p) {
+                            snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[q]);
+                            st += buf;
+                        }
+                    st += ";";
+                    for (u32 e = 0; e < x.net_count; ++e) {
+                        if (x.net_species[e] != i) continue;
+                        u32 idx = x.jac_index[p][e];
+                        snprintf(buf, sizeof(buf), " a%u = simk_fma(%d.0, d, a%u);", idx, -(int)x.net_coef[e], idx);
+                        st += buf;
+                    }
+                    st += " }";
+                    op.stmt = st;
+                    ops.push_back(op);
                 }
-                w(" }\n");
+            }
+            // Eliminations against finished rows k < i: register values, or
+            // workspace reloads (fenced mode / beyond the register window).
+            for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
+                const KinLuDiv &dv = m.lu_div[d];
+                const bool kk_reg = regs && near(dv.kk, i);
+                Op op;
+                if (!kk_reg) {
+                    snprintf(buf, sizeof(buf), "W(%u)", dv.kk);
+                    op.load = buf;
+                    snprintf(buf, sizeof(buf), "const double l%u = a%u * %%s; a%u = l%u;", d, dv.ik, dv.ik, d);
+                } else {
+                    snprintf(buf, sizeof(buf), "const double l%u = a%u * a%u; a%u = l%u;", d, dv.ik, dv.kk, dv.ik, d);
+                }
+                op.stmt = buf;
+                ops.push_back(op);
+                for (u32 su = dv.sub_begin; su < dv.sub_end; ++su) {
+                    const KinLuSub &sb = m.lu_sub[su];
+                    Op o2;
+                    if (regs && near(sb.kj, i)) {
+                        snprintf(buf, sizeof(buf), "a%u = simk_fnma(l%u, a%u, a%u);", sb.ij, d, sb.kj, sb.ij);
+                    } else {
+                        snprintf(buf, sizeof(buf), "W(%u)", sb.kj);
+                        o2.load = buf;
+                        snprintf(buf, sizeof(buf), "a%u = simk_fnma(l%u, %%s, a%u);", sb.ij, d, sb.ij);
+                    }
+                    o2.stmt = buf;
+                    ops.push_back(o2);
+                }
+            }
+            snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
+            stmt(buf);
+            for (u32 idx : row) {
+                snprintf(buf, sizeof(buf), "WS(%u, a%u);", idx, idx);
+                stmt(buf);
             }
         }
-        // Eliminations against finished rows k < i.
-        for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
-            const KinLuDiv &dv = m.lu_div[d];
-            w("            { const double l = a%u * W(%u); a%u = l;\n", dv.ik, dv.kk, dv.ik);
-            for (u32 s = dv.sub_begin; s < dv.sub_end; ++s)
-                w("              a%u = simk_fnma(l, W(%u), a%u);\n", m.lu_sub[s].ij, m.lu_sub[s].kj, m.lu_sub[s].ij);
-            w("            }\n");
-        }
-        w("            a%u = 1.0 / (a%u + g);\n", m.diag[i], m.diag[i]);
-        for (u32 idx : row) w("            WS(%u, a%u);\n", idx, idx);
-        if (o.fence_rows) w("            SIMK_MEMORY_FENCE();\n");
-        w("        }\n");
+        barrier();
+        w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid);
+        w("    }\n");
     }
-    w("        SIMK_MEMORY_FENCE();\n    }\n");
 
     // solve
-    w("    SIMK_DEV void solve(Vec &x) const {\n        SIMK_MEMORY_FENCE();\n");
-    for (u32 i = 0; i < n; ++i)
-        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e)
-            w("        x.v[%u] = simk_fnma(W(%u), x.v[%u], x.v[%u]);\n", i, m.lower[e].index, m.lower[e].col, i);
-    for (u32 ii = n; ii-- > 0;) {
-        for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e)
-            w("        x.v[%u] = simk_fnma(W(%u), x.v[%u], x.v[%u]);\n", ii, m.upper[e].index, m.upper[e].col, ii);
-        w("        x.v[%u] = x.v[%u] * W(%u);\n", ii, ii, m.diag[ii]);
+    {
+        std::vector<Op> ops;
+        char buf[160];
+        for (u32 i = 0; i < n; ++i)
+            for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
+                snprintf(buf, sizeof(buf), "W(%u)", m.lower[e].index);
+                Op op;
+                op.load = buf;
+                snprintf(buf, sizeof(buf), "x.v[%u] = simk_fnma(%%s, x.v[%u], x.v[%u]);", i, m.lower[e].col, i);
+                op.stmt = buf;
+                ops.push_back(op);
+            }
+        for (u32 ii = n; ii-- > 0;) {
+            for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
+                Op op;
+                snprintf(buf, sizeof(buf), "W(%u)", m.upper[e].index);
+                op.load = buf;
+                snprintf(buf, sizeof(buf), "x.v[%u] = simk_fnma(%%s, x.v[%u], x.v[%u]);", ii, m.upper[e].col, ii);
+                op.stmt = buf;
+                ops.push_back(op);
+            }
+            Op op;
+            snprintf(buf, sizeof(buf), "W(%u)", m.diag[ii]);
+            op.load = buf;
+            snprintf(buf, sizeof(buf), "x.v[%u] = x.v[%u] * %%s;", ii, ii);
+            op.stmt = buf;
+            ops.push_back(op);
+        }
+        w("    SIMK_DEV void solve(Vec &x) const {\n        SIMK_MEMORY_FENCE();\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid);
+        w("    }\n};\n\n");
     }
-    w("    }\n};\n\n");
 
     // Method with compile-time coefficients.
     u32 eval[4], save_f[4];
@@ -214,7 +386,7 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
           o.waves_per_eu, name);
     else
         w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) %s(SimkKinArgs A) {\n", name);
-    w("    GenMethod me;\n    simk_lane_body<GenOps, GenMethod>(A, me);\n}\n");
+    w("    GenMethod me;\n    simk_persistent_body<GenOps, GenMethod>(A, me);\n}\n");
     g.source = std::move(w.s);
     return g;
 }

@@ -12,9 +12,11 @@ namespace simk {
 
 // Knobs the autotuner sweeps. Each changes speed only, never results.
 struct GenOptions {
-    u32 fence_rows = 1;      // compiler memory fence after every factored row (else only after the factor)
+    u32 fence_rows = 0;      // 1: re-read finished rows from the workspace (fence per row); 0: keep them in registers
     u32 waves_per_eu = 0;    // amdgpu_waves_per_eu lower bound (0: compiler default)
     u32 unroll_stages = 0;   // unroll the stage loop (bigger code, constant stage offsets)
+    u32 batch = 16;          // loads issued this many statements ahead (software pipelining)
+    u32 reg_window = 8;      // register mode: finished rows read from registers up to this many rows back (0: all)
 };
 
 struct GenResult {

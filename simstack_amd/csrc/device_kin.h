@@ -71,7 +71,10 @@ struct SimkKinArgs {
     double duration;
     double *ws;              // workspace: [wave][slot][64] f64
     simk_u32 ws_slots;       // slots per lane
-    simk_u32 n_species, n_reactions, pad_;
+    simk_u32 n_species, n_reactions;
+    simk_u32 repeats;        // consecutive advances of `duration` per cell (h carried)
+    simk_u32 *queue;         // next unassigned queue position (starts at gridDim.x * 64)
+    const simk_u32 *order;   // queue position -> cell (null: identity)
     const SimkDevTables *tables;
     SimkDevCtl ctl;
     SimkDevMethod method;
@@ -125,9 +128,11 @@ struct SimkRows {
     __amdgpu_buffer_rsrc_t rs;
     simk_u32 voff;           // lane * 8
     simk_u32 stride;         // bytes between rows
-    SIMK_DEV void init(const double *wave_base, simk_u32 lane, simk_u32 row_stride_bytes, simk_u32 bytes) {
-        rs = __builtin_amdgcn_make_buffer_rsrc((void *)wave_base, 0, (int)bytes, SIMK_RSRC_FLAGS);
-        voff = lane * 8u;
+    // base: first element seen by lane offset 0; voff_bytes: this lane's byte
+    // offset; bytes: extent of the array from base (accesses beyond read 0).
+    SIMK_DEV void init(const double *base, simk_u32 voff_bytes, simk_u32 row_stride_bytes, simk_u32 bytes) {
+        rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)bytes, SIMK_RSRC_FLAGS);
+        voff = voff_bytes;
         stride = row_stride_bytes;
     }
     SIMK_DEV double ld(simk_u32 row) const {
@@ -173,195 +178,254 @@ struct SimkWsVec {
 // only), and a stage whose point equals the previous evaluation point reuses
 // the saved f instead of recomputing the same bits (Rodas3 stage 2).
 
+// One step attempt of kin_integrate's loop body (kin_solver.h:216-311) for the
+// active lanes: step size, -J/LU, the stages, error norm, accept or reject,
+// next h. The caller has already counted the attempt against max_steps.
 template <class P, class M>
-SIMK_DEV void simk_integrate(P &p, const M &me, const SimkDevCtl &ctl, typename P::Vec &y, double &t,
-                             const double t_end, double &h, double &steps, double &rejects, double &failed) {
+SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P::Vec &y, double &t,
+                           const double t_end, double &h, double &steps, double &rejects, bool &reject_last,
+                           bool &reject_more) {
     const simk_u32 n = p.n();
-    bool active = (t < t_end) && !(failed > 0.0);
-    bool reject_last = false, reject_more = false;
-    h = h > 0.0 ? h : ctl.h_start;
-    simk_u32 iterations = 0;
-    while (active) {
-        if (++iterations > ctl.max_steps) {
-            failed = 1.0;
-            break;
-        }
-        const double remaining = t_end - t;
-        double h_cap = remaining;
-        if (ctl.h_max > 0.0) h_cap = simk_min(h_cap, ctl.h_max);
-        const double h_wanted = simk_max(h, ctl.h_min);
-        const bool last = h_wanted >= remaining;
-        const double hs = last ? remaining : simk_min(h_wanted, h_cap);
+    const double remaining = t_end - t;
+    double h_cap = remaining;
+    if (ctl.h_max > 0.0) h_cap = simk_min(h_cap, ctl.h_max);
+    const double h_wanted = simk_max(h, ctl.h_min);
+    const bool last = h_wanted >= remaining;
+    const double hs = last ? remaining : simk_min(h_wanted, h_cap);
 
-        const double ghinv = 1.0 / (hs * me.gamma0());
-        p.jac_factor(y, ghinv);
-        const double hinv = 1.0 / hs;
+    const double ghinv = 1.0 / (hs * me.gamma0());
+    p.jac_factor(y, ghinv);
+    const double hinv = 1.0 / hs;
 
-        typename P::Vec x = p.vec(3);
-        // One stage: its point, f there (or the saved f), the c-combination
-        // of earlier stages, the solve. Rolled by default so the solve and f
-        // are emitted once; the generator may unroll it.
-        auto stage = [&](const simk_u32 st) {
-            const simk_u32 packed = st * (st - 1) / 2;
-            if (st == 0 || me.eval(st)) {
-                typename P::Vec pt = p.vec(2);
+    typename P::Vec x = p.vec(3);
+    // One stage: its point, f there (or the saved f), the c-combination of
+    // earlier stages, the solve. Rolled by default so the solve and f are
+    // emitted once; the generator may unroll it.
+    auto stage = [&](const simk_u32 st) {
+        const simk_u32 packed = st * (st - 1) / 2;
+        if (st == 0 || me.eval(st)) {
+            typename P::Vec pt = p.vec(2);
 #pragma unroll
-                for (simk_u32 s = 0; s < n; ++s) pt.set(s, y.get(s));
-                for (simk_u32 j = 0; j < st; ++j) {
-                    const double a = me.a(packed + j);
-                    if (a == 0.0) continue;
-#pragma unroll
-                    for (simk_u32 s = 0; s < n; ++s) pt.set(s, simk_fma(a, p.kget(j, s), pt.get(s)));
-                }
-                p.fun(pt, x);
-                if (me.save_f(st)) p.fsave(x);
-            } else {
-                p.fload(x);
-            }
+            for (simk_u32 s = 0; s < n; ++s) pt.set(s, y.get(s));
             for (simk_u32 j = 0; j < st; ++j) {
-                const double c = me.c(packed + j);
-                if (c == 0.0) continue;
-                const double hc = c * hinv;
+                const double a = me.a(packed + j);
+                if (a == 0.0) continue;
 #pragma unroll
-                for (simk_u32 s = 0; s < n; ++s) x.set(s, simk_fma(hc, p.kget(j, s), x.get(s)));
+                for (simk_u32 s = 0; s < n; ++s) pt.set(s, simk_fma(a, p.kget(j, s), pt.get(s)));
             }
-            p.solve(x);
-            p.kstore(st, x);
-        };
-        if constexpr (M::kUnrollStages) {
-#pragma unroll
-            for (simk_u32 st = 0; st < 4; ++st)
-                if (st < me.stages()) stage(st);
+            p.fun(pt, x);
+            if (me.save_f(st)) p.fsave(x);
         } else {
-#pragma unroll 1
-            for (simk_u32 st = 0; st < me.stages(); ++st) stage(st);
+            p.fload(x);
         }
+        for (simk_u32 j = 0; j < st; ++j) {
+            const double c = me.c(packed + j);
+            if (c == 0.0) continue;
+            const double hc = c * hinv;
+#pragma unroll
+            for (simk_u32 s = 0; s < n; ++s) x.set(s, simk_fma(hc, p.kget(j, s), x.get(s)));
+        }
+        p.solve(x);
+        p.kstore(st, x);
+    };
+    if constexpr (M::kUnrollStages) {
+#pragma unroll
+        for (simk_u32 st = 0; st < 4; ++st)
+            if (st < me.stages()) stage(st);
+    } else {
+#pragma unroll 1
+        for (simk_u32 st = 0; st < me.stages(); ++st) stage(st);
+    }
 
-        typename P::Vec ynew = p.vec(2);
-        double err = 0.0;
+    typename P::Vec ynew = p.vec(2);
+    double err = 0.0;
+#pragma unroll
+    for (simk_u32 s = 0; s < n; ++s) {
+        double yn = y.get(s), yerr = 0.0;
+#pragma unroll
+        for (simk_u32 st = 0; st < 4; ++st) {
+            if (st >= me.stages()) break;
+            const double mc = me.m(st), ec = me.e(st);
+            if (mc == 0.0 && ec == 0.0) continue;
+            const double kv = p.kfinal(st, s, x);
+            if (mc != 0.0) yn = simk_fma(mc, kv, yn);
+            if (ec != 0.0) yerr = simk_fma(ec, kv, yerr);
+        }
+        const double scale = simk_fma(ctl.rtol, simk_max(simk_abs(y.get(s)), simk_abs(yn)), ctl.atol);
+        const double q = yerr / scale;
+        err = simk_fma(q, q, err);
+        ynew.set(s, yn);
+    }
+    err = simk_sqrt(err * ctl.inv_n);
+    if (!simk_is_finite(err)) err = 1e10;
+
+    double fac = simk_step_factor(err, me.order());
+    bool ok = (err <= 1.0) || (hs <= ctl.h_min);
+    if (ctl.fixed_step) {
+        ok = true;
+        fac = 1.0;
+    }
+    if (ok) {
 #pragma unroll
         for (simk_u32 s = 0; s < n; ++s) {
-            double yn = y.get(s), yerr = 0.0;
-#pragma unroll
-            for (simk_u32 st = 0; st < 4; ++st) {
-                if (st >= me.stages()) break;
-                const double mc = me.m(st), ec = me.e(st);
-                if (mc == 0.0 && ec == 0.0) continue;
-                const double kv = p.kfinal(st, s, x);
-                if (mc != 0.0) yn = simk_fma(mc, kv, yn);
-                if (ec != 0.0) yerr = simk_fma(ec, kv, yerr);
-            }
-            const double scale = simk_fma(ctl.rtol, simk_max(simk_abs(y.get(s)), simk_abs(yn)), ctl.atol);
-            const double q = yerr / scale;
-            err = simk_fma(q, q, err);
-            ynew.set(s, yn);
+            double v = ynew.get(s);
+            if (ctl.clamp_negative) v = simk_max(v, 0.0);
+            y.set(s, v);
         }
-        err = simk_sqrt(err * ctl.inv_n);
-        if (!simk_is_finite(err)) err = 1e10;
-
-        double fac = simk_step_factor(err, me.order());
-        bool ok = (err <= 1.0) || (hs <= ctl.h_min);
-        if (ctl.fixed_step) {
-            ok = true;
-            fac = 1.0;
-        }
-        if (ok) {
-#pragma unroll
-            for (simk_u32 s = 0; s < n; ++s) {
-                double v = ynew.get(s);
-                if (ctl.clamp_negative) v = simk_max(v, 0.0);
-                y.set(s, v);
-            }
-            t = last ? t_end : t + hs;
-        }
-        const double hfac = hs * fac;
-        double h_acc = hfac;
-        if (reject_last) h_acc = simk_min(h_acc, hs);
-        if (last && fac >= 1.0) h_acc = simk_max(h_acc, h_wanted);
-        double h_rej = reject_more ? hs * 0.1 : simk_min(hfac, hs);
-        if (ctl.fixed_step) {
-            h_acc = h_wanted;
-            h_rej = h_wanted;
-        }
-        h = ok ? h_acc : h_rej;
-        reject_more = !ok && reject_last;
-        reject_last = !ok;
-        steps = steps + (ok ? 1.0 : 0.0);
-        rejects = rejects + (ok ? 0.0 : 1.0);
-        active = t < t_end;
+        t = last ? t_end : t + hs;
     }
+    const double hfac = hs * fac;
+    double h_acc = hfac;
+    if (reject_last) h_acc = simk_min(h_acc, hs);
+    if (last && fac >= 1.0) h_acc = simk_max(h_acc, h_wanted);
+    double h_rej = reject_more ? hs * 0.1 : simk_min(hfac, hs);
+    if (ctl.fixed_step) {
+        h_acc = h_wanted;
+        h_rej = h_wanted;
+    }
+    h = ok ? h_acc : h_rej;
+    reject_more = !ok && reject_last;
+    reject_last = !ok;
+    steps = steps + (ok ? 1.0 : 0.0);
+    rejects = rejects + (ok ? 0.0 : 1.0);
 }
 
-// kin_advance (kin_solver.h:322-347) for one lane: pulse segments over local
-// time [0, duration], h restarts at h_start after each pulse.
-template <class P, class M>
-SIMK_DEV void simk_advance(P &p, const M &me, const SimkDevCtl &ctl, typename P::Vec &y, const double duration,
-                           double &h, const double period, const double first, const double count, double &steps,
-                           double &rejects, double &pulses, double &failed) {
-    double t = 0.0, applied = 0.0;
-    for (;;) {
-        const bool pending = applied < count;
-        const double next_pulse = simk_fma(applied, period, first);
-        const double seg_end = pending ? simk_min(next_pulse, duration) : duration;
-        simk_integrate(p, me, ctl, y, t, seg_end, h, steps, rejects, failed);
-        const bool alive = !(failed > 0.0);
-        const bool hit = pending && alive && (t >= seg_end);
-        if (hit) {
-            const simk_u32 n = p.n();
-#pragma unroll
-            for (simk_u32 s = 0; s < n; ++s) y.set(s, y.get(s) + p.pulse(s));
-            applied = applied + 1.0;
-            pulses = pulses + 1.0;
-            h = ctl.h_start;
-        }
-        // The reference's exit test; the extra "!pending" form is equivalent for
-        // every finite count and ends the loop where a NaN count never would.
-        const bool done = ((t >= duration) && (applied >= count || !pending)) || !alive;
-        if (done) break;
-    }
-}
+// Lane phases of the persistent kernel.
+#define SIMK_PH_CELL 0      // needs a cell from the queue
+#define SIMK_PH_SEGMENT 1   // starts an integration segment (kin_advance loop head)
+#define SIMK_PH_CHECK 2     // counts the next attempt against max_steps
+#define SIMK_PH_ATTEMPT 3   // makes a step attempt this iteration
+#define SIMK_PH_END 4       // segment ended: pulse, done test
+#define SIMK_PH_EXIT 5      // queue exhausted
 
-// The per-lane body of a launch (dbd_advance_group, src/sim/dbd_lanes.h:28-78):
-// load the cell's state, advance, store state and add the counters.
+// Persistent, lane-refilling advance. Every lane runs the reference's per-cell
+// control flow as a state machine — dbd_advance_group (dbd_lanes.h:28-78)
+// around kin_advance (kin_solver.h:322-347) around kin_integrate's loop
+// (:204-312) — and one wave-wide iteration makes one step attempt for every
+// lane that has one pending. A lane whose cell is finished stores it and takes
+// the next queue position, so lanes stay busy while the queue lasts. Each
+// cell's arithmetic is exactly the reference's, whatever lane or order runs
+// it. `repeats` runs that many consecutive advances of `duration` per cell
+// (the result of that many calls with unchanged inputs), carrying h.
 template <class P, class M>
-SIMK_DEV void simk_lane_body(const SimkKinArgs &A, const M &me) {
+SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
     const simk_u32 wave = blockIdx.x;
     const simk_u32 lane = threadIdx.x;
-    const simk_u32 cell = wave * SIMK_WAVE + lane;
-    if (cell >= A.lanes) return;
-    const simk_u32 w0 = wave * SIMK_WAVE;
+    const SimkDevCtl &ctl = A.ctl;
     const simk_u32 stride = A.padded * 8u;
-    const simk_u32 tail = (A.padded - w0) * 8u;   // bytes of one row from this wave on
-    SimkRows ys, one;
-    ys.init(A.y + w0, lane, stride, (A.n_species - 1) * stride + tail);
+    const simk_u32 row_bytes = A.padded * 8u;
     P p;
     p.init(A, wave, lane);
     typename P::Vec y = p.vec(0);
     const simk_u32 n = p.n();
+
+    SimkRows ys, hs_, fs;
+    ys.init(A.y, 0, stride, A.n_species * row_bytes);
+    hs_.init(A.h, 0, 0, row_bytes);
+    fs.init(A.failed, 0, 0, row_bytes);
+
+    simk_u32 pos = wave * SIMK_WAVE + lane;   // first queue position: no atomic needed
+    simk_u32 cell = 0, phase = SIMK_PH_CELL, it = 0, rep = 0;
+    double h = 0.0, t = 0.0, applied = 0.0, seg_end = 0.0, period = 0.0, first = 0.0, count = 0.0;
+    double steps = 0.0, rejects = 0.0, pulses = 0.0, failed = 0.0;
+    bool pending = false, reject_last = false, reject_more = false, fresh = true;
+
+    for (;;) {
+        // Resolve each lane's transitions until it has an attempt to make or
+        // the queue is exhausted (bounded: every pass consumes a segment, a
+        // pulse, an advance or a cell).
+        while (phase != SIMK_PH_ATTEMPT && phase != SIMK_PH_EXIT) {
+            if (phase == SIMK_PH_CELL) {
+                if (!fresh) {
+                    // Wave-aggregated refill: one atomic for every lane asking now.
+                    const simk_u64 ask = __builtin_amdgcn_read_exec();
+                    const simk_u32 rank = __builtin_amdgcn_mbcnt_hi((simk_u32)(ask >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((simk_u32)ask, 0u));
+                    simk_u32 base = 0;
+                    if (rank == 0) base = atomicAdd(A.queue, (simk_u32)__builtin_popcountll(ask));
+                    const simk_u32 leader = (simk_u32)__builtin_ctzll(ask);
+                    base = (simk_u32)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)base);
+                    pos = base + rank;
+                }
+                fresh = false;
+                if (pos >= A.lanes) {
+                    phase = SIMK_PH_EXIT;
+                    break;
+                }
+                cell = A.order ? A.order[pos] : pos;
+                const simk_u32 voff = cell * 8u;
+                ys.voff = hs_.voff = fs.voff = voff;
+                p.set_cell(A, cell);
 #pragma unroll
-    for (simk_u32 s = 0; s < n; ++s) y.set(s, ys.ld(s));
-    one.init(A.h + w0, lane, 0, tail);
-    double h = one.ld(0);
-    one.init(A.period + w0, lane, 0, tail);
-    const double period = one.ld(0);
-    one.init(A.pulse_first + w0, lane, 0, tail);
-    const double first = one.ld(0);
-    one.init(A.pulse_count + w0, lane, 0, tail);
-    const double count = one.ld(0);
-    SimkRows fs;
-    fs.init(A.failed + w0, lane, 0, tail);
-    double failed = fs.ld(0);
-    double steps = 0.0, rejects = 0.0, pulses = 0.0;
-    simk_advance(p, me, A.ctl, y, A.duration, h, period, first, count, steps, rejects, pulses, failed);
+                for (simk_u32 s = 0; s < n; ++s) y.set(s, ys.ld(s));
+                h = hs_.ld(0);
+                failed = fs.ld(0);
+                SimkRows one;
+                one.init(A.period, voff, 0, row_bytes);
+                period = one.ld(0);
+                one.init(A.pulse_first, voff, 0, row_bytes);
+                first = one.ld(0);
+                one.init(A.pulse_count, voff, 0, row_bytes);
+                count = one.ld(0);
+                steps = rejects = pulses = 0.0;
+                t = 0.0;
+                applied = 0.0;
+                rep = 0;
+                phase = SIMK_PH_SEGMENT;
+            } else if (phase == SIMK_PH_SEGMENT) {
+                pending = applied < count;
+                const double next_pulse = simk_fma(applied, period, first);
+                seg_end = pending ? simk_min(next_pulse, A.duration) : A.duration;
+                h = h > 0.0 ? h : ctl.h_start;
+                reject_last = reject_more = false;
+                it = 0;
+                phase = ((t < seg_end) && !(failed > 0.0)) ? SIMK_PH_CHECK : SIMK_PH_END;
+            } else if (phase == SIMK_PH_CHECK) {
+                if (++it > ctl.max_steps) {
+                    failed = 1.0;
+                    phase = SIMK_PH_END;
+                } else {
+                    phase = SIMK_PH_ATTEMPT;
+                }
+            } else {   // SIMK_PH_END
+                const bool alive = !(failed > 0.0);
+                const bool hit = pending && alive && (t >= seg_end);
+                if (hit) {
 #pragma unroll
-    for (simk_u32 s = 0; s < n; ++s) ys.st(s, y.get(s));
-    one.init(A.h + w0, lane, 0, tail);
-    one.st(0, h);
-    fs.st(0, failed);
-    one.init(A.steps + w0, lane, 0, tail);
-    one.st(0, one.ld(0) + steps);
-    one.init(A.rejects + w0, lane, 0, tail);
-    one.st(0, one.ld(0) + rejects);
-    one.init(A.pulses + w0, lane, 0, tail);
-    one.st(0, one.ld(0) + pulses);
+                    for (simk_u32 s = 0; s < n; ++s) y.set(s, y.get(s) + p.pulse(s));
+                    applied = applied + 1.0;
+                    pulses = pulses + 1.0;
+                    h = ctl.h_start;
+                }
+                // The reference's exit test; "!pending" is the same for every
+                // finite count and also ends the loop for a NaN count.
+                const bool done = ((t >= A.duration) && (applied >= count || !pending)) || !alive;
+                if (!done) {
+                    phase = SIMK_PH_SEGMENT;
+                } else if (++rep < A.repeats) {
+                    t = 0.0;
+                    applied = 0.0;
+                    phase = SIMK_PH_SEGMENT;
+                } else {
+#pragma unroll
+                    for (simk_u32 s = 0; s < n; ++s) ys.st(s, y.get(s));
+                    hs_.st(0, h);
+                    fs.st(0, failed);
+                    SimkRows acc;
+                    acc.init(A.steps, cell * 8u, 0, row_bytes);
+                    acc.st(0, acc.ld(0) + steps);
+                    acc.init(A.rejects, cell * 8u, 0, row_bytes);
+                    acc.st(0, acc.ld(0) + rejects);
+                    acc.init(A.pulses, cell * 8u, 0, row_bytes);
+                    acc.st(0, acc.ld(0) + pulses);
+                    phase = SIMK_PH_CELL;
+                }
+            }
+        }
+        if (__builtin_amdgcn_ballot_w64(phase == SIMK_PH_ATTEMPT) == 0) break;
+        if (phase == SIMK_PH_ATTEMPT) {
+            simk_attempt(p, me, ctl, y, t, seg_end, h, steps, rejects, reject_last, reject_more);
+            phase = (t < seg_end) ? SIMK_PH_CHECK : SIMK_PH_END;
+        }
+    }
 }

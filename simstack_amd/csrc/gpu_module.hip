@@ -39,12 +39,14 @@ struct TableOps {
         n_ = T->n;
         R_ = T->R;
         nnz_ = T->nnz;
-        const simk_u32 w0 = wave * SIMK_WAVE, stride = A.padded * 8u, tail = (A.padded - w0) * 8u;
-        kr.init(A.k + w0, lane, stride, (R_ - 1) * stride + tail);
-        sr.init(A.src + w0, lane, stride, (n_ - 1) * stride + tail);
-        pr.init(A.pulse + w0, lane, stride, (n_ - 1) * stride + tail);
-        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);
+        const simk_u32 stride = A.padded * 8u;
+        kr.init(A.k, 0, stride, R_ * stride);
+        sr.init(A.src, 0, stride, n_ * stride);
+        pr.init(A.pulse, 0, stride, n_ * stride);
+        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u,
+                A.ws_slots * SIMK_WAVE * 8u);
     }
+    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) { kr.voff = sr.voff = pr.voff = cell * 8u; }
     // Workspace slots: [0, nnz) LU; vectors i = 0..3 at nnz + i n; K_st at
     // nnz + (4 + st) n; the reused f at nnz + 8 n.
     SIMK_DEV simk_u32 n() const { return n_; }
@@ -140,7 +142,7 @@ struct ValueMethod {   // method copied by value from the kernel argument
 
 extern "C" __global__ void __launch_bounds__(SIMK_WAVE) simk_table_kernel(SimkKinArgs A) {
     ValueMethod me{A.method};
-    simk_lane_body<TableOps, ValueMethod>(A, me);
+    simk_persistent_body<TableOps, ValueMethod>(A, me);
 }
 
 // The reference's M1 point demo (a spring/swirl particle step), kept so every
@@ -160,6 +162,19 @@ __global__ void simk_step_points_kernel(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy
     x[i] = __builtin_fmaf(nvx, dt, px);
     y[i] = __builtin_fmaf(nvy, dt, py);
     z[i] = __builtin_fmaf(nvz, dt, pz);
+}
+
+// Plain streaming copy (16 bytes per lane per iteration): the reference
+// point for HBM byte counters (a copy of B bytes moves exactly 2 B).
+__global__ void __launch_bounds__(256) simk_copy_kernel(double2 *dst, const double2 *src, u64 n) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Sets the persistent kernels' queue head (a kernel, not a memset, so the
+// sequence stays valid inside a captured graph).
+__global__ void simk_queue_reset_kernel(u32 *queue, u32 value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *queue = value;
 }
 
 // ---------------------------------------------------------------------------
@@ -201,12 +216,18 @@ struct State {
     JitKernel jit;
     bool jit_ok = false;
     std::string jit_log;
-    // workspace
+    // workspace and queue
     double *ws = nullptr;
     u64 ws_bytes = 0;
+    u32 *queue = nullptr;
+    u32 cus = 0;
+    u32 grid_override = 0;     // persistent waves per launch (0: resident waves)
+    int blocks_per_cu_gen = 0, blocks_per_cu_table = 0;
+    const u32 *order = nullptr;
     // last call
     u32 last_kernel = 0;
     u32 last_slots = 0;
+    u32 last_waves = 0;
     f64 last_ms = 0.0;
 };
 
@@ -471,6 +492,12 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     g.ctl.clamp_negative = control->clamp_negative ? 1 : 0;
     e = upload_tables(m);
     if (e != hipSuccess) return err_code(e);
+    if (!g.blocks_per_cu_table) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_table_kernel, SIMK_WAVE, 0) != hipSuccess)
+            blocks = 1;
+        g.blocks_per_cu_table = blocks;
+    }
     g.have_mech = true;
 
     // Generated straight-line kernel (compiled or loaded from the disk cache).
@@ -484,6 +511,10 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
             if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
                 hipFunction_t fn;
                 if (hipModuleGetFunction(&fn, mod, gr.kernel.c_str()) == hipSuccess) {
+                    int blocks = 0;
+                    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, SIMK_WAVE, 0) != hipSuccess)
+                        blocks = 1;
+                    g.blocks_per_cu_gen = blocks;
                     g.jit.mod = mod;
                     g.jit.fn = fn;
                     g.jit.ws_slots = gr.ws_slots;
@@ -501,10 +532,12 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     return 0;
 }
 
-// Launches one advance of lanes [0, lanes) on the module stream.
-hipError_t launch(const GpuKinLaunch *L) {
+// Launches `repeats` consecutive advances of queue positions [0, lanes) on the
+// module stream with the persistent kernel: grid = resident waves (or fewer
+// when there are fewer cells), lanes refill from the queue.
+hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     const u32 lanes = L->lane_count ? L->lane_count : L->padded;
-    if (lanes == 0) return hipSuccess;
+    if (lanes == 0 || repeats == 0) return hipSuccess;
     if (lanes > L->padded) return hipErrorInvalidValue;
     if (!L->y || !L->h || !L->steps || !L->rejects || !L->pulses || !L->failed || !L->k || !L->src || !L->pulse ||
         !L->period || !L->pulse_first || !L->pulse_count)
@@ -513,14 +546,23 @@ hipError_t launch(const GpuKinLaunch *L) {
     if (g.mode != 0) return hipErrorNotSupported;
     if (g.kernel_kind == 1 && !g.jit_ok) return hipErrorInvalidImage;
     const bool use_gen = g.jit_ok && g.kernel_kind != 2;
-    const u32 waves = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const u32 slots = use_gen ? g.jit.ws_slots : g.mech.nnz + 9 * g.mech.n;
     // Buffer resources address each [row][padded] array with 32-bit offsets.
     const u64 rows = g.mech.R > g.mech.n ? g.mech.R : g.mech.n;
     if (rows * L->padded * 8ull >= (1ull << 31) || (u64)slots * SIMK_WAVE * 8ull >= (1ull << 31))
         return hipErrorInvalidValue;
+    const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
+    const int per_cu = use_gen ? g.blocks_per_cu_gen : g.blocks_per_cu_table;
+    u32 resident = (u32)(per_cu > 0 ? per_cu : 1) * (g.cus ? g.cus : 1);
+    if (g.grid_override) resident = g.grid_override;
+    const u32 waves = needed < resident ? needed : resident;
     hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(double));
     if (e != hipSuccess) return e;
+    if (!g.queue) {
+        e = hipMalloc(&g.queue, 256);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, g.queue, waves * SIMK_WAVE);
     SimkKinArgs A;
     memset(&A, 0, sizeof(A));
     A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
@@ -533,11 +575,15 @@ hipError_t launch(const GpuKinLaunch *L) {
     A.ws_slots = slots;
     A.n_species = g.mech.n;
     A.n_reactions = g.mech.R;
+    A.repeats = repeats;
+    A.queue = g.queue;
+    A.order = g.order;
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
     A.method = g.dev_method;
     g.last_kernel = use_gen ? 1 : 2;
     g.last_slots = slots;
+    g.last_waves = waves;
     if (use_gen) {
         size_t size = sizeof(A);
         void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
@@ -593,6 +639,7 @@ GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info) {
     g.arena.size = arena_bytes;
     g.arena.used = 0;
     g.device = (int)device_index;
+    g.cus = (u32)prop.multiProcessorCount;
     g.ready = true;
     return 0;
 }
@@ -652,7 +699,7 @@ GPU_API s32 gpu_kin_set_mech(const KinMech *mech, const KinMethod *method, const
 GPU_API s32 gpu_kin_advance(const GpuKinLaunch *L, f32 *kernel_ms) {
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
     hipEventRecord(g.kin_start, g.stream);
-    hipError_t e = launch(L);
+    hipError_t e = launch(L, 1);
     if (e != hipSuccess) return err_code(e);
     hipEventRecord(g.kin_stop, g.stream);
     e = hipEventSynchronize(g.kin_stop);
@@ -680,7 +727,7 @@ GPU_API s32 gpu_kin_advance_async(const GpuKinLaunch *L, void *host_block, void 
     hipError_t e = hipMemcpy2DAsync(device_block, pitch, host_block, pitch, width, rows, hipMemcpyHostToDevice, g.stream);
     if (e != hipSuccess) return err_code(e);
     hipEventRecord(g.kin_start, g.stream);
-    e = launch(L);
+    e = launch(L, 1);
     if (e != hipSuccess) return err_code(e);
     hipEventRecord(g.kin_stop, g.stream);
     e = hipMemcpy2DAsync(host_block, pitch, device_block, pitch, width, mutable_rows, hipMemcpyDeviceToHost, g.stream);
@@ -707,6 +754,31 @@ GPU_API s32 gpu_kin_set_mode(u32 mode) {
     return 0;
 }
 
+GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *L, u32 repeats, f32 *kernel_ms) {
+    if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
+    hipEventRecord(g.kin_start, g.stream);
+    hipError_t e = launch(L, repeats);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_stop, g.stream);
+    e = hipEventSynchronize(g.kin_stop);
+    if (e != hipSuccess) return err_code(e);
+    f32 ms = 0.0f;
+    hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    g.last_ms = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_kin_set_grid(u32 waves) {
+    g.grid_override = waves;
+    return 0;
+}
+
+GPU_API s32 simk_kin_set_order(const u32 *order_device) {
+    g.order = order_device;
+    return 0;
+}
+
 GPU_API s32 simk_kin_set_kernel(u32 kind) {
     if (kind > 2) return err_code(hipErrorInvalidValue);
     g.kernel_kind = kind;
@@ -718,6 +790,8 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.fence_rows = o->fence_rows;
     g.gen.waves_per_eu = o->waves_per_eu;
     g.gen.unroll_stages = o->unroll_stages;
+    g.gen.reg_window = o->reg_window == 0 ? 8u : (o->reg_window == 0xFFFFFFFFu ? 0u : o->reg_window);
+    g.gen.batch = o->batch == 0 ? 16u : o->batch;
     return 0;
 }
 
@@ -726,6 +800,7 @@ GPU_API s32 simk_kin_info(SimkKinInfo *out) {
     memset(out, 0, sizeof(*out));
     out->kernel = g.last_kernel;
     out->ws_slots = g.last_slots;
+    out->waves = g.last_waves;
     out->species = g.mech.n;
     out->reactions = g.mech.R;
     out->nnz = g.mech.nnz;
@@ -750,6 +825,8 @@ GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const Ki
         o.fence_rows = options->fence_rows;
         o.waves_per_eu = options->waves_per_eu;
         o.unroll_stages = options->unroll_stages;
+        o.reg_window = options->reg_window == 0 ? 8u : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
+        o.batch = options->batch == 0 ? 16u : options->batch;
     }
     KinMethod me = *method;
     me.name = nullptr;
@@ -760,6 +837,25 @@ GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const Ki
     bool ok = jit_code(gr, code, cached, text);
     if (log && log_size) snprintf(log, log_size, "%s", text.c_str());
     return ok ? 0 : err_code(hipErrorInvalidImage);
+}
+
+GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms) {
+    if (!dst || !src || (bytes & 15) || ((u64)dst & 15) || ((u64)src & 15)) return err_code(hipErrorInvalidValue);
+    const u64 n = bytes / 16;
+    const u32 block = 256;
+    u64 grid = (n + block - 1) / block;
+    if (grid > 65536) grid = 65536;
+    if (grid == 0) return 0;
+    hipEventRecord(g.kin_start, g.stream);
+    hipLaunchKernelGGL(simk_copy_kernel, dim3((u32)grid), dim3(block), 0, g.stream, (double2 *)dst,
+                       (const double2 *)src, n);
+    hipEventRecord(g.kin_stop, g.stream);
+    hipError_t e = hipEventSynchronize(g.kin_stop);
+    if (e != hipSuccess) return err_code(e);
+    f32 t = 0.0f;
+    hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
+    if (ms) *ms = t;
+    return err_code(hipGetLastError());
 }
 
 }  // extern "C"

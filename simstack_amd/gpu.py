@@ -16,11 +16,12 @@ V = C.c_void_p
 
 class SimkGenOptions(C.Structure):
     _fields_ = [("fence_rows", C.c_uint32), ("waves_per_eu", C.c_uint32), ("unroll_stages", C.c_uint32),
-                ("pad_", C.c_uint32)]
+                ("reg_window", C.c_uint32), ("batch", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class SimkKinInfo(C.Structure):
-    _fields_ = [("kernel", C.c_uint32), ("ws_slots", C.c_uint32), ("species", C.c_uint32),
+    _fields_ = [("kernel", C.c_uint32), ("ws_slots", C.c_uint32), ("waves", C.c_uint32), ("pad_", C.c_uint32),
+                ("species", C.c_uint32),
                 ("reactions", C.c_uint32), ("nnz", C.c_uint32), ("lu_div", C.c_uint32), ("lu_sub", C.c_uint32),
                 ("jit_cached", C.c_uint32), ("jit_ms", C.c_double), ("last_kernel_ms", C.c_double),
                 ("signature", C.c_uint64), ("kernel_name", C.c_char * 64)]
@@ -31,8 +32,9 @@ REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_use
                      "gpu_copy_to_device", "gpu_copy_to_host", "gpu_sync", "gpu_step_points", "gpu_kin_set_mech",
                      "gpu_kin_advance", "gpu_host_register", "gpu_host_unregister", "gpu_kin_advance_async",
                      "gpu_kin_wait", "gpu_kin_set_mode")
-EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
-                     "simk_jit_precompile")
+EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_set_grid", "simk_kin_set_order",
+                     "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
+                     "simk_jit_precompile", "simk_copy_device")
 
 
 def _sig(lib, name, res, *args):
@@ -69,6 +71,10 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_kin_info", C.c_int32, P(SimkKinInfo))
     _sig(lib, "simk_jit_precompile", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(SimkGenOptions),
          C.c_char_p, C.c_uint32)
+    _sig(lib, "simk_copy_device", C.c_int32, V, V, C.c_uint64, P(C.c_float))
+    _sig(lib, "simk_kin_advance_n", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, P(C.c_float))
+    _sig(lib, "simk_kin_set_grid", C.c_int32, C.c_uint32)
+    _sig(lib, "simk_kin_set_order", C.c_int32, V)
     return lib
 
 
@@ -76,7 +82,7 @@ def precompile(mech, meth, options=None, lib=None):
     """Generates and compiles a mechanism's kernel into the on-disk cache (no device needed)."""
     lib = lib or load_library()
     log = C.create_string_buffer(1 << 16)
-    opt = options or SimkGenOptions(1, 0, 0, 0)
+    opt = options or SimkGenOptions()
     rc = lib.simk_jit_precompile(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(opt), log,
                                  1 << 16)
     if rc != 0:
@@ -107,9 +113,12 @@ class GpuModule:
     def set_kernel(self, kind):
         self._check(self.lib.simk_kin_set_kernel(kind), "simk_kin_set_kernel")
 
-    def set_gen_options(self, fence_rows=1, waves_per_eu=0):
-        self._check(self.lib.simk_kin_set_gen_options(C.byref(SimkGenOptions(fence_rows, waves_per_eu, 0, 0))),
-                    "simk_kin_set_gen_options")
+    def set_gen_options(self, fence_rows=0, waves_per_eu=0, unroll_stages=0, reg_window=0, batch=0):
+        opt = SimkGenOptions(fence_rows, waves_per_eu, unroll_stages, reg_window, batch)
+        self._check(self.lib.simk_kin_set_gen_options(C.byref(opt)), "simk_kin_set_gen_options")
+
+    def set_grid(self, waves):
+        self._check(self.lib.simk_kin_set_grid(waves), "simk_kin_set_grid")
 
     def kin_info(self):
         info = SimkKinInfo()
@@ -160,9 +169,13 @@ class DeviceZones:
         L.duration = duration
         return L
 
-    def advance(self, duration, lane_count=0, precision=0):
-        """gpu_kin_advance on the device block; returns kernel milliseconds."""
+    def advance(self, duration, lane_count=0, precision=0, repeats=1):
+        """gpu_kin_advance (or, repeats > 1, simk_kin_advance_n) on the device block;
+        returns kernel milliseconds."""
         L = self.launch_desc(duration, lane_count, precision)
         ms = C.c_float(0.0)
-        self.gm._check(self.gm.lib.gpu_kin_advance(C.byref(L), C.byref(ms)), "gpu_kin_advance")
+        if repeats == 1:
+            self.gm._check(self.gm.lib.gpu_kin_advance(C.byref(L), C.byref(ms)), "gpu_kin_advance")
+        else:
+            self.gm._check(self.gm.lib.simk_kin_advance_n(C.byref(L), repeats, C.byref(ms)), "simk_kin_advance_n")
         return ms.value

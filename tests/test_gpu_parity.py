@@ -114,3 +114,52 @@ def test_async_path(gm):
     oracle_run(mech, meth, ctl, b, [0.5])
     assert compare_bits(a, b) == ZERO
     assert tms.value >= kms.value > 0.0
+
+
+def test_persistent_refill_and_order(gm):
+    """Fewer resident waves than cells (lanes refill from the queue) and a
+    shuffled queue order give the same bits as the oracle: a cell's result does
+    not depend on which lane, wave or position runs it."""
+    from simstack_amd import gpu
+    sp, mech, z = make_problem("c3c", 2000)
+    meth, ctl = problem.method(2), problem.control()
+    a, b, c = z.copy(), z.copy(), z.copy()
+    try:
+        gm.set_grid(4)
+        gpu_run(gm, mech, meth, ctl, a, [0.5], kernel=1)
+        gm.set_grid(3)
+        gm.arena_reset()
+        gm.set_mech(mech, meth, ctl)
+        dz = gpu.DeviceZones(gm, b)
+        perm = np.random.default_rng(1).permutation(2000).astype(np.uint32)
+        d_order = gm.alloc(perm.nbytes)
+        gm.to_device(d_order, perm)
+        gm._check(gm.lib.simk_kin_set_order(d_order), "set_order")
+        dz.upload()
+        dz.advance(0.5)
+        dz.download(mutable_only=False)
+    finally:
+        gm.lib.simk_kin_set_order(None)
+        gm.set_grid(0)
+    oracle_run(mech, meth, ctl, c, [0.5])
+    assert compare_bits(a, c) == ZERO
+    assert compare_bits(b, c) == ZERO
+
+
+def test_fused_repeats_equal_separate_calls(gm):
+    """simk_kin_advance_n(3) == three gpu_kin_advance calls == the oracle."""
+    from simstack_amd import gpu
+    sp, mech, z = make_problem("c2", 512)
+    meth, ctl = problem.method(2), problem.control()
+    a, c = z.copy(), z.copy()
+    gm.arena_reset()
+    gm.set_kernel(1)
+    gm.set_mech(mech, meth, ctl)
+    dz = gpu.DeviceZones(gm, a)
+    dz.upload()
+    dz.advance(0.5, repeats=3)
+    dz.download(mutable_only=False)
+    lanes = np.arange(0, 512, 7)
+    sub = c.subset(lanes)
+    oracle_run(mech, meth, ctl, sub, [0.5, 0.5, 0.5])
+    assert compare_bits(a.subset(lanes), sub) == ZERO

@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--reg-window", type=int, default=0, help="generator register window (0: default)")
     ap.add_argument("--waves-per-eu", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64 (bit-exact), 2 mixed precision")
+    ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (cells shard with no "
@@ -84,7 +86,8 @@ def main():
 
     gm = gpu.GpuModule(arena_bytes=z0.block.nbytes + (64 << 20))
     gm.set_kernel(args.kernel)
-    gm.set_gen_options(reg_window=args.reg_window, waves_per_eu=args.waves_per_eu, batch=args.batch)
+    gm.set_gen_options(reg_window=args.reg_window, waves_per_eu=args.waves_per_eu, batch=args.batch,
+                       sweeps=args.sweeps)
     gm.set_grid(args.grid)
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z0.copy())
@@ -92,7 +95,7 @@ def main():
     # Warmup: advance from the initial state, then restore it on the device.
     dz.upload()
     if args.warmup:
-        dz.advance(1.0, repeats=args.warmup)
+        dz.advance(1.0, repeats=args.warmup, precision=args.precision)
     dz.z.block[:] = z0.block
     dz.upload()
 
@@ -103,9 +106,9 @@ def main():
     t0 = time.perf_counter()
     if args.separate:
         for _ in range(args.steps):
-            kernel_ms.append(dz.advance(1.0))
+            kernel_ms.append(dz.advance(1.0, precision=args.precision))
     else:
-        kernel_ms.append(dz.advance(1.0, repeats=args.steps))
+        kernel_ms.append(dz.advance(1.0, repeats=args.steps, precision=args.precision))
     gm.lib.gpu_sync()
     wall = time.perf_counter() - t0
     dz.download()

@@ -100,7 +100,9 @@ typedef struct SimkGenOptions {
     u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
     u32 reg_window;      /* rows back a finished row stays in registers (0: 8, ~0u: all) */
     u32 batch;           /* loads issued this many statements ahead (0: 16) */
-    u32 reserved[3];
+    u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 2 mixed-precision kernel */
+    u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
+    u32 reserved;
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 

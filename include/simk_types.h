@@ -153,7 +153,8 @@ typedef struct GpuKinLaunch {
     f64 *k, *src, *pulse, *period, *pulse_first, *pulse_count;
     u32 padded;
     u32 lane_count;
-    u32 precision;
+    u32 precision;   /* 0: FP64 (bit-identical to the CPU path); 1: the reference's f32 delta lanes;
+                        2: mixed precision (FP32 Jacobian/LU, FP64 residual refinement and control) */
     f64 duration;
 } GpuKinLaunch;
 

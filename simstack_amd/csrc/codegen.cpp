@@ -120,9 +120,23 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     char name[64];
     snprintf(name, sizeof(name), "simk_gen_%016llx", (unsigned long long)sig);
     g.kernel = name;
+    const bool mixed = o.precision == 2;
+    const u32 sweeps = mixed ? o.sweeps : 0;
+    if (mixed) snprintf(name, sizeof(name), "simk_mix%u_%016llx", sweeps, (unsigned long long)sig);
+    g.kernel = name;
     const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;   // the last stage stays in registers
-    const u32 foff = m.nnz + stored_stages * n;                       // the reused f
-    g.ws_slots = foff + n;
+    // Workspace slots per lane (f64 units): FP64 mode [LU | K stages | f];
+    // mixed mode [K stages | f | refinement rhs | FP32 LU (two per slot)].
+    const u32 koff = mixed ? 0 : m.nnz;
+    const u32 foff = koff + stored_stages * n;
+    const u32 rhsoff = foff + n;
+    const u32 lu32off = rhsoff + (sweeps ? n : 0);
+    const u32 end = mixed ? lu32off + (m.nnz + 1) / 2 : foff + n;
+    // The cell's rate constants and sources are copied into the lane's slots
+    // when the lane takes the cell, so every later read is a coalesced wave
+    // transaction whatever cells the lanes of a wave hold.
+    const u32 kcp = end, scp = kcp + m.R;
+    g.ws_slots = scp + n;
 
     Out w;
     w("// generated by simstack_amd codegen: %u species, %u reactions, %u non-zeros\n", n, m.R, m.nnz);
@@ -130,30 +144,58 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w.s += "\n";
     w("struct GenOps {\n");
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
-    w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, STAGES = %u;\n", m.nnz, foff, me.stages);
+    w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u;\n", koff, foff, rhsoff,
+      me.stages);
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
+    if (mixed) w("    SimkRows32 w32;\n");
     w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
     w("        const simk_u32 stride = A.padded * 8u;\n");
     w("        kr.init(A.k, 0, stride, R * stride);\n");
     w("        sr.init(A.src, 0, stride, N * stride);\n");
     w("        pr.init(A.pulse, 0, stride, N * stride);\n");
     w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    if (mixed)
+        w("        w32.init(A.ws + ((simk_u64)wave * A.ws_slots + %u) * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, %u * SIMK_WAVE * 8u);\n",
+          lu32off, g.ws_slots - lu32off);
     w("    }\n");
-    w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) { kr.voff = sr.voff = pr.voff = cell * 8u; }\n");
+    w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
+    w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
+    w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) WS(%u + r, kr.ld(r));\n", kcp);
+    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) WS(%u + s, sr.ld(s));\n", scp);
+    w("    }\n");
     w("    static SIMK_DEV constexpr simk_u32 n() { return N; }\n");
     w("    SIMK_DEV Vec vec(int) const { return Vec(); }\n");
-    w("    SIMK_DEV double K(simk_u32 r) const { return kr.ld(r); }\n");
-    w("    SIMK_DEV double S(simk_u32 s) const { return sr.ld(s); }\n");
+    w("    SIMK_DEV double K(simk_u32 r) const { return ws.ld(%u + r); }\n", kcp);
+    w("    SIMK_DEV double S(simk_u32 s) const { return ws.ld(%u + s); }\n", scp);
     w("    SIMK_DEV double pulse(simk_u32 s) const { return pr.ld(s); }\n");
     w("    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }\n");
     w("    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }\n");
+    if (mixed) {
+        w("    SIMK_DEV float W32(simk_u32 i) const { return w32.ld(i); }\n");
+        w("    SIMK_DEV void W32S(simk_u32 i, float v) const { w32.st(i, v); }\n");
+    }
     w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(KOFF + st * N + s); }\n");
     w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
     w("        if (st + 1 >= STAGES) return;\n");
     w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(KOFF + st * N + s, x.v[s]);\n    }\n");
-    w("    SIMK_DEV double kfinal(simk_u32 st, simk_u32 s, const Vec &x) const {\n");
-    w("        return st + 1 == STAGES ? x.v[s] : kget(st, s);\n    }\n");
+    {
+        // v += c K_j, loads software-pipelined like the solve's.
+        std::vector<Op> ops;
+        char buf[96];
+        for (u32 s2 = 0; s2 < n; ++s2) {
+            Op op;
+            snprintf(buf, sizeof(buf), "W(KOFF + j * N + %u)", s2);
+            op.load = buf;
+            snprintf(buf, sizeof(buf), "v.v[%u] = simk_fma(c, %%s, v.v[%u]);", s2, s2);
+            op.stmt = buf;
+            ops.push_back(op);
+        }
+        w("    SIMK_DEV void kaxpy(simk_u32 j, double c, Vec &v) const {\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid);
+        w("    }\n");
+    }
     w("    SIMK_DEV void fsave(const Vec &x) const {\n");
     w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(FOFF + s, x.v[s]);\n    }\n");
     w("    SIMK_DEV void fload(Vec &x) const {\n");
@@ -281,79 +323,165 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
             }
             // Eliminations against finished rows k < i: register values, or
             // workspace reloads (fenced mode / beyond the register window).
+            // Mixed precision: the row is rounded to FP32 here and factored in
+            // FP32 (b = f32 entries, FP32 fma).
+            const char *V = mixed ? "b" : "a";
+            const char *T = mixed ? "float" : "double";
+            const char *LD = mixed ? "W32" : "W";
+            if (mixed)
+                for (u32 idx : row) {
+                    snprintf(buf, sizeof(buf), "float b%u = (float)a%u;", idx, idx);
+                    stmt(buf);
+                }
             for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
                 const KinLuDiv &dv = m.lu_div[d];
                 const bool kk_reg = regs && near(dv.kk, i);
                 Op op;
                 if (!kk_reg) {
-                    snprintf(buf, sizeof(buf), "W(%u)", dv.kk);
+                    snprintf(buf, sizeof(buf), "%s(%u)", LD, dv.kk);
                     op.load = buf;
-                    snprintf(buf, sizeof(buf), "const double l%u = a%u * %%s; a%u = l%u;", d, dv.ik, dv.ik, d);
+                    snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %%s; %s%u = l%u;", T, d, V, dv.ik, V, dv.ik, d);
                 } else {
-                    snprintf(buf, sizeof(buf), "const double l%u = a%u * a%u; a%u = l%u;", d, dv.ik, dv.kk, dv.ik, d);
+                    snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %s%u; %s%u = l%u;", T, d, V, dv.ik, V, dv.kk, V,
+                             dv.ik, d);
                 }
                 op.stmt = buf;
                 ops.push_back(op);
                 for (u32 su = dv.sub_begin; su < dv.sub_end; ++su) {
                     const KinLuSub &sb = m.lu_sub[su];
                     Op o2;
+                    const char *fn = mixed ? "__builtin_fmaf(-l" : "simk_fnma(l";
                     if (regs && near(sb.kj, i)) {
-                        snprintf(buf, sizeof(buf), "a%u = simk_fnma(l%u, a%u, a%u);", sb.ij, d, sb.kj, sb.ij);
+                        snprintf(buf, sizeof(buf), "%s%u = %s%u, %s%u, %s%u);", V, sb.ij, fn, d, V, sb.kj, V, sb.ij);
                     } else {
-                        snprintf(buf, sizeof(buf), "W(%u)", sb.kj);
+                        snprintf(buf, sizeof(buf), "%s(%u)", LD, sb.kj);
                         o2.load = buf;
-                        snprintf(buf, sizeof(buf), "a%u = simk_fnma(l%u, %%s, a%u);", sb.ij, d, sb.ij);
+                        snprintf(buf, sizeof(buf), "%s%u = %s%u, %%s, %s%u);", V, sb.ij, fn, d, V, sb.ij);
                     }
                     o2.stmt = buf;
                     ops.push_back(o2);
                 }
             }
-            snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
-            stmt(buf);
-            for (u32 idx : row) {
-                snprintf(buf, sizeof(buf), "WS(%u, a%u);", idx, idx);
+            if (mixed) {
+                snprintf(buf, sizeof(buf), "b%u = 1.0f / (b%u + gf);", m.diag[i], m.diag[i]);
                 stmt(buf);
+                for (u32 idx : row) {
+                    snprintf(buf, sizeof(buf), "W32S(%u, b%u);", idx, idx);
+                    stmt(buf);
+                }
+            } else {
+                snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
+                stmt(buf);
+                for (u32 idx : row) {
+                    snprintf(buf, sizeof(buf), "WS(%u, a%u);", idx, idx);
+                    stmt(buf);
+                }
             }
         }
         barrier();
         w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
+        if (mixed) w("        const float gf = (float)g;\n");
         u32 uid = 0;
         emit_ops(w, ops, o.batch, "        ", uid);
         w("    }\n");
     }
 
-    // solve
-    {
+    // solve: forward over lower, backward over upper and the reciprocal pivot.
+    auto solve_ops = [&](const char *vec, bool f32) {
         std::vector<Op> ops;
         char buf[160];
+        const char *LD = f32 ? "W32" : "W";
+        const char *FN = f32 ? "__builtin_fmaf(-%s" : "simk_fnma(%s";
         for (u32 i = 0; i < n; ++i)
             for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
-                snprintf(buf, sizeof(buf), "W(%u)", m.lower[e].index);
                 Op op;
+                snprintf(buf, sizeof(buf), "%s(%u)", LD, m.lower[e].index);
                 op.load = buf;
-                snprintf(buf, sizeof(buf), "x.v[%u] = simk_fnma(%%s, x.v[%u], x.v[%u]);", i, m.lower[e].col, i);
-                op.stmt = buf;
+                std::string f = FN;
+                snprintf(buf, sizeof(buf), "%s[%u] = ", vec, i);
+                std::string st = buf + f + ", ";
+                snprintf(buf, sizeof(buf), "%s[%u], %s[%u]);", vec, m.lower[e].col, vec, i);
+                op.stmt = st + buf;
                 ops.push_back(op);
             }
         for (u32 ii = n; ii-- > 0;) {
             for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
                 Op op;
-                snprintf(buf, sizeof(buf), "W(%u)", m.upper[e].index);
+                snprintf(buf, sizeof(buf), "%s(%u)", LD, m.upper[e].index);
                 op.load = buf;
-                snprintf(buf, sizeof(buf), "x.v[%u] = simk_fnma(%%s, x.v[%u], x.v[%u]);", ii, m.upper[e].col, ii);
-                op.stmt = buf;
+                std::string f = FN;
+                snprintf(buf, sizeof(buf), "%s[%u] = ", vec, ii);
+                std::string st = buf + f + ", ";
+                snprintf(buf, sizeof(buf), "%s[%u], %s[%u]);", vec, m.upper[e].col, vec, ii);
+                op.stmt = st + buf;
                 ops.push_back(op);
             }
             Op op;
-            snprintf(buf, sizeof(buf), "W(%u)", m.diag[ii]);
+            snprintf(buf, sizeof(buf), "%s(%u)", LD, m.diag[ii]);
             op.load = buf;
-            snprintf(buf, sizeof(buf), "x.v[%u] = x.v[%u] * %%s;", ii, ii);
+            snprintf(buf, sizeof(buf), "%s[%u] = %s[%u] * %%s;", vec, ii, vec, ii);
             op.stmt = buf;
             ops.push_back(op);
         }
-        w("    SIMK_DEV void solve(Vec &x) const {\n        SIMK_MEMORY_FENCE();\n");
+        return ops;
+    };
+    if (!mixed) {
+        w("    SIMK_DEV void solve(Vec &x, const Vec &, double) const {\n        SIMK_MEMORY_FENCE();\n");
         u32 uid = 0;
-        emit_ops(w, ops, o.batch, "        ", uid);
+        emit_ops(w, solve_ops("x.v", false), o.batch, "        ", uid);
+        w("    }\n};\n\n");
+    } else {
+        // FP32 triangular solves on a float vector.
+        w("    SIMK_DEV void solve32(float *z) const {\n        SIMK_MEMORY_FENCE();\n");
+        u32 uid = 0;
+        emit_ops(w, solve_ops("z", true), o.batch, "        ", uid);
+        w("    }\n");
+        // r += J(y) x in FP64, matrix-free from the reactions (d = k prod_{q!=p} y_q).
+        std::vector<Op> jx;
+        char buf[200];
+        for (u32 r = 0; r < m.R; ++r) {
+            const Reaction &x = m.rx[r];
+            Op op;
+            snprintf(buf, sizeof(buf), "K(%u)", r);
+            op.load = buf;
+            std::string st = "{ const double kr = %s;";
+            for (u32 p = 0; p < x.reactant_count; ++p) {
+                st += " { const double t = kr";
+                for (u32 q = 0; q < x.reactant_count; ++q)
+                    if (q != p) {
+                        snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[q]);
+                        st += buf;
+                    }
+                snprintf(buf, sizeof(buf), " * x.v[%u];", x.reactant[p]);
+                st += buf;
+                for (u32 e = 0; e < x.net_count; ++e) {
+                    snprintf(buf, sizeof(buf), " r[%u] = simk_fma(%d.0, t, r[%u]);", x.net_species[e],
+                             (int)x.net_coef[e], x.net_species[e]);
+                    st += buf;
+                }
+                st += " }";
+            }
+            st += " }";
+            op.stmt = st;
+            jx.push_back(op);
+        }
+        w("    SIMK_DEV void solve(Vec &x, const Vec &y, const double g) const {\n");
+        if (sweeps) w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(RHSOFF + s, x.v[s]);\n");
+        w("        float z[N];\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) z[s] = (float)x.v[s];\n");
+        w("        solve32(z);\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = (double)z[s];\n");
+        if (sweeps) {
+            w("#pragma unroll 1\n        for (simk_u32 sweep = 0; sweep < %u; ++sweep) {\n", sweeps);
+            w("            double r[N];\n");
+            w("#pragma unroll\n            for (simk_u32 s = 0; s < N; ++s) r[s] = simk_fnma(g, x.v[s], W(RHSOFF + s));\n");
+            u32 uid2 = 0;
+            emit_ops(w, jx, o.batch, "            ", uid2);
+            w("#pragma unroll\n            for (simk_u32 s = 0; s < N; ++s) z[s] = (float)r[s];\n");
+            w("            solve32(z);\n");
+            w("#pragma unroll\n            for (simk_u32 s = 0; s < N; ++s) x.v[s] = x.v[s] + (double)z[s];\n");
+            w("        }\n");
+        }
         w("    }\n};\n\n");
     }
 

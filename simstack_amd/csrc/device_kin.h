@@ -144,6 +144,24 @@ struct SimkRows {
     }
 };
 
+// The same for f32 arrays (lane stride 4 bytes): the mixed-precision LU.
+struct SimkRows32 {
+    __amdgpu_buffer_rsrc_t rs;
+    simk_u32 voff;
+    simk_u32 stride;
+    SIMK_DEV void init(const void *base, simk_u32 voff_bytes, simk_u32 row_stride_bytes, simk_u32 bytes) {
+        rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)bytes, SIMK_RSRC_FLAGS);
+        voff = voff_bytes;
+        stride = row_stride_bytes;
+    }
+    SIMK_DEV float ld(simk_u32 row) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)(row * stride), 0));
+    }
+    SIMK_DEV void st(simk_u32 row, float x) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(simk_u32, x), rs, (int)voff, (int)(row * stride), 0);
+    }
+};
+
 // A species vector in registers (generated kernels: N is a compile-time constant).
 template <simk_u32 N>
 struct SimkRegVec {
@@ -164,11 +182,13 @@ struct SimkWsVec {
 // The integrator, written once over an Ops policy P:
 //   P::Vec                 per-lane species vector (get/set)
 //   p.n()                  species count (constexpr in generated code)
-//   p.vec(i)               storage for vector i (0 y, 2 stage point / y_new, 3 stage solution)
+//   p.vec(i)               storage for vector i (0 y, 2 stage point / y_new, 3 stage solution, 4 error)
 //   p.fun(y, f)            f = src + sum_r net_r k_r prod y         (kin_solver.h:39-53)
 //   p.jac_factor(y, g)     LU of (I g - J(y)) into the workspace    (:75-113)
-//   p.solve(x)             x = (LU)^-1 x                             (:122-139)
-//   p.kget(j, s), p.kstore(st, x), p.kfinal(st, s, x)   stage vectors K_st
+//   p.solve(x, y, g)       x = (LU)^-1 x (:122-139); the mixed-precision policy
+//                          also refines against (I g - J(y)) in FP64
+//   p.kstore(st, x)        keeps stage vector K_st (the last stage stays in x)
+//   p.kaxpy(j, c, v)       v += c K_j, elementwise fma
 //   p.fsave(x), p.fload(x) the f a later stage reuses
 //   p.pulse(s)             the cell's pulse increment
 // and a method policy M (compile-time constants in generated code).
@@ -200,7 +220,9 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
     typename P::Vec x = p.vec(3);
     // One stage: its point, f there (or the saved f), the c-combination of
     // earlier stages, the solve. Rolled by default so the solve and f are
-    // emitted once; the generator may unroll it.
+    // emitted once; the generator may unroll it. Every vector update runs
+    // over stages j in ascending order for each element, as in the
+    // reference's loops (kin_solver.h:238-259); only the loop nesting differs.
     auto stage = [&](const simk_u32 st) {
         const simk_u32 packed = st * (st - 1) / 2;
         if (st == 0 || me.eval(st)) {
@@ -209,9 +231,7 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
             for (simk_u32 s = 0; s < n; ++s) pt.set(s, y.get(s));
             for (simk_u32 j = 0; j < st; ++j) {
                 const double a = me.a(packed + j);
-                if (a == 0.0) continue;
-#pragma unroll
-                for (simk_u32 s = 0; s < n; ++s) pt.set(s, simk_fma(a, p.kget(j, s), pt.get(s)));
+                if (a != 0.0) p.kaxpy(j, a, pt);
             }
             p.fun(pt, x);
             if (me.save_f(st)) p.fsave(x);
@@ -220,12 +240,9 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
         }
         for (simk_u32 j = 0; j < st; ++j) {
             const double c = me.c(packed + j);
-            if (c == 0.0) continue;
-            const double hc = c * hinv;
-#pragma unroll
-            for (simk_u32 s = 0; s < n; ++s) x.set(s, simk_fma(hc, p.kget(j, s), x.get(s)));
+            if (c != 0.0) p.kaxpy(j, c * hinv, x);
         }
-        p.solve(x);
+        p.solve(x, y, ghinv);
         p.kstore(st, x);
     };
     if constexpr (M::kUnrollStages) {
@@ -237,22 +254,35 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
         for (simk_u32 st = 0; st < me.stages(); ++st) stage(st);
     }
 
+    // y_new = y + sum m_st K_st and the error estimate sum e_st K_st (stages in
+    // ascending order per element; kin_solver.h:264-277). The last stage's K
+    // is still in x. The error vector is only materialised when an earlier
+    // stage contributes to it.
+    const simk_u32 S = me.stages();
     typename P::Vec ynew = p.vec(2);
+#pragma unroll
+    for (simk_u32 s = 0; s < n; ++s) ynew.set(s, y.get(s));
+    bool early_e = false;
+    for (simk_u32 st = 0; st + 1 < S; ++st) early_e = early_e || me.e(st) != 0.0;
+    typename P::Vec yerr = p.vec(4);
+    if (early_e) {
+#pragma unroll
+        for (simk_u32 s = 0; s < n; ++s) yerr.set(s, 0.0);
+    }
+    for (simk_u32 st = 0; st + 1 < S; ++st) {
+        if (me.m(st) != 0.0) p.kaxpy(st, me.m(st), ynew);
+        if (me.e(st) != 0.0) p.kaxpy(st, me.e(st), yerr);
+    }
+    const double m_last = me.m(S - 1), e_last = me.e(S - 1);
     double err = 0.0;
 #pragma unroll
     for (simk_u32 s = 0; s < n; ++s) {
-        double yn = y.get(s), yerr = 0.0;
-#pragma unroll
-        for (simk_u32 st = 0; st < 4; ++st) {
-            if (st >= me.stages()) break;
-            const double mc = me.m(st), ec = me.e(st);
-            if (mc == 0.0 && ec == 0.0) continue;
-            const double kv = p.kfinal(st, s, x);
-            if (mc != 0.0) yn = simk_fma(mc, kv, yn);
-            if (ec != 0.0) yerr = simk_fma(ec, kv, yerr);
-        }
+        double yn = ynew.get(s);
+        if (m_last != 0.0) yn = simk_fma(m_last, x.get(s), yn);
+        double ye = early_e ? yerr.get(s) : 0.0;
+        if (e_last != 0.0) ye = simk_fma(e_last, x.get(s), ye);
         const double scale = simk_fma(ctl.rtol, simk_max(simk_abs(y.get(s)), simk_abs(yn)), ctl.atol);
-        const double q = yerr / scale;
+        const double q = ye / scale;
         err = simk_fma(q, q, err);
         ynew.set(s, yn);
     }

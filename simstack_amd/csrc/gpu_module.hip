@@ -47,8 +47,8 @@ struct TableOps {
                 A.ws_slots * SIMK_WAVE * 8u);
     }
     SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) { kr.voff = sr.voff = pr.voff = cell * 8u; }
-    // Workspace slots: [0, nnz) LU; vectors i = 0..3 at nnz + i n; K_st at
-    // nnz + (4 + st) n; the reused f at nnz + 8 n.
+    // Workspace slots: [0, nnz) LU; vectors i = 0..4 at nnz + i n; K_st at
+    // nnz + (5 + st) n; the reused f at nnz + 9 n.
     SIMK_DEV simk_u32 n() const { return n_; }
     SIMK_DEV Vec vec(int i) const { return Vec{ws, nnz_ + (simk_u32)i * n_}; }
     SIMK_DEV double K(simk_u32 r) const { return kr.ld(r); }
@@ -56,17 +56,19 @@ struct TableOps {
     SIMK_DEV double pulse(simk_u32 s) const { return pr.ld(s); }
     SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }
     SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }
-    SIMK_DEV simk_u32 koff(simk_u32 st) const { return nnz_ + (4 + st) * n_; }
+    SIMK_DEV simk_u32 koff(simk_u32 st) const { return nnz_ + (5 + st) * n_; }
     SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(koff(st) + s); }
     SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {
         for (simk_u32 s = 0; s < n_; ++s) WS(koff(st) + s, x.get(s));
     }
-    SIMK_DEV double kfinal(simk_u32 st, simk_u32 s, const Vec &) const { return kget(st, s); }
+    SIMK_DEV void kaxpy(simk_u32 j, double c, const Vec &v) const {
+        for (simk_u32 s = 0; s < n_; ++s) v.set(s, simk_fma(c, kget(j, s), v.get(s)));
+    }
     SIMK_DEV void fsave(const Vec &x) const {
-        for (simk_u32 s = 0; s < n_; ++s) WS(nnz_ + 8 * n_ + s, x.get(s));
+        for (simk_u32 s = 0; s < n_; ++s) WS(nnz_ + 9 * n_ + s, x.get(s));
     }
     SIMK_DEV void fload(const Vec &x) const {
-        for (simk_u32 s = 0; s < n_; ++s) x.set(s, W(nnz_ + 8 * n_ + s));
+        for (simk_u32 s = 0; s < n_; ++s) x.set(s, W(nnz_ + 9 * n_ + s));
     }
 
     // kin_fun (kin_solver.h:39-53)
@@ -114,7 +116,7 @@ struct TableOps {
         }
     }
     // kin_solve (kin_solver.h:122-139)
-    SIMK_DEV void solve(const Vec &x) const {
+    SIMK_DEV void solve(const Vec &x, const Vec &, double) const {
         for (simk_u32 i = 0; i < n_; ++i)
             for (simk_u32 e = T->lower_begin[i]; e < T->lower_begin[i + 1]; ++e)
                 x.set(i, simk_fnma(W(T->lower[2 * e]), x.get(T->lower[2 * e + 1]), x.get(i)));
@@ -191,6 +193,7 @@ struct JitKernel {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     u32 ws_slots = 0;
+    int blocks_per_cu = 1;
     std::string name;
     bool cached = false;
     f64 ms = 0.0;
@@ -215,6 +218,9 @@ struct State {
     void *d_tables = nullptr;   // SimkDevTables followed by its arrays
     JitKernel jit;
     bool jit_ok = false;
+    JitKernel jit_mixed;       // compiled on the first mixed-precision launch
+    bool jit_mixed_ok = false;
+    u32 sweeps = 1;
     std::string jit_log;
     // workspace and queue
     double *ws = nullptr;
@@ -228,6 +234,7 @@ struct State {
     u32 last_kernel = 0;
     u32 last_slots = 0;
     u32 last_waves = 0;
+    std::string last_name;
     f64 last_ms = 0.0;
 };
 
@@ -333,9 +340,12 @@ bool jit_code(const GenResult &gr, std::vector<char> &code, bool &cached, std::s
 void free_mech_resources() {
     if (g.d_tables) hipFree(g.d_tables);
     g.d_tables = nullptr;
-    if (g.jit.mod) hipModuleUnload(g.jit.mod);
+    if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
     g.jit = JitKernel();
     g.jit_ok = false;
+    if (g.jit_mixed.mod) (void)hipModuleUnload(g.jit_mixed.mod);
+    g.jit_mixed = JitKernel();
+    g.jit_mixed_ok = false;
 }
 
 template <class T>
@@ -467,6 +477,37 @@ hipError_t ensure_ws(u64 bytes) {
     return hipSuccess;
 }
 
+// Generates, compiles (or loads from the cache) and loads one kernel variant
+// of the current mechanism.
+bool load_generated(const GenOptions &opt, JitKernel &out) {
+    auto t0 = std::chrono::steady_clock::now();
+    GenResult gr = generate(g.mech, g.method, opt, kDeviceKinSource);
+    std::vector<char> code;
+    bool cached = false, ok = false;
+    if (jit_code(gr, code, cached, g.jit_log)) {
+        hipModule_t mod;
+        if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
+            hipFunction_t fn;
+            if (hipModuleGetFunction(&fn, mod, gr.kernel.c_str()) == hipSuccess) {
+                int blocks = 0;
+                if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, SIMK_WAVE, 0) != hipSuccess)
+                    blocks = 1;
+                out.mod = mod;
+                out.fn = fn;
+                out.ws_slots = gr.ws_slots;
+                out.name = gr.kernel;
+                out.cached = cached;
+                out.blocks_per_cu = blocks;
+                ok = true;
+            } else {
+                (void)hipModuleUnload(mod);
+            }
+        }
+    }
+    out.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return ok;
+}
+
 s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *control) {
     if (!g.ready) return err_code(hipErrorNotInitialized);
     if (!method || !control) return err_code(hipErrorInvalidValue);
@@ -501,34 +542,8 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     g.have_mech = true;
 
     // Generated straight-line kernel (compiled or loaded from the disk cache).
-    {
-        auto t0 = std::chrono::steady_clock::now();
-        GenResult gr = generate(m, g.method, g.gen, kDeviceKinSource);
-        std::vector<char> code;
-        bool cached = false;
-        if (jit_code(gr, code, cached, g.jit_log)) {
-            hipModule_t mod;
-            if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
-                hipFunction_t fn;
-                if (hipModuleGetFunction(&fn, mod, gr.kernel.c_str()) == hipSuccess) {
-                    int blocks = 0;
-                    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, SIMK_WAVE, 0) != hipSuccess)
-                        blocks = 1;
-                    g.blocks_per_cu_gen = blocks;
-                    g.jit.mod = mod;
-                    g.jit.fn = fn;
-                    g.jit.ws_slots = gr.ws_slots;
-                    g.jit.name = gr.kernel;
-                    g.jit.cached = cached;
-                    g.jit_ok = true;
-                } else {
-                    hipModuleUnload(mod);
-                }
-            }
-        }
-        g.jit.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (!g.jit_ok && g.kernel_kind == 1) return err_code(hipErrorInvalidImage);
-    }
+    g.jit_ok = load_generated(g.gen, g.jit);
+    if (!g.jit_ok && g.kernel_kind == 1) return err_code(hipErrorInvalidImage);
     return 0;
 }
 
@@ -542,17 +557,31 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     if (!L->y || !L->h || !L->steps || !L->rejects || !L->pulses || !L->failed || !L->k || !L->src || !L->pulse ||
         !L->period || !L->pulse_first || !L->pulse_count)
         return hipErrorInvalidValue;
-    if (L->precision != 0) return hipErrorNotSupported;
+    if (L->precision != 0 && L->precision != 2) return hipErrorNotSupported;
     if (g.mode != 0) return hipErrorNotSupported;
-    if (g.kernel_kind == 1 && !g.jit_ok) return hipErrorInvalidImage;
-    const bool use_gen = g.jit_ok && g.kernel_kind != 2;
-    const u32 slots = use_gen ? g.jit.ws_slots : g.mech.nnz + 9 * g.mech.n;
+    const JitKernel *jk = nullptr;
+    if (L->precision == 2) {
+        // Mixed precision exists only as a generated kernel.
+        if (!g.jit_mixed_ok) {
+            GenOptions mo = g.gen;
+            mo.precision = 2;
+            mo.sweeps = g.sweeps;
+            g.jit_mixed_ok = load_generated(mo, g.jit_mixed);
+            if (!g.jit_mixed_ok) return hipErrorInvalidImage;
+        }
+        jk = &g.jit_mixed;
+    } else {
+        if (g.kernel_kind == 1 && !g.jit_ok) return hipErrorInvalidImage;
+        if (g.jit_ok && g.kernel_kind != 2) jk = &g.jit;
+    }
+    const bool use_gen = jk != nullptr;
+    const u32 slots = use_gen ? jk->ws_slots : g.mech.nnz + 10 * g.mech.n;
     // Buffer resources address each [row][padded] array with 32-bit offsets.
     const u64 rows = g.mech.R > g.mech.n ? g.mech.R : g.mech.n;
     if (rows * L->padded * 8ull >= (1ull << 31) || (u64)slots * SIMK_WAVE * 8ull >= (1ull << 31))
         return hipErrorInvalidValue;
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
-    const int per_cu = use_gen ? g.blocks_per_cu_gen : g.blocks_per_cu_table;
+    const int per_cu = use_gen ? jk->blocks_per_cu : g.blocks_per_cu_table;
     u32 resident = (u32)(per_cu > 0 ? per_cu : 1) * (g.cus ? g.cus : 1);
     if (g.grid_override) resident = g.grid_override;
     const u32 waves = needed < resident ? needed : resident;
@@ -581,13 +610,14 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
     A.method = g.dev_method;
-    g.last_kernel = use_gen ? 1 : 2;
+    g.last_kernel = use_gen ? (L->precision == 2 ? 3 : 1) : 2;
+    g.last_name = use_gen ? jk->name : std::string("simk_table_kernel");
     g.last_slots = slots;
     g.last_waves = waves;
     if (use_gen) {
         size_t size = sizeof(A);
         void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
-        return hipModuleLaunchKernel(g.jit.fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+        return hipModuleLaunchKernel(jk->fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
     }
     hipLaunchKernelGGL(simk_table_kernel, dim3(waves), dim3(SIMK_WAVE), 0, g.stream, A);
     return hipGetLastError();
@@ -792,6 +822,13 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.unroll_stages = o->unroll_stages;
     g.gen.reg_window = o->reg_window == 0 ? 8u : (o->reg_window == 0xFFFFFFFFu ? 0u : o->reg_window);
     g.gen.batch = o->batch == 0 ? 16u : o->batch;
+    const u32 sw = o->sweeps == 0 ? 1u : (o->sweeps == 0xFFFFFFFFu ? 0u : o->sweeps);
+    if (sw != g.sweeps && g.jit_mixed.mod) {
+        (void)hipModuleUnload(g.jit_mixed.mod);
+        g.jit_mixed = JitKernel();
+        g.jit_mixed_ok = false;
+    }
+    g.sweeps = sw;
     return 0;
 }
 
@@ -811,7 +848,7 @@ GPU_API s32 simk_kin_info(SimkKinInfo *out) {
     out->last_kernel_ms = g.last_ms;
     out->signature = g.sig;
     snprintf(out->kernel_name, sizeof(out->kernel_name), "%s",
-             g.last_kernel == 2 ? "simk_table_kernel" : g.jit.name.c_str());
+             g.last_name.empty() ? g.jit.name.c_str() : g.last_name.c_str());
     return 0;
 }
 
@@ -827,6 +864,8 @@ GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const Ki
         o.unroll_stages = options->unroll_stages;
         o.reg_window = options->reg_window == 0 ? 8u : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
         o.batch = options->batch == 0 ? 16u : options->batch;
+        o.precision = options->precision;
+        o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
     }
     KinMethod me = *method;
     me.name = nullptr;

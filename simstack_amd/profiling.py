@@ -1,0 +1,103 @@
+"""Summaries of rocprofv3 runs (kernel trace + HBM byte counters) for profiles/.
+
+    python -m simstack_amd.profiling --name NAME --cmd "COMMAND" \\
+        --trace KT.db --fetch F.db --write W.db --copy-fetch CF.db --copy-write CW.db \\
+        [--l2 L2.db] [--sq SQ.db] --kernel simk_gen --copy-bytes 1073741824 \\
+        --out profiles/NAME.md [--traffic-json profiles/traffic_NAME.json]
+
+FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by what the same
+counters read for the module's plain copy kernel of known size
+(simstack_amd.calibrate), so traffic is in bytes.
+"""
+import argparse
+import json
+import sqlite3
+
+
+def _q(db, sql):
+    c = sqlite3.connect(db)
+    try:
+        return c.execute(sql).fetchall()
+    finally:
+        c.close()
+
+
+def kernel_durations(db, prefix):
+    rows = _q(db, "select name, duration from kernels")
+    return [d for n, d in rows if n.startswith(prefix)]
+
+
+def counter(db, prefix, name):
+    rows = _q(db, "select kernel_name, counter_name, value, dispatch_id from counters_collection")
+    per = {}
+    for k, c, v, d in rows:
+        if k.startswith(prefix) and c == name:
+            per[d] = per.get(d, 0.0) + v
+    return [per[d] for d in sorted(per)]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--name", required=True)
+    ap.add_argument("--cmd", required=True)
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--copy-fetch", required=True)
+    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--l2")
+    ap.add_argument("--sq")
+    ap.add_argument("--kernel", default="simk_gen")
+    ap.add_argument("--algo-flops-per-launch", type=float, default=0.0)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--traffic-json")
+    a = ap.parse_args()
+
+    dur = kernel_durations(a.trace, a.kernel)
+    avg_ns = sum(dur) / len(dur)
+    cf = counter(a.copy_fetch, "simk_copy_kernel", "FETCH_SIZE")
+    cw = counter(a.copy_write, "simk_copy_kernel", "WRITE_SIZE")
+    f_scale = a.copy_bytes / (min(cf) * 1024.0)
+    w_scale = a.copy_bytes / (min(cw) * 1024.0)
+    fetch = counter(a.fetch, a.kernel, "FETCH_SIZE")
+    write = counter(a.write, a.kernel, "WRITE_SIZE")
+    rd = sum(fetch) / len(fetch) * 1024.0 * f_scale
+    wr = sum(write) / len(write) * 1024.0 * w_scale
+    lines = [f"# {a.name}", "", "Command (kernel trace; the counter passes run the same command under",
+             "`rocprofv3 --pmc FETCH_SIZE`, `--pmc WRITE_SIZE`, ...):", "", "```",
+             f"rocprofv3 --kernel-trace --stats -d <dir> -o kt -- {a.cmd}", "```", "",
+             "Calibration: `rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -- python3 -m simstack_amd.calibrate "
+             f"--mib {a.copy_bytes >> 20} --reps 2` (a copy of {a.copy_bytes} bytes reads and writes that many).", "",
+             "| quantity | value |", "|---|---|",
+             f"| kernel `{a.kernel}*` dispatches | {len(dur)} |",
+             f"| average duration per dispatch | {avg_ns / 1e6:.3f} ms |",
+             f"| FETCH_SIZE scale (copy kernel) | {f_scale:.3f} |",
+             f"| WRITE_SIZE scale (copy kernel) | {w_scale:.3f} |",
+             f"| HBM bytes read per dispatch | {rd:.4e} |",
+             f"| HBM bytes written per dispatch | {wr:.4e} |",
+             f"| HBM traffic per dispatch | {rd + wr:.4e} |",
+             f"| HBM bandwidth (traffic / avg duration) | {(rd + wr) / avg_ns:.1f} GB/s |"]
+    if a.algo_flops_per_launch:
+        lines.append(f"| algorithmic FP64 flops per dispatch | {a.algo_flops_per_launch:.4e} |")
+        lines.append(f"| achieved FP64 | {a.algo_flops_per_launch / avg_ns / 1e3:.3f} TFLOP/s |")
+    if a.l2:
+        hit = sum(counter(a.l2, a.kernel, "TCC_HIT_sum"))
+        miss = sum(counter(a.l2, a.kernel, "TCC_MISS_sum"))
+        lines.append(f"| L2 hit rate | {hit / max(hit + miss, 1):.3f} |")
+    if a.sq:
+        for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_WAVES",
+                  "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY"):
+            v = counter(a.sq, a.kernel, c)
+            if v:
+                lines.append(f"| {c} (sum per dispatch) | {sum(v) / len(v):.4e} |")
+    open(a.out, "w").write("\n".join(lines) + "\n")
+    if a.traffic_json:
+        json.dump({"kernel": a.kernel, "bytes_per_launch": rd + wr, "bytes_read": rd, "bytes_written": wr,
+                   "avg_duration_ms": avg_ns / 1e6, "fetch_scale": f_scale, "write_scale": w_scale,
+                   "command": a.cmd}, open(a.traffic_json, "w"), indent=1)
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()

@@ -54,6 +54,62 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration):
                       f"({int(steps)} accepted steps, {wall:.1f} s wall)"}
 
 
+def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
+                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0):
+    """Advances `cells` cells of the workload `steps` x 1 s from the seeded initial
+    state (after `warmup` untimed seconds on a copy). Returns timings and the block."""
+    z0 = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
+    gm = run_workload.gm
+    if gm is None:
+        gm = run_workload.gm = gpu.GpuModule(arena_bytes=1 << 30)
+    gm.arena_reset()
+    need = z0.block.nbytes + (64 << 20)
+    if need > gm.info.arena_bytes:
+        gm = run_workload.gm = gpu.GpuModule(arena_bytes=need)
+    gm.set_kernel(kernel)
+    gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps)
+    gm.set_grid(grid)
+    gm.set_mech(mech, meth, ctl)
+    dz = gpu.DeviceZones(gm, z0.copy())
+    dz.upload()
+    if warmup:
+        dz.advance(1.0, repeats=warmup, precision=precision)
+    dz.z.block[:] = z0.block
+    dz.upload()
+    kernel_ms = []
+    gm.lib.gpu_sync()
+    t0 = time.perf_counter()
+    if separate:
+        for _ in range(steps):
+            kernel_ms.append(dz.advance(1.0, precision=precision))
+    else:
+        kernel_ms.append(dz.advance(1.0, repeats=steps, precision=precision))
+    gm.lib.gpu_sync()
+    wall = time.perf_counter() - t0
+    dz.download()
+    z = dz.z
+    acc = float((z.steps[:cells] - z0.steps[:cells]).sum())
+    att = acc + float((z.rejects[:cells] - z0.rejects[:cells]).sum())
+    info = gm.kin_info()
+    return {"wall": wall, "kernel_ms": kernel_ms, "accepted": acc, "attempts": att, "zones": z,
+            "failed": int(z.failed[:cells].sum()), "kernel": info.kernel_name.decode(), "waves": info.waves}
+
+
+run_workload.gm = None
+
+
+def rel_err_groups(got, ref, atol, rtol):
+    """Max |got - ref| / max(|got|, |ref|) (test_kinetics.cpp:35-38) over entries
+    above atol/rtol, and per concentration group of the reference."""
+    d = np.abs(got - ref)
+    s = np.maximum(np.abs(got), np.abs(ref))
+    out = {}
+    for name, mask in (("above atol/rtol", ref > atol / rtol), ("1e-9..1e-6", (ref > 1e-9) & (ref <= 1e-6)),
+                       ("1e-11..1e-9", (ref > 1e-11) & (ref <= 1e-9))):
+        out[name] = float((d[mask] / s[mask]).max()) if mask.any() else 0.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +118,8 @@ def main():
     ap.add_argument("--cells", type=int, default=65536)
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 262,144-cell mixed vs FP64 comparison")
+    ap.add_argument("--mixed-cells", type=int, default=262144)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generated, 2 table walk")
     ap.add_argument("--separate", action="store_true",
                     help="one gpu_kin_advance launch per step instead of one fused simk_kin_advance_n launch")
@@ -70,7 +128,7 @@ def main():
     ap.add_argument("--waves-per-eu", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64 (bit-exact), 2 mixed precision")
-    ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1)")
+    ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1; ~0u: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (cells shard with no "
@@ -82,49 +140,25 @@ def main():
     mech = problem.Mechanism.synthetic(sp)
     meth, ctl = problem.method(2), problem.control()
     flops_attempt = mech.flops_per_attempt()
-    z0 = problem.Zones(mech.n, mech.R, args.cells).fill_synthetic(sp, mech.params)
-
-    gm = gpu.GpuModule(arena_bytes=z0.block.nbytes + (64 << 20))
-    gm.set_kernel(args.kernel)
-    gm.set_gen_options(reg_window=args.reg_window, waves_per_eu=args.waves_per_eu, batch=args.batch,
-                       sweeps=args.sweeps)
-    gm.set_grid(args.grid)
-    gm.set_mech(mech, meth, ctl)
-    dz = gpu.DeviceZones(gm, z0.copy())
-
-    # Warmup: advance from the initial state, then restore it on the device.
-    dz.upload()
-    if args.warmup:
-        dz.advance(1.0, repeats=args.warmup, precision=args.precision)
-    dz.z.block[:] = z0.block
-    dz.upload()
-
-    steps0 = z0.steps[:args.cells].copy()
-    rej0 = z0.rejects[:args.cells].copy()
-    kernel_ms = []
-    gm.lib.gpu_sync()
-    t0 = time.perf_counter()
-    if args.separate:
-        for _ in range(args.steps):
-            kernel_ms.append(dz.advance(1.0, precision=args.precision))
-    else:
-        kernel_ms.append(dz.advance(1.0, repeats=args.steps, precision=args.precision))
-    gm.lib.gpu_sync()
-    wall = time.perf_counter() - t0
-    dz.download()
-    z = dz.z
-    acc = float((z.steps[:args.cells] - steps0).sum())
-    att = acc + float((z.rejects[:args.cells] - rej0).sum())
-    failed = int(z.failed[:args.cells].sum())
-    info = gm.kin_info()
-
+    knobs = dict(kernel=args.kernel, grid=args.grid, reg_window=args.reg_window, waves_per_eu=args.waves_per_eu,
+                 batch=args.batch)
+    r = run_workload(gpu, problem, mech, sp, meth, ctl, args.cells, args.steps, args.warmup, args.precision,
+                     args.sweeps, args.separate, **knobs)
+    kernel_ms = r["kernel_ms"]
+    wall, acc, att = r["wall"], r["accepted"], r["attempts"]
     avg_ms = float(np.mean(kernel_ms))
     flops_per_launch = att * flops_attempt / len(kernel_ms)
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    traffic = None
+    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": None, "kernel_ms_avg": avg_ms,
+            "flops_per_attempt": flops_attempt}
     tpath = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    if os.path.exists(tpath) and args.precision == 0 and args.cells == 65536 and not args.separate \
+            and args.steps == 10:
+        t = json.load(open(tpath))
+        roof["traffic"] = t["bytes_per_launch"]
+        roof["hbm_GBps"] = t["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+        roof["hbm_frac"] = roof["hbm_GBps"] / PEAK_HBM_GBS
 
     out = {
         "metric": "cell-steps/s at rtol 1e-6 (FP64, 1 GPU)",
@@ -137,20 +171,34 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == 0 else "f32 LU / f64 state",
         "data": "synthetic (seeded mechanism and cells)",
         "config": {"workload": "48 species / 150 reactions, 65,536 cells, 10 x 1 s operator-split, Rodas3, "
                                "rtol 1e-6 atol 1e-12", "cells": args.cells, "species": mech.n,
                    "reactions": mech.R, "nnz": mech.nnz, "lu_div": mech.lu_div, "lu_sub": mech.lu_sub,
-                   "kernel": info.kernel_name.decode(), "waves": info.waves, "launches": len(kernel_ms),
+                   "kernel": r["kernel"], "waves": r["waves"], "launches": len(kernel_ms),
                    "parallelism": "1 GPU, one cell per lane, persistent lane-refilling kernel"},
         "attempts_per_s": att / wall,
         "accepted_steps": acc,
-        "failed_cells": failed,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": traffic,
-                     "kernel_ms_avg": avg_ms, "flops_per_attempt": flops_attempt},
+        "failed_cells": r["failed"],
+        "roofline": roof,
     }
+    if not args.no_mixed and args.precision == 0:
+        f64r = run_workload(gpu, problem, mech, sp, meth, ctl, args.mixed_cells, args.steps, args.warmup, 0, 0,
+                            False, **knobs)
+        mix = {}
+        for name, sw in (("mixed_norefine", 0xFFFFFFFF), ("mixed_1sweep", 1)):
+            mr = run_workload(gpu, problem, mech, sp, meth, ctl, args.mixed_cells, args.steps, args.warmup, 2, sw,
+                              False, **knobs)
+            n = args.mixed_cells
+            mix[name] = {"value": mr["accepted"] / mr["wall"], "vs_fp64": (mr["accepted"] / mr["wall"]) /
+                         (f64r["accepted"] / f64r["wall"]),
+                         "max_rel_err_vs_fp64": rel_err_groups(mr["zones"].y[:, :n], f64r["zones"].y[:, :n],
+                                                               ctl.atol, ctl.rtol),
+                         "steps_ratio": mr["accepted"] / f64r["accepted"], "kernel": mr["kernel"]}
+        out["mixed"] = {"workload": f"same mechanism, {args.mixed_cells} cells, 10 x 1 s fused",
+                        "fp64_value": f64r["accepted"] / f64r["wall"], **mix,
+                        "note": "errors vs the FP64 GPU path, which is bit-identical to the FP64 oracle"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
     print(json.dumps(out), flush=True)

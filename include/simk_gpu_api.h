@@ -127,6 +127,47 @@ GPU_API s32 simk_kin_info(SimkKinInfo *out);
 GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const KinMethod *method,
                                 const SimkGenOptions *options, char *log, u32 log_size);
 
+/* simk_jit_precompile for the multi-mechanism kernel of simk_kin_set_mechs
+ * (FP64; same cache). */
+GPU_API s32 simk_jit_precompile_multi(u32 count, const void *const *mechs, const SimkCaps *caps,
+                                      const KinMethod *method, const SimkGenOptions *options, char *log,
+                                      u32 log_size);
+
+/* Ragged batches: up to four mechanisms (KinMech at capacities caps[i])
+ * with one method and control, compiled into one persistent kernel. Destroys
+ * every captured graph (they launch the previous kernel). */
+GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCaps *caps, const KinMethod *method,
+                               const KinControl *control);
+/* One launch advancing every mechanism's block (launches[i] for mechanism i,
+ * FP64): waves start on a mechanism in proportion to its estimated work and
+ * move to the others as queues empty; each cell's result is that of
+ * gpu_kin_advance on its own block. `repeats` as in simk_kin_advance_n. */
+GPU_API s32 simk_kin_advance_multi(u32 count, const GpuKinLaunch *launches, u32 repeats, f32 *kernel_ms);
+/* Captures `steps` consecutive simk_kin_advance_multi sequences (queue reset
+ * kernel + persistent launch each, no memset nodes) into a HIP graph with its
+ * own workspace and queue counters (allocated here, freed by
+ * simk_graph_destroy); simk_graph_replay launches it on the module stream and
+ * waits. The graph keeps the launches' device pointers: the blocks must stay
+ * allocated while it is replayed. */
+GPU_API s32 simk_graph_capture_multi(u32 count, const GpuKinLaunch *launches, u32 repeats, u32 steps,
+                                     u32 *graph_id);
+GPU_API s32 simk_graph_replay(u32 graph_id, f32 *ms);
+GPU_API s32 simk_graph_destroy(u32 graph_id);
+
+/* Operator-split upwind advection along channels (the transport row update
+ * of nft_transport, src/sim/nft.cpp:1055-1081): fields are rows of a
+ * [field][padded] array, cell = channel * nodes + node (nodes <= 8192). Each
+ * of `substeps` substeps sets x_j <- x_j + c_j (x_{j-1} - x_j) from the old
+ * upstream value, node 0 reading inflow[field] (0 when inflow is null), as
+ * one fused multiply-add. courant: [channels * nodes]. Enqueued on the module
+ * stream (ordered with the solver launches); gpu_sync waits for it. */
+GPU_API s32 simk_upwind_f64(f64 *y, u64 padded, u32 fields, u32 channels, u32 nodes, const f64 *courant,
+                            u32 substeps, const f64 *inflow);
+/* The same in f32 with zero inflow; outlet_sum[field * channels + channel]
+ * (may be null) accumulates the outlet node's value before every substep. */
+GPU_API s32 simk_upwind_f32(f32 *x, u64 padded, u32 fields, u32 channels, u32 nodes, const f32 *courant,
+                            u32 substeps, f32 *outlet_sum);
+
 /* Measurement aid: device-to-device copy of `bytes` (16-byte multiple and
  * alignment) by a plain streaming kernel, timed with stream events. HBM
  * byte counters of this kernel calibrate the solver kernels' counters. */

@@ -96,6 +96,7 @@ typedef struct SimkSynthSpec {
     f64 oxidant_log10_spread;    /* per-cell factor 10^U[0, spread] on the first oxidant_species y0 */
     u32 oxidant_species;
     u32 pad_;
+    f64 load_log10_spread;       /* per-cell factor 10^-U[0, spread] on every initial concentration (dissolved load) */
 } SimkSynthSpec;
 
 /* Draws a mass-conserving mechanism. reactants/products: [R][3] (0xFF none);

@@ -545,10 +545,9 @@ ORACLE_API void oracle_upwind(f32 *x, const f32 *c, u32 nodes, u32 fields, u32 s
     }
 }
 
-/* The same update in f64 on the chemistry state, with an inflow value per
- * field (node 0: x_0 + c_0 (inflow - x_0); inflow 0 gives the reference's
- * x_0 - c_0 x_0 bit for bit). The plug-flow chain of dbd_flow_exchange
- * (src/sim/dbd.cpp:468-483) is the one-substep case with c_j = a. */
+/* The same fused update in f64 on the chemistry state, with an inflow value
+ * per field (node 0: x_0 + c_0 (inflow - x_0); inflow 0 gives the reference's
+ * x_0 - c_0 x_0 bit for bit). */
 ORACLE_API void oracle_upwind_f64(f64 *x, const f64 *c, u32 nodes, u32 fields, u32 stride, u32 substeps,
                                   const f64 *inflow) {
     for (u32 s = 0; s < substeps; ++s)

@@ -72,7 +72,7 @@ class SimkSynthSpec(C.Structure):
                 ("T_lo", C.c_double), ("T_hi", C.c_double), ("log10_y_lo", C.c_double), ("log10_y_hi", C.c_double),
                 ("pH_lo", C.c_double), ("pH_hi", C.c_double), ("acid_frac", C.c_double), ("base_frac", C.c_double),
                 ("water_frac", C.c_double), ("oxidant_log10_spread", C.c_double), ("oxidant_species", C.c_uint32),
-                ("pad_", C.c_uint32)]
+                ("pad_", C.c_uint32), ("load_log10_spread", C.c_double)]
 
 
 class SimkRateLaw(C.Structure):

@@ -1,0 +1,123 @@
+// Operator-split upwind film advection on the device (the transport row
+// update of nft_transport, src/sim/nft.cpp:1055-1081, and the plug-flow
+// relaxation of dbd_flow_exchange, src/sim/dbd.cpp:468-483).
+//
+// Per substep every node of a channel takes x_j + c_j (x_{j-1} - x_j) from the
+// OLD value of its upstream neighbour (node 0 reads the inflow): a Jacobi
+// update, so all nodes of a substep are independent. One workgroup owns one
+// (field, channel) row of `nodes` values: it reads the row once, keeps it in
+// registers (chunk of consecutive nodes per thread) for every substep,
+// exchanges only chunk boundaries through LDS, and writes the row once — HBM
+// traffic per launch is one read and one write of the field, whatever the
+// substep count. Rows are [field][padded] of the zone block with cell =
+// channel * nodes + node.
+#include <hip/hip_runtime.h>
+
+#include "simk_gpu_api.h"
+
+namespace {
+
+constexpr unsigned kThreads = 256;
+constexpr unsigned kMaxChunk = 32;   // nodes per thread: up to 8192 nodes per channel
+
+template <class T>
+__device__ __forceinline__ T fma_t(T a, T b, T c);
+template <>
+__device__ __forceinline__ double fma_t<double>(double a, double b, double c) {
+    return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+}
+
+template <class T, unsigned CHUNK>
+__global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned long long padded, unsigned channels,
+                                                               unsigned nodes, const T *courant, unsigned substeps,
+                                                               const T *inflow, T *outlet_sum) {
+    const unsigned row = blockIdx.x;   // field * channels + channel
+    const unsigned field = row / channels, ch = row % channels;
+    const unsigned t = threadIdx.x;
+    T *xr = x + field * padded + (unsigned long long)ch * nodes;
+    const T *cr = courant + (unsigned long long)ch * nodes;
+    __shared__ T stage[kThreads * CHUNK];
+    __shared__ T last[kThreads];
+
+    // Coalesced load of the row through LDS, then each thread takes its chunk.
+    for (unsigned j = t; j < kThreads * CHUNK; j += kThreads) stage[j] = j < nodes ? xr[j] : T(0);
+    __syncthreads();
+    T v[CHUNK], c[CHUNK];
+#pragma unroll
+    for (unsigned i = 0; i < CHUNK; ++i) v[i] = stage[t * CHUNK + i];
+    __syncthreads();
+    for (unsigned j = t; j < kThreads * CHUNK; j += kThreads) stage[j] = j < nodes ? cr[j] : T(0);
+    __syncthreads();
+#pragma unroll
+    for (unsigned i = 0; i < CHUNK; ++i) c[i] = stage[t * CHUNK + i];
+
+    const unsigned first = t * CHUNK;
+    const T in = inflow ? inflow[field] : T(0);
+    const unsigned last_thread = (nodes - 1) / CHUNK;
+    T out = (outlet_sum && t == last_thread) ? outlet_sum[row] : T(0);
+    for (unsigned s = 0; s < substeps; ++s) {
+        if (outlet_sum && t == last_thread) out = out + v[(nodes - 1) - first];
+        last[t] = v[CHUNK - 1];
+        __syncthreads();
+        const T left = t ? last[t - 1] : in;
+        // Right to left inside the chunk so each update reads the old neighbour.
+#pragma unroll
+        for (unsigned i = CHUNK; i-- > 0;) {
+            const T up = i ? v[i - 1] : left;
+            if (first + i < nodes) v[i] = fma_t<T>(c[i], up - v[i], v[i]);
+        }
+        __syncthreads();
+    }
+    if (outlet_sum && t == last_thread) outlet_sum[row] = out;
+
+#pragma unroll
+    for (unsigned i = 0; i < CHUNK; ++i) stage[t * CHUNK + i] = v[i];
+    __syncthreads();
+    for (unsigned j = t; j < nodes; j += kThreads) xr[j] = stage[j];
+}
+
+template <class T>
+hipError_t launch_upwind(hipStream_t stream, T *x, u64 padded, u32 fields, u32 channels, u32 nodes,
+                         const T *courant, u32 substeps, const T *inflow, T *outlet_sum) {
+    if (!x || !courant || fields == 0 || channels == 0 || nodes == 0) return hipErrorInvalidValue;
+    if ((u64)channels * nodes > padded) return hipErrorInvalidValue;
+    const u32 chunk_needed = (nodes + kThreads - 1) / kThreads;
+    const dim3 grid(fields * channels), block(kThreads);
+    if (chunk_needed <= 4)
+        hipLaunchKernelGGL((simk_upwind_kernel<T, 4>), grid, block, 0, stream, x, (unsigned long long)padded, channels,
+                           nodes, courant, substeps, inflow, outlet_sum);
+    else if (chunk_needed <= 16)
+        hipLaunchKernelGGL((simk_upwind_kernel<T, 16>), grid, block, 0, stream, x, (unsigned long long)padded,
+                           channels, nodes, courant, substeps, inflow, outlet_sum);
+    else if (chunk_needed <= kMaxChunk)
+        hipLaunchKernelGGL((simk_upwind_kernel<T, kMaxChunk>), grid, block, 0, stream, x, (unsigned long long)padded,
+                           channels, nodes, courant, substeps, inflow, outlet_sum);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Launches on the module stream (simk_module_stream) without waiting.
+hipStream_t simk_module_stream();
+
+extern "C" {
+
+GPU_API s32 simk_upwind_f64(f64 *y, u64 padded, u32 fields, u32 channels, u32 nodes, const f64 *courant,
+                            u32 substeps, const f64 *inflow) {
+    return (s32)launch_upwind<f64>(simk_module_stream(), y, padded, fields, channels, nodes, courant, substeps,
+                                   inflow, nullptr);
+}
+
+GPU_API s32 simk_upwind_f32(f32 *x, u64 padded, u32 fields, u32 channels, u32 nodes, const f32 *courant,
+                            u32 substeps, f32 *outlet_sum) {
+    return (s32)launch_upwind<f32>(simk_module_stream(), x, padded, fields, channels, nodes, courant, substeps,
+                                   nullptr, outlet_sum);
+}
+
+}  // extern "C"

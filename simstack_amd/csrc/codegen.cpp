@@ -113,17 +113,11 @@ void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     }
 }
 
-GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, const char *device_header) {
-    GenResult g;
+// Emits one mechanism's Ops policy struct `sname`; returns its workspace slots per lane.
+static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const GenOptions &o, const char *sname) {
     const u32 n = m.n;
-    u64 sig = signature(m);
-    char name[64];
-    snprintf(name, sizeof(name), "simk_gen_%016llx", (unsigned long long)sig);
-    g.kernel = name;
     const bool mixed = o.precision == 2;
     const u32 sweeps = mixed ? o.sweeps : 0;
-    if (mixed) snprintf(name, sizeof(name), "simk_mix%u_%016llx", sweeps, (unsigned long long)sig);
-    g.kernel = name;
     const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;   // the last stage stays in registers
     // Workspace slots per lane (f64 units): FP64 mode [LU | K stages | f];
     // mixed mode [K stages | f | refinement rhs | FP32 LU (two per slot)].
@@ -136,13 +130,8 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     // when the lane takes the cell, so every later read is a coalesced wave
     // transaction whatever cells the lanes of a wave hold.
     const u32 kcp = end, scp = kcp + m.R;
-    g.ws_slots = scp + n;
-
-    Out w;
-    w("// generated by simstack_amd codegen: %u species, %u reactions, %u non-zeros\n", n, m.R, m.nnz);
-    w.s += device_header;
-    w.s += "\n";
-    w("struct GenOps {\n");
+    const u32 ws_slots = scp + n;
+    w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
     w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u;\n", koff, foff, rhsoff,
       me.stages);
@@ -157,7 +146,7 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
     if (mixed)
         w("        w32.init(A.ws + ((simk_u64)wave * A.ws_slots + %u) * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, %u * SIMK_WAVE * 8u);\n",
-          lu32off, g.ws_slots - lu32off);
+          lu32off, ws_slots - lu32off);
     w("    }\n");
     w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
     w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
@@ -485,7 +474,11 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
         w("    }\n};\n\n");
     }
 
-    // Method with compile-time coefficients.
+    return ws_slots;
+}
+
+// Emits the method policy with compile-time coefficients.
+static void emit_method(Out &w, const KinMethod &me, const GenOptions &o) {
     u32 eval[4], save_f[4];
     stage_plan(me, eval, save_f);
     w("struct GenMethod {\n");
@@ -509,12 +502,67 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     for (int q = 0; q < 4; ++q) w("        if (i == %d) return %s;\n", q, save_f[q] ? "true" : "false");
     w("        return false;\n    }\n};\n\n");
 
+}
+
+GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, const char *device_header) {
+    GenResult g;
+    u64 sig = signature(m);
+    char name[64];
+    const bool mixed = o.precision == 2;
+    if (mixed) snprintf(name, sizeof(name), "simk_mix%u_%016llx", o.sweeps, (unsigned long long)sig);
+    else snprintf(name, sizeof(name), "simk_gen_%016llx", (unsigned long long)sig);
+    g.kernel = name;
+    Out w;
+    w("// generated by simstack_amd codegen: %u species, %u reactions, %u non-zeros\n", m.n, m.R, m.nnz);
+    w.s += device_header;
+    w.s += "\n";
+    g.ws_slots = emit_ops_struct(w, m, me, o, "GenOps");
+    emit_method(w, me, o);
     if (o.waves_per_eu)
         w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) __attribute__((amdgpu_waves_per_eu(%u))) %s(SimkKinArgs A) {\n",
           o.waves_per_eu, name);
     else
         w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) %s(SimkKinArgs A) {\n", name);
     w("    GenMethod me;\n    simk_persistent_body<GenOps, GenMethod>(A, me);\n}\n");
+    g.source = std::move(w.s);
+    return g;
+}
+
+GenResult generate_multi(const std::vector<const Mech *> &ms, const KinMethod &me, const GenOptions &o,
+                         const char *device_header) {
+    GenResult g;
+    u64 sig = 14695981039346656037ull;
+    for (const Mech *m : ms) {
+        u64 s1 = signature(*m);
+        sig = fnv1a64(&s1, 8, sig);
+    }
+    char name[64];
+    snprintf(name, sizeof(name), "simk_multi%u_%016llx", (u32)ms.size(), (unsigned long long)sig);
+    g.kernel = name;
+    Out w;
+    w("// generated by simstack_amd codegen: %u mechanisms in one persistent launch\n", (u32)ms.size());
+    w.s += device_header;
+    w.s += "\n";
+    for (size_t i = 0; i < ms.size(); ++i) {
+        char sname[32];
+        snprintf(sname, sizeof(sname), "GenOps%u", (u32)i);
+        g.mech_slots.push_back(emit_ops_struct(w, *ms[i], me, o, sname));
+        if (g.mech_slots.back() > g.ws_slots) g.ws_slots = g.mech_slots.back();
+    }
+    emit_method(w, me, o);
+    // Waves start on the mechanism whose wave range holds their index and move
+    // on to the others once its queue is empty, so no SIMD idles while any
+    // mechanism still has cells.
+    w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) %s(SimkMultiArgs MA) {\n", name);
+    w("    GenMethod me;\n    simk_u32 first = 0;\n");
+    w("    for (simk_u32 m = 1; m < MA.count; ++m)\n        if (blockIdx.x >= MA.start[m]) first = m;\n");
+    w("    for (simk_u32 pass = 0; pass < MA.count; ++pass) {\n");
+    w("        simk_u32 m = first + pass;\n        if (m >= MA.count) m -= MA.count;\n");
+    w("        switch (m) {\n");
+    for (size_t i = 0; i < ms.size(); ++i)
+        w("            case %u: simk_persistent_body<GenOps%u, GenMethod>(MA.a[%u], me); break;\n", (u32)i, (u32)i,
+          (u32)i);
+    w("            default: break;\n        }\n    }\n}\n");
     g.source = std::move(w.s);
     return g;
 }

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "mech.hpp"
 
@@ -24,12 +25,17 @@ struct GenOptions {
 struct GenResult {
     std::string source;   // complete translation unit (device header included)
     std::string kernel;   // extern "C" kernel name
-    u32 ws_slots = 0;     // workspace slots per lane the kernel uses
+    u32 ws_slots = 0;     // workspace slots per lane the kernel uses (the maximum over mechanisms)
+    std::vector<u32> mech_slots;   // multi-mechanism kernels: slots of each mechanism
 };
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);
 
 GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, const char *device_header);
+
+// One persistent kernel for up to four mechanisms (ragged batches).
+GenResult generate_multi(const std::vector<const Mech *> &ms, const KinMethod &me, const GenOptions &o,
+                         const char *device_header);
 
 }  // namespace simk

@@ -73,6 +73,7 @@ struct SimkKinArgs {
     simk_u32 ws_slots;       // slots per lane
     simk_u32 n_species, n_reactions;
     simk_u32 repeats;        // consecutive advances of `duration` per cell (h carried)
+    simk_u32 queue_first;    // 1: even the first queue positions come from the atomic (queue starts at 0)
     simk_u32 *queue;         // next unassigned queue position (starts at gridDim.x * 64)
     const simk_u32 *order;   // queue position -> cell (null: identity)
     const SimkDevTables *tables;
@@ -354,11 +355,11 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
     hs_.init(A.h, 0, 0, row_bytes);
     fs.init(A.failed, 0, 0, row_bytes);
 
-    simk_u32 pos = wave * SIMK_WAVE + lane;   // first queue position: no atomic needed
+    simk_u32 pos = wave * SIMK_WAVE + lane;   // first queue position without an atomic, unless queue_first
     simk_u32 cell = 0, phase = SIMK_PH_CELL, it = 0, rep = 0;
     double h = 0.0, t = 0.0, applied = 0.0, seg_end = 0.0, period = 0.0, first = 0.0, count = 0.0;
     double steps = 0.0, rejects = 0.0, pulses = 0.0, failed = 0.0;
-    bool pending = false, reject_last = false, reject_more = false, fresh = true;
+    bool pending = false, reject_last = false, reject_more = false, fresh = !A.queue_first;
 
     for (;;) {
         // Resolve each lane's transitions until it has an attempt to make or
@@ -459,3 +460,11 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
         }
     }
 }
+
+// Several mechanisms in one launch (ragged batches): per-mechanism arguments
+// and the first wave of each mechanism's starting range.
+struct SimkMultiArgs {
+    SimkKinArgs a[4];
+    simk_u32 count;
+    simk_u32 start[4];
+};

@@ -178,6 +178,9 @@ __global__ void __launch_bounds__(256) simk_copy_kernel(double2 *dst, const doub
 __global__ void simk_queue_reset_kernel(u32 *queue, u32 value) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *queue = value;
 }
+__global__ void simk_queues_zero_kernel(u32 *queue, u32 count) {
+    if (blockIdx.x == 0 && threadIdx.x < count) queue[threadIdx.x] = 0;
+}
 
 // ---------------------------------------------------------------------------
 // Module state (one device, one stream; calls come from one thread at a time).
@@ -222,6 +225,21 @@ struct State {
     bool jit_mixed_ok = false;
     u32 sweeps = 1;
     std::string jit_log;
+    // ragged batches: several mechanisms in one persistent launch
+    std::vector<Mech> multi_mechs;
+    std::vector<SimkDevCtl> multi_ctl;
+    JitKernel multi_jit;
+    bool multi_ok = false;
+    // captured step sequences
+    // Captured multi-mechanism steps. Each graph owns its workspace and queue
+    // counters, so later launches that grow the shared workspace never leave a
+    // graph pointing at freed memory.
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        double *ws = nullptr;
+        u32 *queue = nullptr;
+    };
+    std::vector<Graph> graphs;
     // workspace and queue
     double *ws = nullptr;
     u64 ws_bytes = 0;
@@ -623,7 +641,147 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     return hipGetLastError();
 }
 
+// Kernel arguments of one advance of a launch's block (shared by the single
+// and the multi-mechanism launches).
+SimkKinArgs make_args(const GpuKinLaunch *L, u32 lanes, const Mech &m, const SimkDevCtl &ctl, u32 slots,
+                      u32 repeats, u32 *queue, u32 queue_first) {
+    SimkKinArgs A;
+    memset(&A, 0, sizeof(A));
+    A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
+    A.k = L->k; A.src = L->src; A.pulse = L->pulse; A.period = L->period; A.pulse_first = L->pulse_first;
+    A.pulse_count = L->pulse_count;
+    A.padded = L->padded;
+    A.lanes = lanes;
+    A.duration = L->duration;
+    A.ws = g.ws;
+    A.ws_slots = slots;
+    A.n_species = m.n;
+    A.n_reactions = m.R;
+    A.repeats = repeats;
+    A.queue_first = queue_first;
+    A.queue = queue;
+    A.order = nullptr;
+    A.tables = nullptr;
+    A.ctl = ctl;
+    A.method = g.dev_method;
+    return A;
+}
+
+bool launch_ok(const GpuKinLaunch *L) {
+    return L && L->y && L->h && L->steps && L->rejects && L->pulses && L->failed && L->k && L->src && L->pulse &&
+           L->period && L->pulse_first && L->pulse_count && L->padded > 0 &&
+           (L->lane_count ? L->lane_count : L->padded) <= L->padded;
+}
+
+// SimkGenOptions (0 = default, ~0u = none where documented) to generator options.
+GenOptions gen_options(const SimkGenOptions *options) {
+    GenOptions o;
+    if (!options) return o;
+    o.fence_rows = options->fence_rows;
+    o.waves_per_eu = options->waves_per_eu;
+    o.unroll_stages = options->unroll_stages;
+    o.reg_window = options->reg_window == 0 ? 8u : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
+    o.batch = options->batch == 0 ? 16u : options->batch;
+    o.precision = options->precision;
+    o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
+    return o;
+}
+
+void free_graph(State::Graph &r) {
+    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+    if (r.ws) (void)hipFree(r.ws);
+    if (r.queue) (void)hipFree(r.queue);
+    r = State::Graph();
+}
+
+// Caller has synchronized the stream.
+void destroy_graphs() {
+    for (State::Graph &r : g.graphs) free_graph(r);
+    g.graphs.clear();
+}
+
+// One persistent launch over several mechanisms' blocks (ragged batch):
+// prepare sizes everything (may allocate), enqueue only records stream work.
+struct MultiPlan {
+    SimkMultiArgs MA;
+    u32 waves = 0, slots = 0;
+    double *ws = nullptr;
+    u32 *queue = nullptr;
+};
+
+// own = true: allocate a workspace and queue counters for this plan alone (a
+// captured graph keeps them); else use the module's shared ones.
+hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPlan &P, bool own = false) {
+    if (!g.multi_ok || count != g.multi_mechs.size() || repeats == 0 || !Ls) return hipErrorInvalidValue;
+    memset(&P.MA, 0, sizeof(P.MA));
+    P.MA.count = count;
+    u64 total_waves = 0;
+    f64 weight[4], wsum = 0.0;
+    for (u32 i = 0; i < count; ++i) {
+        const GpuKinLaunch *L = Ls + i;
+        if (!launch_ok(L) || L->precision != 0) return hipErrorInvalidValue;
+        const Mech &m = g.multi_mechs[i];
+        const u64 rows = m.R > m.n ? m.R : m.n;
+        if (rows * L->padded * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
+        const u32 lanes = L->lane_count ? L->lane_count : L->padded;
+        total_waves += (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
+        weight[i] = (f64)lanes * (f64)(m.nnz + 4 * m.n);   // per-cell cost ~ LU plus stage vectors
+        wsum += weight[i];
+    }
+    P.slots = g.multi_jit.ws_slots;
+    if ((u64)P.slots * SIMK_WAVE * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
+    u32 resident = (u32)(g.multi_jit.blocks_per_cu > 0 ? g.multi_jit.blocks_per_cu : 1) * (g.cus ? g.cus : 1);
+    if (g.grid_override) resident = g.grid_override;
+    P.waves = (u32)(total_waves < resident ? total_waves : resident);
+    if (P.waves == 0) P.waves = 1;
+    const u64 ws_bytes = (u64)P.waves * P.slots * SIMK_WAVE * sizeof(double);
+    hipError_t e;
+    if (own) {
+        e = hipMalloc(&P.ws, ws_bytes);
+        if (e != hipSuccess) return e;
+        e = hipMalloc(&P.queue, 256);
+        if (e != hipSuccess) {
+            (void)hipFree(P.ws);
+            P.ws = nullptr;
+            return e;
+        }
+    } else {
+        e = ensure_ws(ws_bytes);
+        if (e != hipSuccess) return e;
+        if (!g.queue) {
+            e = hipMalloc(&g.queue, 256);
+            if (e != hipSuccess) return e;
+        }
+        P.ws = g.ws;
+        P.queue = g.queue;
+    }
+    f64 acc = 0.0;
+    for (u32 i = 0; i < count; ++i) {
+        const GpuKinLaunch *L = Ls + i;
+        const u32 lanes = L->lane_count ? L->lane_count : L->padded;
+        P.MA.a[i] = make_args(L, lanes, g.multi_mechs[i], g.multi_ctl[i], P.slots, repeats, P.queue + i, 1);
+        P.MA.a[i].ws = P.ws;
+        P.MA.start[i] = (u32)(acc / (wsum > 0 ? wsum : 1.0) * P.waves);
+        acc += weight[i];
+    }
+    return hipSuccess;
+}
+
+hipError_t enqueue_multi(const MultiPlan &P) {
+    hipLaunchKernelGGL(simk_queues_zero_kernel, dim3(1), dim3(64), 0, g.stream, P.queue, P.MA.count);
+    SimkMultiArgs MA = P.MA;
+    size_t size = sizeof(MA);
+    void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &MA, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+    g.last_kernel = 4;
+    g.last_slots = P.slots;
+    g.last_waves = P.waves;
+    g.last_name = g.multi_jit.name;
+    return hipModuleLaunchKernel(g.multi_jit.fn, P.waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+}
+
 }  // namespace
+
+hipStream_t simk_module_stream() { return g.stream; }
 
 // ---------------------------------------------------------------------------
 // Exports
@@ -817,12 +975,13 @@ GPU_API s32 simk_kin_set_kernel(u32 kind) {
 
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     if (!o) return err_code(hipErrorInvalidValue);
-    g.gen.fence_rows = o->fence_rows;
-    g.gen.waves_per_eu = o->waves_per_eu;
-    g.gen.unroll_stages = o->unroll_stages;
-    g.gen.reg_window = o->reg_window == 0 ? 8u : (o->reg_window == 0xFFFFFFFFu ? 0u : o->reg_window);
-    g.gen.batch = o->batch == 0 ? 16u : o->batch;
-    const u32 sw = o->sweeps == 0 ? 1u : (o->sweeps == 0xFFFFFFFFu ? 0u : o->sweeps);
+    const GenOptions n = gen_options(o);
+    g.gen.fence_rows = n.fence_rows;
+    g.gen.waves_per_eu = n.waves_per_eu;
+    g.gen.unroll_stages = n.unroll_stages;
+    g.gen.reg_window = n.reg_window;
+    g.gen.batch = n.batch;
+    const u32 sw = n.sweeps;
     if (sw != g.sweeps && g.jit_mixed.mod) {
         (void)hipModuleUnload(g.jit_mixed.mod);
         g.jit_mixed = JitKernel();
@@ -857,19 +1016,33 @@ GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const Ki
     if (!mech || !caps || !method) return err_code(hipErrorInvalidValue);
     Mech m;
     if (!from_bytes(mech, *caps, m).empty()) return err_code(hipErrorInvalidValue);
-    GenOptions o;
-    if (options) {
-        o.fence_rows = options->fence_rows;
-        o.waves_per_eu = options->waves_per_eu;
-        o.unroll_stages = options->unroll_stages;
-        o.reg_window = options->reg_window == 0 ? 8u : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
-        o.batch = options->batch == 0 ? 16u : options->batch;
-        o.precision = options->precision;
-        o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
-    }
+    GenOptions o = gen_options(options);
     KinMethod me = *method;
     me.name = nullptr;
     GenResult gr = generate(m, me, o, kDeviceKinSource);
+    std::vector<char> code;
+    bool cached = false;
+    std::string text;
+    bool ok = jit_code(gr, code, cached, text);
+    if (log && log_size) snprintf(log, log_size, "%s", text.c_str());
+    return ok ? 0 : err_code(hipErrorInvalidImage);
+}
+
+GPU_API s32 simk_jit_precompile_multi(u32 count, const void *const *mechs, const SimkCaps *caps,
+                                      const KinMethod *method, const SimkGenOptions *options, char *log,
+                                      u32 log_size) {
+    if (count == 0 || count > 4 || !mechs || !caps || !method) return err_code(hipErrorInvalidValue);
+    std::vector<Mech> ms(count);
+    std::vector<const Mech *> ptrs;
+    for (u32 i = 0; i < count; ++i) {
+        if (!mechs[i] || !from_bytes(mechs[i], caps[i], ms[i]).empty()) return err_code(hipErrorInvalidValue);
+        ptrs.push_back(&ms[i]);
+    }
+    GenOptions o = gen_options(options);
+    o.precision = 0;
+    KinMethod me = *method;
+    me.name = nullptr;
+    GenResult gr = generate_multi(ptrs, me, o, kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
     std::string text;
@@ -895,6 +1068,136 @@ GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms) {
     hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
     if (ms) *ms = t;
     return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCaps *caps, const KinMethod *method,
+                               const KinControl *control) {
+    if (!g.ready) return err_code(hipErrorNotInitialized);
+    if (count == 0 || count > 4 || !mechs || !caps || !method || !control) return err_code(hipErrorInvalidValue);
+    if (method->stages < 1 || method->stages > 4) return err_code(hipErrorInvalidValue);
+    std::vector<Mech> ms(count);
+    for (u32 i = 0; i < count; ++i)
+        if (!mechs[i] || !from_bytes(mechs[i], caps[i], ms[i]).empty()) return err_code(hipErrorInvalidValue);
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return err_code(e);
+    destroy_graphs();   // they launch the kernel of the module unloaded here
+    if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
+    g.multi_jit = JitKernel();
+    g.multi_ok = false;
+    g.multi_mechs = ms;
+    g.method = *method;
+    g.method.name = nullptr;
+    fill_dev_method(g.method, g.dev_method);
+    g.multi_ctl.assign(count, SimkDevCtl());
+    for (u32 i = 0; i < count; ++i) {
+        SimkDevCtl &c = g.multi_ctl[i];
+        c.rtol = control->rtol;
+        c.atol = control->atol;
+        c.h_min = control->h_min;
+        c.h_max = control->h_max;
+        c.h_start = control->h_start;
+        c.inv_n = 1.0 / (f64)ms[i].n;
+        c.max_steps = control->max_steps;
+        c.fixed_step = control->fixed_step ? 1 : 0;
+        c.clamp_negative = control->clamp_negative ? 1 : 0;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<const Mech *> ptrs;
+    for (const Mech &m : g.multi_mechs) ptrs.push_back(&m);
+    GenOptions o = g.gen;
+    o.precision = 0;
+    GenResult gr = generate_multi(ptrs, g.method, o, kDeviceKinSource);
+    std::vector<char> code;
+    bool cached = false;
+    if (!jit_code(gr, code, cached, g.jit_log)) return err_code(hipErrorInvalidImage);
+    hipModule_t mod;
+    e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) return err_code(e);
+    hipFunction_t fn;
+    e = hipModuleGetFunction(&fn, mod, gr.kernel.c_str());
+    if (e != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return err_code(e);
+    }
+    int blocks = 1;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, SIMK_WAVE, 0) != hipSuccess) blocks = 1;
+    g.multi_jit.mod = mod;
+    g.multi_jit.fn = fn;
+    g.multi_jit.ws_slots = gr.ws_slots;
+    g.multi_jit.name = gr.kernel;
+    g.multi_jit.cached = cached;
+    g.multi_jit.blocks_per_cu = blocks;
+    g.multi_jit.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    g.multi_ok = true;
+    return 0;
+}
+
+GPU_API s32 simk_kin_advance_multi(u32 count, const GpuKinLaunch *launches, u32 repeats, f32 *kernel_ms) {
+    MultiPlan P;
+    hipError_t e = prepare_multi(count, launches, repeats, P);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_start, g.stream);
+    e = enqueue_multi(P);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_stop, g.stream);
+    e = hipEventSynchronize(g.kin_stop);
+    if (e != hipSuccess) return err_code(e);
+    f32 ms = 0.0f;
+    hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    g.last_ms = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_graph_capture_multi(u32 count, const GpuKinLaunch *launches, u32 repeats, u32 steps,
+                                     u32 *graph_id) {
+    if (!graph_id || steps == 0) return err_code(hipErrorInvalidValue);
+    MultiPlan P;
+    hipError_t e = prepare_multi(count, launches, repeats, P, true);   // every allocation happens here
+    if (e != hipSuccess) return err_code(e);
+    State::Graph rec;
+    rec.ws = P.ws;
+    rec.queue = P.queue;
+    e = hipStreamSynchronize(g.stream);
+    if (e == hipSuccess) e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        free_graph(rec);
+        return err_code(e);
+    }
+    hipError_t le = hipSuccess;
+    for (u32 s = 0; s < steps && le == hipSuccess; ++s) le = enqueue_multi(P);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(g.stream, &graph);
+    if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(&rec.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (le != hipSuccess || e != hipSuccess) {
+        free_graph(rec);
+        return err_code(le != hipSuccess ? le : e);
+    }
+    g.graphs.push_back(rec);
+    *graph_id = (u32)(g.graphs.size() - 1);
+    return 0;
+}
+
+GPU_API s32 simk_graph_replay(u32 graph_id, f32 *ms) {
+    if (graph_id >= g.graphs.size() || !g.graphs[graph_id].exec) return err_code(hipErrorInvalidValue);
+    hipEventRecord(g.kin_start, g.stream);
+    hipError_t e = hipGraphLaunch(g.graphs[graph_id].exec, g.stream);
+    if (e != hipSuccess) return err_code(e);
+    hipEventRecord(g.kin_stop, g.stream);
+    e = hipEventSynchronize(g.kin_stop);
+    if (e != hipSuccess) return err_code(e);
+    f32 t = 0.0f;
+    hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
+    if (ms) *ms = t;
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_graph_destroy(u32 graph_id) {
+    if (graph_id >= g.graphs.size() || !g.graphs[graph_id].exec) return err_code(hipErrorInvalidValue);
+    hipError_t e = hipStreamSynchronize(g.stream);
+    free_graph(g.graphs[graph_id]);
+    return err_code(e);
 }
 
 }  // extern "C"

@@ -390,10 +390,12 @@ extern "C" SIMK_HOST_API void simk_synth_cells(const SimkSynthSpec *sp, const f6
         f64 T = sp->T_lo + (sp->T_hi - sp->T_lo) * rng.uniform();
         f64 pH = use_pH ? sp->pH_lo + (sp->pH_hi - sp->pH_lo) * rng.uniform() : 7.0;
         f64 ox = sp->oxidant_log10_spread > 0.0 ? pow(10.0, sp->oxidant_log10_spread * rng.uniform()) : 1.0;
+        f64 load = sp->load_log10_spread > 0.0 ? pow(10.0, -sp->load_log10_spread * rng.uniform()) : 1.0;
         for (u32 s = 0; s < n; ++s) {
             f64 ly = sp->log10_y_lo + (sp->log10_y_hi - sp->log10_y_lo) * rng.uniform();
             f64 v = pow(10.0, ly);
             if (s < sp->oxidant_species) v *= ox;
+            v *= load;
             if (y) y[(u64)s * padded + i] = v;
         }
         if (k) {

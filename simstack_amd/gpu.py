@@ -35,7 +35,9 @@ REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_use
                      "gpu_kin_wait", "gpu_kin_set_mode")
 EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_set_grid", "simk_kin_set_order",
                      "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
-                     "simk_jit_precompile", "simk_copy_device")
+                     "simk_jit_precompile", "simk_copy_device", "simk_upwind_f64", "simk_upwind_f32",
+                     "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
+                     "simk_graph_destroy", "simk_jit_precompile_multi")
 
 
 def _sig(lib, name, res, *args):
@@ -76,6 +78,15 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_kin_advance_n", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, P(C.c_float))
     _sig(lib, "simk_kin_set_grid", C.c_int32, C.c_uint32)
     _sig(lib, "simk_kin_set_order", C.c_int32, V)
+    _sig(lib, "simk_kin_set_mechs", C.c_int32, C.c_uint32, V, V, P(_ffi.KinMethod), P(_ffi.KinControl))
+    _sig(lib, "simk_jit_precompile_multi", C.c_int32, C.c_uint32, V, V, P(_ffi.KinMethod), P(SimkGenOptions),
+         C.c_char_p, C.c_uint32)
+    _sig(lib, "simk_kin_advance_multi", C.c_int32, C.c_uint32, V, C.c_uint32, P(C.c_float))
+    _sig(lib, "simk_graph_capture_multi", C.c_int32, C.c_uint32, V, C.c_uint32, C.c_uint32, P(C.c_uint32))
+    _sig(lib, "simk_graph_replay", C.c_int32, C.c_uint32, P(C.c_float))
+    _sig(lib, "simk_graph_destroy", C.c_int32, C.c_uint32)
+    _sig(lib, "simk_upwind_f64", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
+    _sig(lib, "simk_upwind_f32", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
     return lib
 
 
@@ -86,6 +97,19 @@ def precompile(mech, meth, options=None, lib=None):
     opt = options or SimkGenOptions()
     rc = lib.simk_jit_precompile(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(opt), log,
                                  1 << 16)
+    if rc != 0:
+        raise RuntimeError(f"kernel compile failed ({rc}):\n{log.value.decode(errors='replace')[-4000:]}")
+    return log.value.decode(errors="replace")
+
+
+def precompile_multi(mechs, meth, options=None, lib=None):
+    """precompile() for the multi-mechanism kernel of GpuModule.set_mechs."""
+    lib = lib or load_library()
+    log = C.create_string_buffer(1 << 16)
+    opt = options or SimkGenOptions()
+    ptrs = (C.c_void_p * len(mechs))(*[m.bytes.ctypes.data for m in mechs])
+    caps = (_ffi.SimkCaps * len(mechs))(*[m.caps for m in mechs])
+    rc = lib.simk_jit_precompile_multi(len(mechs), ptrs, caps, C.byref(meth), C.byref(opt), log, 1 << 16)
     if rc != 0:
         raise RuntimeError(f"kernel compile failed ({rc}):\n{log.value.decode(errors='replace')[-4000:]}")
     return log.value.decode(errors="replace")
@@ -117,6 +141,34 @@ class GpuModule:
     def set_gen_options(self, fence_rows=0, waves_per_eu=0, unroll_stages=0, reg_window=0, batch=0, sweeps=0):
         opt = SimkGenOptions(fence_rows, waves_per_eu, unroll_stages, reg_window, batch, 0, sweeps)
         self._check(self.lib.simk_kin_set_gen_options(C.byref(opt)), "simk_kin_set_gen_options")
+
+    def set_mechs(self, mechs, meth, ctl):
+        """Ragged batch: up to four mechanisms compiled into one persistent kernel."""
+        ptrs = (C.c_void_p * len(mechs))(*[m.bytes.ctypes.data for m in mechs])
+        caps = (_ffi.SimkCaps * len(mechs))(*[m.caps for m in mechs])
+        self._check(self.lib.simk_kin_set_mechs(len(mechs), ptrs, caps, C.byref(meth), C.byref(ctl)),
+                    "simk_kin_set_mechs")
+
+    def advance_multi(self, dzs, duration, repeats=1):
+        """One launch advancing every DeviceZones in dzs (mechanism i = dzs[i])."""
+        Ls = (_ffi.GpuKinLaunch * len(dzs))(*[d.launch_desc(duration) for d in dzs])
+        ms = C.c_float()
+        self._check(self.lib.simk_kin_advance_multi(len(dzs), Ls, repeats, C.byref(ms)), "simk_kin_advance_multi")
+        return ms.value
+
+    def capture_multi(self, dzs, duration, repeats, steps):
+        Ls = (_ffi.GpuKinLaunch * len(dzs))(*[d.launch_desc(duration) for d in dzs])
+        gid = C.c_uint32()
+        self._check(self.lib.simk_graph_capture_multi(len(dzs), Ls, repeats, steps, C.byref(gid)), "capture")
+        return gid.value
+
+    def destroy_graph(self, gid):
+        self._check(self.lib.simk_graph_destroy(gid), "simk_graph_destroy")
+
+    def replay(self, gid):
+        ms = C.c_float()
+        self._check(self.lib.simk_graph_replay(gid, C.byref(ms)), "simk_graph_replay")
+        return ms.value
 
     def set_grid(self, waves):
         self._check(self.lib.simk_kin_set_grid(waves), "simk_kin_set_grid")

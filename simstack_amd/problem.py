@@ -22,16 +22,17 @@ PRESETS = {
     # 48 / 150 with ~18 % fill after the symbolic LU, pH 2-7
     "c2": dict(_COMMON, seed=2, n_species=48, n_reactions=150, band=3, long_range=0.02,
                density_lo=0.16, density_hi=0.20, pH_lo=2.0, pH_hi=7.0, acid_frac=0.1, base_frac=0.1),
-    # the three ragged-batch mechanisms: 60/200, 40/130, 24/70, stiffness spread
+    # the three ragged-batch mechanisms: 60/200, 40/130, 24/70; per-cell T, oxidant and load
+    # spread (attempts per 1 s vary about 10x between the 5th and 95th percentile cells)
     "c3a": dict(_COMMON, seed=31, n_species=60, n_reactions=200, band=3, long_range=0.02,
                 density_lo=0.10, density_hi=0.22, T_lo=278.0, T_hi=318.0,
-                oxidant_log10_spread=3.0, oxidant_species=8),
+                oxidant_log10_spread=3.0, oxidant_species=8, load_log10_spread=5.0),
     "c3b": dict(_COMMON, seed=32, n_species=40, n_reactions=130, band=3, long_range=0.02,
                 density_lo=0.12, density_hi=0.26, T_lo=278.0, T_hi=318.0,
-                oxidant_log10_spread=3.0, oxidant_species=6),
+                oxidant_log10_spread=3.0, oxidant_species=6, load_log10_spread=5.0),
     "c3c": dict(_COMMON, seed=33, n_species=24, n_reactions=70, band=3, long_range=0.02,
                 density_lo=0.15, density_hi=0.40, T_lo=278.0, T_hi=318.0,
-                oxidant_log10_spread=3.0, oxidant_species=4),
+                oxidant_log10_spread=3.0, oxidant_species=4, load_log10_spread=5.0),
 }
 
 

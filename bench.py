@@ -110,6 +110,109 @@ def rel_err_groups(got, ref, atol, rtol):
     return out
 
 
+def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0):
+    """Three mechanisms (60/200, 40/130, 24/70) with 1k / 20k / 200k cells in one
+    persistent launch per operator-split step, the steps captured in one HIP
+    graph; beside it the same steps as eager launches and as one mechanism at a
+    time (three kernels, each over its own cells only)."""
+    names = ("c3a", "c3b", "c3c")
+    meth, ctl = problem.method(2), problem.control()
+    mechs, z0s = [], []
+    for name, n in zip(names, cells):
+        sp = problem.spec(name)
+        m = problem.Mechanism.synthetic(sp)
+        mechs.append(m)
+        z0s.append(problem.Zones(m.n, m.R, n).fill_synthetic(sp, m.params))
+    gm = run_workload.gm
+    need = sum(z.block.nbytes for z in z0s) + (64 << 20)
+    if gm is None or need > gm.info.arena_bytes:
+        gm = run_workload.gm = gpu.GpuModule(arena_bytes=need)
+    gm.arena_reset()
+    gm.set_grid(0)
+    gm.set_gen_options()
+    dzs = [gpu.DeviceZones(gm, z.copy()) for z in z0s]
+
+    def reset():
+        for d, z in zip(dzs, z0s):
+            d.z.block[:] = z.block
+            d.upload()
+
+    def totals():
+        for d in dzs:
+            d.download()
+        acc = sum(float(d.z.steps[:d.z.count].sum()) for d in dzs)
+        att = acc + sum(float(d.z.rejects[:d.z.count].sum()) for d in dzs)
+        return acc, att, [d.z.steps[:d.z.count].copy() for d in dzs]
+
+    out = {"workload": f"{names[0]} 60/200 x {cells[0]}, {names[1]} 40/130 x {cells[1]}, {names[2]} 24/70 x "
+                       f"{cells[2]} cells; {steps} x {duration:g} s operator-split, Rodas3, rtol 1e-6 atol 1e-12",
+           "steps": steps}
+    # one mechanism at a time
+    reset()
+    gm.lib.gpu_sync()
+    t0 = time.perf_counter()
+    for m, d in zip(mechs, dzs):
+        gm.set_mech(m, meth, ctl)
+        for _ in range(steps):
+            d.advance(duration)
+    gm.lib.gpu_sync()
+    sep_wall = time.perf_counter() - t0
+    sep_acc, _, sep_steps = totals()
+    # one persistent launch per step, eager
+    gm.set_mechs(mechs, meth, ctl)
+    reset()
+    gm.lib.gpu_sync()
+    t0 = time.perf_counter()
+    kms = [gm.advance_multi(dzs, duration) for _ in range(steps)]
+    gm.lib.gpu_sync()
+    eager_wall = time.perf_counter() - t0
+    eager_acc, eager_att, eager_steps = totals()
+    # the same steps captured in one graph
+    reset()
+    gid = gm.capture_multi(dzs, duration, 1, steps)
+    gm.lib.gpu_sync()
+    t0 = time.perf_counter()
+    gm.replay(gid)
+    gm.lib.gpu_sync()
+    graph_wall = time.perf_counter() - t0
+    gm.destroy_graph(gid)
+    graph_acc, graph_att, graph_steps = totals()
+    per_cell = np.concatenate(graph_steps)
+    same = all(np.array_equal(a, b) and np.array_equal(a, c) for a, b, c in zip(graph_steps, eager_steps, sep_steps))
+    out.update({"value": graph_acc / graph_wall, "unit": "cell-steps/s", "graph_wall_s": graph_wall,
+                "eager_value": eager_acc / eager_wall, "separate_value": sep_acc / sep_wall,
+                "graph_vs_separate": (graph_acc / graph_wall) / (sep_acc / sep_wall),
+                "attempts_per_s": graph_att / graph_wall, "kernel_ms_per_step": float(np.mean(kms)),
+                "steps_per_cell_p5_p50_p95": [float(np.percentile(per_cell, q)) for q in (5, 50, 95)],
+                "identical_step_counts_all_modes": bool(same)})
+    return out
+
+
+def serving_block(gpu, problem, rate=200.0, window=1.0, steps=20, substeps=4, dt=1e-3):
+    """Poisson arrivals over the three ragged mechanisms through simk_serve_*."""
+    from simstack_amd import serving
+    pools = serving.Pools()
+    arr = serving.Arrivals(rate=rate, window=window)
+    gm = run_workload.gm
+    if gm is None:
+        gm = run_workload.gm = gpu.GpuModule(arena_bytes=1 << 28)
+    r = serving.run(gm, pools, arr, steps=steps, substeps=substeps, dt=dt)
+    lat = r["latency"]
+    st = r["stats"]
+    busy_s = (st.advect_ms + st.chem_ms + st.copy_ms) / 1e3
+    return {"workload": f"Poisson {rate:g} req/s for {window:g} s ({r['requests']} requests), cells log-uniform "
+                        f"64-262144 in 4096-node channels, mechanism uniform over 60/200, 40/130, 24/70; "
+                        f"{steps} x (upwind advection {substeps} substeps, then chemistry {dt:g} s)",
+            "requests": r["requests"], "cells": r["cells"], "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
+            "wall_s": r["wall"], "requests_per_s": r["requests"] / r["wall"],
+            "cell_steps_per_s": r["accepted"] / r["wall"], "attempts_per_s": r["attempts"] / r["wall"],
+            "operator_split_cell_steps_per_s": r["cells"] * steps / r["wall"],
+            "offered_cells_per_s": r["cells"] / window, "failed_cells": r["failed"],
+            "iterations": int(st.iterations), "peak_admitted_GB": st.peak_admitted_bytes / 1e9,
+            "gpu_busy_s": busy_s, "advect_ms": st.advect_ms, "chem_ms": st.chem_ms, "copy_ms": st.copy_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +232,11 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64 (bit-exact), 2 mixed precision")
     ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1; ~0u: none)")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the three-mechanism ragged batch")
+    ap.add_argument("--ragged-steps", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
+    ap.add_argument("--serve-window", type=float, default=1.0, help="seconds of Poisson arrivals")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (cells shard with no "
@@ -199,6 +307,10 @@ def main():
         out["mixed"] = {"workload": f"same mechanism, {args.mixed_cells} cells, 10 x 1 s fused",
                         "fp64_value": f64r["accepted"] / f64r["wall"], **mix,
                         "note": "errors vs the FP64 GPU path, which is bit-identical to the FP64 oracle"}
+    if not args.no_ragged:
+        out["ragged"] = ragged_block(gpu, problem, args.ragged_steps)
+    if not args.no_serve:
+        out["serving"] = serving_block(gpu, problem, rate=args.serve_rate, window=args.serve_window)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
     print(json.dumps(out), flush=True)

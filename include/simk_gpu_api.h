@@ -168,6 +168,73 @@ GPU_API s32 simk_upwind_f64(f64 *y, u64 padded, u32 fields, u32 channels, u32 no
 GPU_API s32 simk_upwind_f32(f32 *x, u64 padded, u32 fields, u32 channels, u32 nodes, const f32 *courant,
                             u32 substeps, f32 *outlet_sum);
 
+/* ---- serving loop: device-resident requests, continuous batching -------
+ *
+ * Solve requests over the mechanisms of simk_kin_set_mechs. Each request is
+ * a zone block of `cells` cells (the reference's SoA layout, row stride
+ * `padded`) laid out as `channels` channels of `nodes` nodes
+ * (cell = channel * nodes + node). It runs `steps` operator-split steps: the
+ * upwind advection of every species row along its channels (simk_upwind_f64
+ * with the request's courant and inflow), then the chemistry advance of `dt`
+ * seconds (h carried). Its block lives in HBM from admission to completion.
+ *
+ * simk_serve_submit queues a request. Each simk_serve_step is one iteration
+ * of continuous batching: admit queued requests in arrival order while their
+ * blocks fit the HBM budget (and a free lane range of their mechanism's
+ * slab), advect every admitted request in one batched launch, advance all of
+ * them in one multi-mechanism persistent launch, and copy the finished
+ * requests' solver rows (y, h, steps, rejects, pulses, failed) into their
+ * result buffers. A cell's result does not depend on which other requests
+ * share its iterations. Input blocks, courant and inflow arrays must stay
+ * valid and unchanged from submit until the request is admitted, result
+ * buffers until it is reported done. Not re-entrant, like the rest of the
+ * module. */
+typedef struct SimkServeConfig {
+    u64 hbm_budget;      /* bytes of admitted zone blocks at once (0: 8 GiB) */
+    u32 max_requests;    /* queued + admitted at once (0: 4096) */
+    u32 steps;           /* operator-split steps per request (0: 20) */
+    u32 substeps;        /* advection substeps per step (0: 4) */
+    u32 reserved;
+    f64 dt;              /* chemistry seconds per step (0: 1e-3) */
+} SimkServeConfig;
+
+typedef struct SimkServeRequest {
+    u32 mech;            /* index into the simk_kin_set_mechs list */
+    u32 cells;           /* channels * nodes */
+    u32 channels, nodes; /* nodes <= 8192 */
+    u32 padded;          /* row stride of block (>= cells) */
+    u32 reserved;
+    const f64 *block;    /* host zone block (all rows, reference order) holding the request's inputs */
+    const f64 *courant;  /* [channels * nodes], each in [0, 1] */
+    const f64 *inflow;   /* [species] concentrations entering node 0, or null (zero) */
+    f64 *result;         /* host [species + 5][cells]: y, h, steps, rejects, pulses, failed after the last step */
+} SimkServeRequest;
+
+typedef struct SimkServeStats {
+    u64 submitted, admitted, completed;
+    u32 queued, active;          /* now */
+    u64 iterations;
+    u64 admitted_bytes;          /* now */
+    u64 peak_admitted_bytes;
+    f64 advect_ms, chem_ms;      /* summed kernel time (module stream events) */
+    f64 copy_ms;                 /* summed admission + completion copy time */
+} SimkServeStats;
+
+/* Allocates the slabs (one per mechanism, sized so each could hold the whole
+ * budget, capped where a mechanism's row offsets would pass 2^31 bytes),
+ * workspace and staging. Needs simk_kin_set_mechs first; a second open
+ * closes the first. */
+GPU_API s32 simk_serve_open(const SimkServeConfig *config);
+/* Queues a request; *ticket identifies it in simk_serve_step's done list.
+ * hipErrorInvalidValue for a malformed request or one larger than the
+ * budget; hipErrorOutOfMemory when max_requests are in flight. */
+GPU_API s32 simk_serve_submit(const SimkServeRequest *request, u32 *ticket);
+/* One iteration (waits for it). done[0 .. *n_done) = tickets finished in it
+ * (at most max_done are reported; the rest in later calls). */
+GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done);
+GPU_API s32 simk_serve_stats(SimkServeStats *out);
+GPU_API s32 simk_serve_close(void);
+
 /* Measurement aid: device-to-device copy of `bytes` (16-byte multiple and
  * alignment) by a plain streaming kernel, timed with stream events. HBM
  * byte counters of this kernel calibrate the solver kernels' counters. */

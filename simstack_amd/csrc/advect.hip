@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "simk_gpu_api.h"
+#include "module_internal.h"
 
 namespace {
 
@@ -31,15 +32,12 @@ __device__ __forceinline__ float fma_t<float>(float a, float b, float c) {
     return __builtin_fmaf(a, b, c);
 }
 
+// One (field, channel) row of `nodes` values at xr, advanced `substeps`
+// substeps by the whole workgroup (kThreads threads, CHUNK nodes each).
 template <class T, unsigned CHUNK>
-__global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned long long padded, unsigned channels,
-                                                               unsigned nodes, const T *courant, unsigned substeps,
-                                                               const T *inflow, T *outlet_sum) {
-    const unsigned row = blockIdx.x;   // field * channels + channel
-    const unsigned field = row / channels, ch = row % channels;
+__device__ __forceinline__ void upwind_row(T *xr, const T *cr, unsigned nodes, unsigned substeps, T in,
+                                           T *outlet) {
     const unsigned t = threadIdx.x;
-    T *xr = x + field * padded + (unsigned long long)ch * nodes;
-    const T *cr = courant + (unsigned long long)ch * nodes;
     __shared__ T stage[kThreads * CHUNK];
     __shared__ T last[kThreads];
 
@@ -56,11 +54,10 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned lo
     for (unsigned i = 0; i < CHUNK; ++i) c[i] = stage[t * CHUNK + i];
 
     const unsigned first = t * CHUNK;
-    const T in = inflow ? inflow[field] : T(0);
     const unsigned last_thread = (nodes - 1) / CHUNK;
-    T out = (outlet_sum && t == last_thread) ? outlet_sum[row] : T(0);
+    T out = (outlet && t == last_thread) ? *outlet : T(0);
     for (unsigned s = 0; s < substeps; ++s) {
-        if (outlet_sum && t == last_thread) out = out + v[(nodes - 1) - first];
+        if (outlet && t == last_thread) out = out + v[(nodes - 1) - first];
         last[t] = v[CHUNK - 1];
         __syncthreads();
         const T left = t ? last[t - 1] : in;
@@ -72,12 +69,43 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned lo
         }
         __syncthreads();
     }
-    if (outlet_sum && t == last_thread) outlet_sum[row] = out;
+    if (outlet && t == last_thread) *outlet = out;
 
 #pragma unroll
     for (unsigned i = 0; i < CHUNK; ++i) stage[t * CHUNK + i] = v[i];
     __syncthreads();
     for (unsigned j = t; j < nodes; j += kThreads) xr[j] = stage[j];
+}
+
+template <class T, unsigned CHUNK>
+__global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned long long padded, unsigned channels,
+                                                               unsigned nodes, const T *courant, unsigned substeps,
+                                                               const T *inflow, T *outlet_sum) {
+    const unsigned row = blockIdx.x;   // field * channels + channel
+    const unsigned field = row / channels, ch = row % channels;
+    upwind_row<T, CHUNK>(x + field * padded + (unsigned long long)ch * nodes, courant + (unsigned long long)ch * nodes,
+                         nodes, substeps, inflow ? inflow[field] : T(0), outlet_sum ? outlet_sum + row : nullptr);
+}
+
+// Many requests' rows in one launch: block b belongs to the segment whose
+// [row_begin, row_begin + fields * channels) holds b (segments sorted by
+// row_begin, the first at 0).
+template <unsigned CHUNK>
+__global__ void __launch_bounds__(kThreads) simk_upwind_batch_kernel(const SimkUpwindSeg *segs, unsigned count,
+                                                                     unsigned substeps) {
+    const unsigned b = blockIdx.x;
+    unsigned lo = 0, hi = count;   // last segment with row_begin <= b
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) / 2;
+        if (segs[mid].row_begin <= b) lo = mid;
+        else hi = mid;
+    }
+    const SimkUpwindSeg &S = segs[lo];
+    const unsigned row = b - S.row_begin;
+    const unsigned field = row / S.channels, ch = row % S.channels;
+    upwind_row<double, CHUNK>(S.x + field * S.padded + (unsigned long long)ch * S.nodes,
+                              S.courant + (unsigned long long)ch * S.nodes, S.nodes, substeps,
+                              S.inflow ? S.inflow[field] : 0.0, nullptr);
 }
 
 template <class T>
@@ -103,8 +131,21 @@ hipError_t launch_upwind(hipStream_t stream, T *x, u64 padded, u32 fields, u32 c
 
 }  // namespace
 
-// Launches on the module stream (simk_module_stream) without waiting.
-hipStream_t simk_module_stream();
+hipError_t simk_enqueue_upwind_batch(hipStream_t stream, const SimkUpwindSeg *segs, u32 count, u32 total_rows,
+                                     u32 max_nodes, u32 substeps) {
+    if (!segs || count == 0 || total_rows == 0 || max_nodes == 0) return hipErrorInvalidValue;
+    const u32 chunk_needed = (max_nodes + kThreads - 1) / kThreads;
+    const dim3 grid(total_rows), block(kThreads);
+    if (chunk_needed <= 4)
+        hipLaunchKernelGGL((simk_upwind_batch_kernel<4>), grid, block, 0, stream, segs, count, substeps);
+    else if (chunk_needed <= 16)
+        hipLaunchKernelGGL((simk_upwind_batch_kernel<16>), grid, block, 0, stream, segs, count, substeps);
+    else if (chunk_needed <= kMaxChunk)
+        hipLaunchKernelGGL((simk_upwind_batch_kernel<kMaxChunk>), grid, block, 0, stream, segs, count, substeps);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 extern "C" {
 

@@ -18,6 +18,7 @@
 #include "device_kin.h"
 #include "mech.hpp"
 #include "simk_gpu_api.h"
+#include "module_internal.h"
 
 #include "device_kin_src.inc"   // kDeviceKinSource: device_kin.h as text, for hipRTC
 
@@ -782,6 +783,53 @@ hipError_t enqueue_multi(const MultiPlan &P) {
 }  // namespace
 
 hipStream_t simk_module_stream() { return g.stream; }
+
+SimkMultiShape simk_multi_shape() {
+    SimkMultiShape S;
+    memset(&S, 0, sizeof(S));
+    if (!g.multi_ok) return S;
+    S.count = (u32)g.multi_mechs.size();
+    S.slots = g.multi_jit.ws_slots;
+    S.resident_waves = (u32)(g.multi_jit.blocks_per_cu > 0 ? g.multi_jit.blocks_per_cu : 1) * (g.cus ? g.cus : 1);
+    for (u32 i = 0; i < S.count; ++i) {
+        S.n[i] = g.multi_mechs[i].n;
+        S.R[i] = g.multi_mechs[i].R;
+    }
+    return S;
+}
+
+hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *lanes, const u32 *const *orders,
+                                  u32 repeats, double *ws, u32 *queue, u32 waves) {
+    if (!g.multi_ok || count != g.multi_mechs.size() || !Ls || !lanes || !ws || !queue || waves == 0 ||
+        repeats == 0)
+        return hipErrorInvalidValue;
+    MultiPlan P;
+    memset(&P.MA, 0, sizeof(P.MA));
+    P.MA.count = count;
+    P.slots = g.multi_jit.ws_slots;
+    P.waves = waves;
+    P.ws = ws;
+    P.queue = queue;
+    f64 weight[4], wsum = 0.0;
+    for (u32 i = 0; i < count; ++i) {
+        const GpuKinLaunch *L = Ls + i;
+        const Mech &m = g.multi_mechs[i];
+        const u64 rows = m.R > m.n ? m.R : m.n;
+        if (!launch_ok(L) || lanes[i] > L->padded || rows * L->padded * 8ull >= (1ull << 31))
+            return hipErrorInvalidValue;
+        weight[i] = (f64)lanes[i] * (f64)(m.nnz + 4 * m.n);
+        wsum += weight[i];
+    }
+    f64 acc = 0.0;
+    for (u32 i = 0; i < count; ++i) {
+        P.MA.a[i] = make_args(Ls + i, lanes[i], g.multi_mechs[i], g.multi_ctl[i], P.slots, repeats, queue + i, 1);
+        P.MA.a[i].ws = ws;
+        P.MA.a[i].order = orders ? orders[i] : nullptr;
+        P.MA.start[i] = (u32)(acc / (wsum > 0 ? wsum : 1.0) * P.waves);
+        acc += weight[i];
+    }
+    return enqueue_multi(P);
+}
 
 // ---------------------------------------------------------------------------
 // Exports

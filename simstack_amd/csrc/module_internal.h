@@ -1,0 +1,47 @@
+// Internal interface between the translation units of libsimk_gpu.so (the
+// solver module, the advection kernels and the serving loop). Not exported.
+#ifndef SIMK_MODULE_INTERNAL_H
+#define SIMK_MODULE_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include "simk_gpu_api.h"
+
+// The module's one stream (non-blocking, default priority): every launch of
+// the library is ordered on it.
+hipStream_t simk_module_stream();
+
+// One request's advection rows for the batched upwind kernel: fields x
+// channels rows of `nodes` values at x + field * padded + channel * nodes.
+struct SimkUpwindSeg {
+    double *x;
+    unsigned long long padded;
+    const double *courant;   // [channels * nodes]
+    const double *inflow;    // [fields] or null (zero inflow)
+    unsigned fields, channels, nodes;
+    unsigned row_begin;      // first grid row of this segment (prefix sum of fields * channels)
+};
+
+// Enqueues the batched upwind kernel on `stream` (segs: device array sorted by
+// row_begin, the first at 0; max_nodes <= 8192).
+hipError_t simk_enqueue_upwind_batch(hipStream_t stream, const SimkUpwindSeg *segs, u32 count, u32 total_rows,
+                                     u32 max_nodes, u32 substeps);
+
+// The mechanisms of the last successful simk_kin_set_mechs.
+struct SimkMultiShape {
+    u32 count;               // 0: none set
+    u32 slots;               // workspace f64 slots per lane of the multi kernel
+    u32 resident_waves;      // waves the device holds at once for it
+    u32 n[4], R[4];
+};
+SimkMultiShape simk_multi_shape();
+
+// Enqueues the queue reset and one multi-mechanism persistent launch on the
+// module stream. Ls[i]: mechanism i's block (duration, padded, pointers);
+// lanes[i]: queue positions of mechanism i (may be 0); orders[i]: device
+// position -> lane map (null: identity). ws: waves * slots * 64 f64, queue: 4
+// u32, both owned by the caller.
+hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *lanes, const u32 *const *orders,
+                                  u32 repeats, double *ws, u32 *queue, u32 waves);
+
+#endif  // SIMK_MODULE_INTERNAL_H

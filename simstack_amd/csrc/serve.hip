@@ -1,0 +1,395 @@
+// Serving loop: device-resident solve requests with HBM-budget admission and
+// continuous batching over the multi-mechanism persistent kernel.
+//
+// Memory: one slab per mechanism, a zone block of `cap` lanes in the
+// reference's SoA row order (y, h, steps, rejects, pulses, failed, k, src,
+// pulse, period, pulse_first, pulse_count) plus one courant row. A request
+// owns a contiguous lane range of its mechanism's slab from admission to
+// completion, so its state never leaves HBM between steps. A per-slab order
+// array lists the lanes of the admitted requests; the persistent kernel takes
+// cells from it, so every request of every mechanism advances in one launch
+// and a request joins or leaves at any iteration boundary.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "module_internal.h"
+#include "simk_gpu_api.h"
+
+namespace {
+
+constexpr u32 kMaxSpecies = 64;
+constexpr u32 kMaxNodes = 8192;
+
+struct Slab {
+    double *d = nullptr;     // zone block rows + courant row, row stride cap
+    u32 cap = 0;
+    u32 n = 0, R = 0;
+    u64 zone_rows = 0;       // 3n + R + 8
+    std::map<u32, u32> free_ranges;   // offset -> length
+    u32 *d_order = nullptr;  // queue position -> lane
+    u32 *h_order = nullptr;  // pinned staging
+    u32 lanes = 0;           // current order length
+};
+
+struct Request {
+    SimkServeRequest r;
+    u32 ticket = 0;
+    u32 off = 0;
+    u32 steps_done = 0;
+    u64 bytes = 0;
+    u32 slot = 0;            // inflow slot
+};
+
+struct Server {
+    bool open = false;
+    SimkServeConfig cfg;
+    SimkMultiShape shape;
+    std::vector<Slab> slabs;
+    std::unordered_map<u32, Request> reqs;
+    std::deque<u32> queued;          // tickets in arrival order
+    std::vector<u32> active;         // tickets in admission order
+    std::vector<u32> free_slots;
+    std::vector<u32> pending_done;   // finished, not yet reported
+    u32 next_ticket = 1;
+    double *d_ws = nullptr;
+    u32 *d_queue = nullptr;
+    double *d_inflow = nullptr;      // [max_requests][kMaxSpecies]
+    SimkUpwindSeg *d_segs = nullptr, *h_segs = nullptr;
+    u32 seg_count = 0, seg_rows = 0, seg_max_nodes = 0;
+    bool dirty = true;               // membership changed since the order/segments were built
+    hipEvent_t ev[5] = {};
+    SimkServeStats st;
+};
+
+Server S;
+
+s32 code(hipError_t e) { return (s32)e; }
+
+u64 zone_rows(u32 n, u32 R) { return 3ull * n + R + 8; }
+
+// Row offsets (in lanes of the slab) of the zone block, reference order.
+void zone_offsets(u32 n, u32 R, u64 p, u64 off[12]) {
+    const u64 rows[12] = {n * p, p, p, p, p, p, R * p, n * p, n * p, p, p, p};
+    u64 o = 0;
+    for (int i = 0; i < 12; ++i) {
+        off[i] = o;
+        o += rows[i];
+    }
+}
+
+bool take_range(Slab &sl, u32 len, u32 &off) {
+    for (auto it = sl.free_ranges.begin(); it != sl.free_ranges.end(); ++it) {
+        if (it->second < len) continue;
+        off = it->first;
+        const u32 rest = it->second - len;
+        sl.free_ranges.erase(it);
+        if (rest) sl.free_ranges[off + len] = rest;
+        return true;
+    }
+    return false;
+}
+
+void give_range(Slab &sl, u32 off, u32 len) {
+    auto it = sl.free_ranges.emplace(off, len).first;
+    auto next = std::next(it);
+    if (next != sl.free_ranges.end() && it->first + it->second == next->first) {
+        it->second += next->second;
+        sl.free_ranges.erase(next);
+    }
+    if (it != sl.free_ranges.begin()) {
+        auto prev = std::prev(it);
+        if (prev->first + prev->second == it->first) {
+            prev->second += it->second;
+            sl.free_ranges.erase(it);
+        }
+    }
+}
+
+void release_all() {
+    for (Slab &sl : S.slabs) {
+        if (sl.d) (void)hipFree(sl.d);
+        if (sl.d_order) (void)hipFree(sl.d_order);
+        if (sl.h_order) (void)hipHostFree(sl.h_order);
+    }
+    S.slabs.clear();
+    if (S.d_ws) (void)hipFree(S.d_ws);
+    if (S.d_queue) (void)hipFree(S.d_queue);
+    if (S.d_inflow) (void)hipFree(S.d_inflow);
+    if (S.d_segs) (void)hipFree(S.d_segs);
+    if (S.h_segs) (void)hipHostFree(S.h_segs);
+    for (hipEvent_t &e : S.ev)
+        if (e) (void)hipEventDestroy(e);
+    S = Server();
+}
+
+bool same_shape(const SimkMultiShape &a, const SimkMultiShape &b) {
+    if (a.count != b.count || a.slots != b.slots) return false;
+    for (u32 i = 0; i < a.count; ++i)
+        if (a.n[i] != b.n[i] || a.R[i] != b.R[i]) return false;
+    return true;
+}
+
+// Admits queued requests in arrival order while the budget and a lane range
+// allow; enqueues their uploads.
+hipError_t admit(hipStream_t st) {
+    while (!S.queued.empty()) {
+        Request &q = S.reqs[S.queued.front()];
+        if (S.st.admitted_bytes + q.bytes > S.cfg.hbm_budget) break;
+        Slab &sl = S.slabs[q.r.mech];
+        u32 off = 0;
+        if (!take_range(sl, q.r.cells, off)) break;
+        q.off = off;
+        const u64 cap = sl.cap;
+        hipError_t e = hipMemcpy2DAsync(sl.d + off, cap * 8, q.r.block, (u64)q.r.padded * 8, (u64)q.r.cells * 8,
+                                        sl.zone_rows, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.d + sl.zone_rows * cap + off, q.r.courant, (u64)q.r.cells * 8,
+                               hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && q.r.inflow)
+            e = hipMemcpyAsync(S.d_inflow + (u64)q.slot * kMaxSpecies, q.r.inflow, (u64)sl.n * 8,
+                               hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+        S.st.admitted_bytes += q.bytes;
+        if (S.st.admitted_bytes > S.st.peak_admitted_bytes) S.st.peak_admitted_bytes = S.st.admitted_bytes;
+        S.st.admitted++;
+        S.active.push_back(q.ticket);
+        S.queued.pop_front();
+        S.dirty = true;
+    }
+    return hipSuccess;
+}
+
+// Rebuilds the advection segments and the per-slab order arrays of the
+// admitted requests and enqueues their uploads.
+hipError_t rebuild(hipStream_t st) {
+    for (Slab &sl : S.slabs) sl.lanes = 0;
+    u32 rows = 0, max_nodes = 0, count = 0;
+    for (u32 t : S.active) {
+        const Request &q = S.reqs[t];
+        Slab &sl = S.slabs[q.r.mech];
+        SimkUpwindSeg &g = S.h_segs[count++];
+        g.x = sl.d + q.off;
+        g.padded = sl.cap;
+        g.courant = sl.d + sl.zone_rows * sl.cap + q.off;
+        g.inflow = q.r.inflow ? S.d_inflow + (u64)q.slot * kMaxSpecies : nullptr;
+        g.fields = sl.n;
+        g.channels = q.r.channels;
+        g.nodes = q.r.nodes;
+        g.row_begin = rows;
+        rows += sl.n * q.r.channels;
+        if (q.r.nodes > max_nodes) max_nodes = q.r.nodes;
+        for (u32 c = 0; c < q.r.cells; ++c) sl.h_order[sl.lanes++] = q.off + c;
+    }
+    S.seg_count = count;
+    S.seg_rows = rows;
+    S.seg_max_nodes = max_nodes;
+    hipError_t e = hipSuccess;
+    if (count) e = hipMemcpyAsync(S.d_segs, S.h_segs, count * sizeof(SimkUpwindSeg), hipMemcpyHostToDevice, st);
+    for (Slab &sl : S.slabs)
+        if (e == hipSuccess && sl.lanes)
+            e = hipMemcpyAsync(sl.d_order, sl.h_order, (u64)sl.lanes * 4, hipMemcpyHostToDevice, st);
+    S.dirty = false;
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
+    if (S.open) {
+        (void)hipStreamSynchronize(simk_module_stream());
+        release_all();
+    }
+    const SimkMultiShape shape = simk_multi_shape();
+    if (shape.count == 0 || !simk_module_stream()) return code(hipErrorNotInitialized);
+    SimkServeConfig c;
+    memset(&c, 0, sizeof(c));
+    if (config) c = *config;
+    if (!c.hbm_budget) c.hbm_budget = 8ull << 30;
+    if (!c.max_requests) c.max_requests = 4096;
+    if (!c.steps) c.steps = 20;
+    if (!c.substeps) c.substeps = 4;
+    if (!(c.dt > 0.0)) c.dt = 1e-3;
+    for (u32 i = 0; i < shape.count; ++i)
+        if (shape.n[i] > kMaxSpecies) return code(hipErrorInvalidValue);
+    S.cfg = c;
+    S.shape = shape;
+    S.st = SimkServeStats();
+    memset(&S.st, 0, sizeof(S.st));
+    hipError_t e = hipSuccess;
+    S.slabs.resize(shape.count);
+    for (u32 i = 0; i < shape.count && e == hipSuccess; ++i) {
+        Slab &sl = S.slabs[i];
+        sl.n = shape.n[i];
+        sl.R = shape.R[i];
+        sl.zone_rows = zone_rows(sl.n, sl.R);
+        const u64 widest = sl.R > sl.n ? sl.R : sl.n;   // row group whose byte offsets must stay below 2^31
+        u64 cap = c.hbm_budget / ((sl.zone_rows + 1) * 8);
+        const u64 lim = ((1ull << 31) - 1) / (widest * 8);
+        if (cap > lim) cap = lim;
+        cap &= ~63ull;
+        if (cap < 64) {
+            e = hipErrorInvalidValue;
+            break;
+        }
+        sl.cap = (u32)cap;
+        sl.free_ranges[0] = sl.cap;
+        e = hipMalloc(&sl.d, (sl.zone_rows + 1) * cap * 8);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_order, cap * 4);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.h_order, cap * 4, 0);
+    }
+    if (e == hipSuccess) e = hipMalloc(&S.d_ws, (u64)shape.resident_waves * shape.slots * 64 * 8);
+    if (e == hipSuccess) e = hipMalloc(&S.d_queue, 256);
+    if (e == hipSuccess) e = hipMalloc(&S.d_inflow, (u64)c.max_requests * kMaxSpecies * 8);
+    if (e == hipSuccess) e = hipMalloc(&S.d_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg));
+    if (e == hipSuccess) e = hipHostMalloc(&S.h_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg), 0);
+    for (hipEvent_t &ev : S.ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        release_all();
+        return code(e);
+    }
+    S.free_slots.reserve(c.max_requests);
+    for (u32 i = c.max_requests; i-- > 0;) S.free_slots.push_back(i);
+    S.open = true;
+    return 0;
+}
+
+GPU_API s32 simk_serve_submit(const SimkServeRequest *request, u32 *ticket) {
+    if (!S.open) return code(hipErrorNotInitialized);
+    if (!request || !ticket) return code(hipErrorInvalidValue);
+    const SimkServeRequest &r = *request;
+    if (r.mech >= S.shape.count || r.cells == 0 || r.channels == 0 || r.nodes == 0 || r.nodes > kMaxNodes ||
+        (u64)r.channels * r.nodes != r.cells || r.padded < r.cells || !r.block || !r.courant || !r.result)
+        return code(hipErrorInvalidValue);
+    const Slab &sl = S.slabs[r.mech];
+    const u64 bytes = (u64)r.cells * (sl.zone_rows + 1) * 8;
+    if (bytes > S.cfg.hbm_budget || r.cells > sl.cap) return code(hipErrorInvalidValue);
+    if (S.free_slots.empty()) return code(hipErrorOutOfMemory);
+    Request q;
+    q.r = r;
+    q.ticket = S.next_ticket++;
+    if (S.next_ticket == 0) S.next_ticket = 1;
+    q.bytes = bytes;
+    q.slot = S.free_slots.back();
+    S.free_slots.pop_back();
+    S.reqs[q.ticket] = q;
+    S.queued.push_back(q.ticket);
+    S.st.submitted++;
+    *ticket = q.ticket;
+    return 0;
+}
+
+GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
+    if (!S.open) return code(hipErrorNotInitialized);
+    if (n_done) *n_done = 0;
+    if (!same_shape(S.shape, simk_multi_shape())) return code(hipErrorInvalidValue);   // mechanisms changed
+    hipStream_t st = simk_module_stream();
+    hipError_t e = hipEventRecord(S.ev[0], st);
+    if (e == hipSuccess) e = admit(st);
+    if (e == hipSuccess) e = hipEventRecord(S.ev[1], st);
+    if (e != hipSuccess) return code(e);
+    std::vector<u32> finished;
+    const bool ran = !S.active.empty();
+    if (ran) {
+        if (S.dirty) e = rebuild(st);
+        if (e == hipSuccess)
+            e = simk_enqueue_upwind_batch(st, S.d_segs, S.seg_count, S.seg_rows, S.seg_max_nodes, S.cfg.substeps);
+        if (e == hipSuccess) e = hipEventRecord(S.ev[2], st);
+        GpuKinLaunch Ls[4];
+        u32 lanes[4];
+        const u32 *orders[4];
+        u64 waves = 0;
+        for (u32 i = 0; i < S.shape.count && e == hipSuccess; ++i) {
+            Slab &sl = S.slabs[i];
+            u64 off[12];
+            zone_offsets(sl.n, sl.R, sl.cap, off);
+            f64 *rows[12];
+            for (int j = 0; j < 12; ++j) rows[j] = sl.d + off[j];
+            GpuKinLaunch &L = Ls[i];
+            memset(&L, 0, sizeof(L));
+            L.y = rows[0]; L.h = rows[1]; L.steps = rows[2]; L.rejects = rows[3]; L.pulses = rows[4];
+            L.failed = rows[5]; L.k = rows[6]; L.src = rows[7]; L.pulse = rows[8]; L.period = rows[9];
+            L.pulse_first = rows[10]; L.pulse_count = rows[11];
+            L.padded = sl.cap;
+            L.lane_count = sl.cap;
+            L.precision = 0;
+            L.duration = S.cfg.dt;
+            lanes[i] = sl.lanes;
+            orders[i] = sl.d_order;
+            waves += (sl.lanes + 63) / 64;
+        }
+        if (waves > S.shape.resident_waves) waves = S.shape.resident_waves;
+        if (waves == 0) waves = 1;
+        if (e == hipSuccess)
+            e = simk_enqueue_multi_ext(S.shape.count, Ls, lanes, orders, 1, S.d_ws, S.d_queue, (u32)waves);
+        if (e == hipSuccess) e = hipEventRecord(S.ev[3], st);
+        // Requests whose last step was just enqueued: copy their solver rows back.
+        for (size_t i = 0; i < S.active.size() && e == hipSuccess;) {
+            Request &q = S.reqs[S.active[i]];
+            if (++q.steps_done < S.cfg.steps) {
+                ++i;
+                continue;
+            }
+            Slab &sl = S.slabs[q.r.mech];
+            e = hipMemcpy2DAsync(q.r.result, (u64)q.r.cells * 8, sl.d + q.off, (u64)sl.cap * 8, (u64)q.r.cells * 8,
+                                 (u64)sl.n + 5, hipMemcpyDeviceToHost, st);
+            finished.push_back(q.ticket);
+            S.active.erase(S.active.begin() + (long)i);
+            S.dirty = true;
+        }
+        if (e == hipSuccess) e = hipEventRecord(S.ev[4], st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return code(e);
+    S.st.iterations++;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, S.ev[0], S.ev[1]) == hipSuccess) S.st.copy_ms += ms;
+    if (ran) {
+        if (hipEventElapsedTime(&ms, S.ev[1], S.ev[2]) == hipSuccess) S.st.advect_ms += ms;
+        if (hipEventElapsedTime(&ms, S.ev[2], S.ev[3]) == hipSuccess) S.st.chem_ms += ms;
+        if (hipEventElapsedTime(&ms, S.ev[3], S.ev[4]) == hipSuccess) S.st.copy_ms += ms;
+    }
+    for (u32 t : finished) {
+        Request &q = S.reqs[t];
+        give_range(S.slabs[q.r.mech], q.off, q.r.cells);
+        S.st.admitted_bytes -= q.bytes;
+        S.free_slots.push_back(q.slot);
+        S.reqs.erase(t);
+        S.st.completed++;
+        S.pending_done.push_back(t);
+    }
+    u32 k = 0;
+    while (k < max_done && k < S.pending_done.size() && done) {
+        done[k] = S.pending_done[k];
+        ++k;
+    }
+    S.pending_done.erase(S.pending_done.begin(), S.pending_done.begin() + k);
+    if (n_done) *n_done = k;
+    return 0;
+}
+
+GPU_API s32 simk_serve_stats(SimkServeStats *out) {
+    if (!out) return code(hipErrorInvalidValue);
+    *out = S.st;
+    out->queued = (u32)S.queued.size();
+    out->active = (u32)S.active.size();
+    return S.open ? 0 : code(hipErrorNotInitialized);
+}
+
+GPU_API s32 simk_serve_close(void) {
+    if (!S.open) return 0;
+    hipError_t e = hipStreamSynchronize(simk_module_stream());
+    release_all();
+    return code(e);
+}
+
+}  // extern "C"

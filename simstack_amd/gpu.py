@@ -28,6 +28,24 @@ class SimkKinInfo(C.Structure):
                 ("signature", C.c_uint64), ("kernel_name", C.c_char * 64)]
 
 
+class SimkServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_requests", C.c_uint32), ("steps", C.c_uint32),
+                ("substeps", C.c_uint32), ("reserved", C.c_uint32), ("dt", C.c_double)]
+
+
+class SimkServeRequest(C.Structure):
+    _fields_ = [("mech", C.c_uint32), ("cells", C.c_uint32), ("channels", C.c_uint32), ("nodes", C.c_uint32),
+                ("padded", C.c_uint32), ("reserved", C.c_uint32), ("block", C.c_void_p), ("courant", C.c_void_p),
+                ("inflow", C.c_void_p), ("result", C.c_void_p)]
+
+
+class SimkServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("queued", C.c_uint32), ("active", C.c_uint32), ("iterations", C.c_uint64),
+                ("admitted_bytes", C.c_uint64), ("peak_admitted_bytes", C.c_uint64), ("advect_ms", C.c_double),
+                ("chem_ms", C.c_double), ("copy_ms", C.c_double)]
+
+
 # Every export the reference's loader resolves (src/gpu/gpu_api.h:112-115).
 REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_used", "gpu_arena_reset",
                      "gpu_copy_to_device", "gpu_copy_to_host", "gpu_sync", "gpu_step_points", "gpu_kin_set_mech",
@@ -37,7 +55,8 @@ EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_s
                      "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
                      "simk_jit_precompile", "simk_copy_device", "simk_upwind_f64", "simk_upwind_f32",
                      "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
-                     "simk_graph_destroy", "simk_jit_precompile_multi")
+                     "simk_graph_destroy", "simk_jit_precompile_multi", "simk_serve_open", "simk_serve_submit",
+                     "simk_serve_step", "simk_serve_stats", "simk_serve_close")
 
 
 def _sig(lib, name, res, *args):
@@ -87,6 +106,11 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_graph_destroy", C.c_int32, C.c_uint32)
     _sig(lib, "simk_upwind_f64", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
     _sig(lib, "simk_upwind_f32", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
+    _sig(lib, "simk_serve_open", C.c_int32, P(SimkServeConfig))
+    _sig(lib, "simk_serve_submit", C.c_int32, P(SimkServeRequest), P(C.c_uint32))
+    _sig(lib, "simk_serve_step", C.c_int32, P(C.c_uint32), C.c_uint32, P(C.c_uint32))
+    _sig(lib, "simk_serve_stats", C.c_int32, P(SimkServeStats))
+    _sig(lib, "simk_serve_close", C.c_int32)
     return lib
 
 
@@ -169,6 +193,50 @@ class GpuModule:
         ms = C.c_float()
         self._check(self.lib.simk_graph_replay(gid, C.byref(ms)), "simk_graph_replay")
         return ms.value
+
+    # ---- serving loop (simk_serve_*) ----
+    def serve_open(self, hbm_budget=0, max_requests=0, steps=0, substeps=0, dt=0.0):
+        cfg = SimkServeConfig(hbm_budget, max_requests, steps, substeps, 0, dt)
+        self._check(self.lib.simk_serve_open(C.byref(cfg)), "simk_serve_open")
+        self._serve_keep = {}
+
+    def serve_submit(self, mech_index, block_ptr, padded, cells, channels, nodes, courant, result, inflow=None,
+                     keep=None):
+        """Queues a request whose input zone block starts at host address
+        block_ptr (row stride padded); result: float64 [species + 5, cells]."""
+        courant = np.ascontiguousarray(courant, dtype=np.float64)
+        inflow = None if inflow is None else np.ascontiguousarray(inflow, dtype=np.float64)
+        assert result.dtype == np.float64 and result.flags.c_contiguous and result.shape[1] == cells
+        req = SimkServeRequest(mech_index, cells, channels, nodes, padded, 0, block_ptr, courant.ctypes.data,
+                               None if inflow is None else inflow.ctypes.data, result.ctypes.data)
+        t = C.c_uint32()
+        self._check(self.lib.simk_serve_submit(C.byref(req), C.byref(t)), "simk_serve_submit")
+        self._serve_keep[t.value] = (courant, inflow, result, keep)
+        return t.value
+
+    def serve_submit_zones(self, mech_index, zones, channels, nodes, courant, inflow=None):
+        """serve_submit of a whole problem.Zones block; returns (ticket, result array)."""
+        result = np.zeros((zones.n + 5, zones.count))
+        t = self.serve_submit(mech_index, zones.block.ctypes.data, zones.padded, zones.count, channels, nodes,
+                              courant, result, inflow, keep=zones)
+        return t, result
+
+    def serve_step(self, max_done=4096):
+        done = (C.c_uint32 * max_done)()
+        n = C.c_uint32()
+        self._check(self.lib.simk_serve_step(done, max_done, C.byref(n)), "simk_serve_step")
+        out = [done[i] for i in range(n.value)]
+        for t in out:
+            self._serve_keep.pop(t, None)
+        return out
+
+    def serve_stats(self):
+        st = SimkServeStats()
+        self._check(self.lib.simk_serve_stats(C.byref(st)), "simk_serve_stats")
+        return st
+
+    def serve_close(self):
+        self._check(self.lib.simk_serve_close(), "simk_serve_close")
 
     def set_grid(self, waves):
         self._check(self.lib.simk_kin_set_grid(waves), "simk_kin_set_grid")

@@ -235,7 +235,7 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the three-mechanism ragged batch")
     ap.add_argument("--ragged-steps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
-    ap.add_argument("--serve-window", type=float, default=1.0, help="seconds of Poisson arrivals")
+    ap.add_argument("--serve-window", type=float, default=0.25, help="seconds of Poisson arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -181,11 +181,14 @@ GPU_API s32 simk_upwind_f32(f32 *x, u64 padded, u32 fields, u32 channels, u32 no
  * simk_serve_submit queues a request. Each simk_serve_step is one iteration
  * of continuous batching: admit queued requests in arrival order while their
  * blocks fit the HBM budget (and a free lane range of their mechanism's
- * slab), advect every admitted request in one batched launch, advance all of
- * them in one multi-mechanism persistent launch, and copy the finished
+ * slab); advect, in one batched launch, every request whose cells all
+ * finished their previous chemistry step; run one time-sliced
+ * multi-mechanism persistent launch over every cell with chemistry pending
+ * (smallest requests first; a cell still integrating when the slice ends
+ * keeps its exact state and continues in the next iteration); copy finished
  * requests' solver rows (y, h, steps, rejects, pulses, failed) into their
  * result buffers. A cell's result does not depend on which other requests
- * share its iterations. Input blocks, courant and inflow arrays must stay
+ * share its iterations or where slices fall. Input blocks, courant and inflow arrays must stay
  * valid and unchanged from submit until the request is admitted, result
  * buffers until it is reported done. Not re-entrant, like the rest of the
  * module. */
@@ -194,7 +197,7 @@ typedef struct SimkServeConfig {
     u32 max_requests;    /* queued + admitted at once (0: 4096) */
     u32 steps;           /* operator-split steps per request (0: 20) */
     u32 substeps;        /* advection substeps per step (0: 4) */
-    u32 reserved;
+    u32 slice_us;        /* chemistry time slice per iteration, microseconds (0: 20000; ~0u: no slicing) */
     f64 dt;              /* chemistry seconds per step (0: 1e-3) */
 } SimkServeConfig;
 
@@ -218,9 +221,11 @@ typedef struct SimkServeStats {
     u64 peak_admitted_bytes;
     f64 advect_ms, chem_ms;      /* summed kernel time (module stream events) */
     f64 copy_ms;                 /* summed admission + completion copy time */
+    u64 carried;                 /* cells stopped by a slice end and continued later */
 } SimkServeStats;
 
-/* Allocates the slabs (one per mechanism, sized so each could hold the whole
+/* Allocates the slabs (one per mechanism: the zone block rows plus a courant
+ * row and five resume rows per lane, sized so each could hold the whole
  * budget, capped where a mechanism's row offsets would pass 2^31 bytes),
  * workspace and staging. Needs simk_kin_set_mechs first; a second open
  * closes the first. */

@@ -560,7 +560,7 @@ GenResult generate_multi(const std::vector<const Mech *> &ms, const KinMethod &m
     w("        simk_u32 m = first + pass;\n        if (m >= MA.count) m -= MA.count;\n");
     w("        switch (m) {\n");
     for (size_t i = 0; i < ms.size(); ++i)
-        w("            case %u: simk_persistent_body<GenOps%u, GenMethod>(MA.a[%u], me); break;\n", (u32)i, (u32)i,
+        w("            case %u: simk_persistent_body<GenOps%u, GenMethod, true>(MA.a[%u], me); break;\n", (u32)i, (u32)i,
           (u32)i);
     w("            default: break;\n        }\n    }\n}\n");
     g.source = std::move(w.s);

@@ -79,7 +79,21 @@ struct SimkKinArgs {
     const SimkDevTables *tables;
     SimkDevCtl ctl;
     SimkDevMethod method;
+    // Time slicing (0: run every cell to the end of its advance). Otherwise a
+    // wave takes no new cell once `slice` ticks of the 100 MHz real-time
+    // counter have passed since it started, and a lane whose cell is between
+    // two step attempts saves the cell's exact integration state (y, h,
+    // failed and the counters to the block, the loop state to `resume`) and
+    // appends the cell to `carry`; the next launch that takes the cell
+    // continues it where it stopped.
+    simk_u64 slice;
+    double *resume;          // [5][padded]: t, pulses applied, repeat, attempt count, flags
+    simk_u32 *carry;         // cells stopped mid-advance
+    simk_u32 *carry_count;
 };
+
+// resume flags: SIMK_RESUME_VALID + reject_last + 2 * reject_more
+#define SIMK_RESUME_VALID 4.0
 
 // ---- lane vocabulary ------------------------------------------------------
 SIMK_DEV double simk_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -338,7 +352,8 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
 // cell's arithmetic is exactly the reference's, whatever lane or order runs
 // it. `repeats` runs that many consecutive advances of `duration` per cell
 // (the result of that many calls with unchanged inputs), carrying h.
-template <class P, class M>
+// SLICED: compile the time-slice paths (A.slice may then be non-zero).
+template <class P, class M, bool SLICED = false>
 SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
     const simk_u32 wave = blockIdx.x;
     const simk_u32 lane = threadIdx.x;
@@ -360,13 +375,20 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
     double h = 0.0, t = 0.0, applied = 0.0, seg_end = 0.0, period = 0.0, first = 0.0, count = 0.0;
     double steps = 0.0, rejects = 0.0, pulses = 0.0, failed = 0.0;
     bool pending = false, reject_last = false, reject_more = false, fresh = !A.queue_first;
+    const simk_u64 t_begin = (SLICED && A.slice) ? __builtin_amdgcn_s_memrealtime() : 0;
+    bool expired = false;
 
     for (;;) {
+        if (SLICED && A.slice && !expired) expired = (__builtin_amdgcn_s_memrealtime() - t_begin) >= A.slice;
         // Resolve each lane's transitions until it has an attempt to make or
         // the queue is exhausted (bounded: every pass consumes a segment, a
         // pulse, an advance or a cell).
         while (phase != SIMK_PH_ATTEMPT && phase != SIMK_PH_EXIT) {
             if (phase == SIMK_PH_CELL) {
+                if (SLICED && expired) {
+                    phase = SIMK_PH_EXIT;
+                    break;
+                }
                 if (!fresh) {
                     // Wave-aggregated refill: one atomic for every lane asking now.
                     const simk_u64 ask = __builtin_amdgcn_read_exec();
@@ -403,6 +425,28 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
                 applied = 0.0;
                 rep = 0;
                 phase = SIMK_PH_SEGMENT;
+                if (SLICED && A.resume) {
+                    SimkRows rr;
+                    rr.init(A.resume, voff, row_bytes, 5u * row_bytes);
+                    const double flags = rr.ld(4);
+                    if (flags >= SIMK_RESUME_VALID) {
+                        // Continue a stopped advance: the loop state as it was
+                        // before the attempt that did not start (the segment's
+                        // h, attempt count and reject flags are not reset).
+                        t = rr.ld(0);
+                        applied = rr.ld(1);
+                        rep = (simk_u32)rr.ld(2);
+                        it = (simk_u32)rr.ld(3);
+                        const simk_u32 f = (simk_u32)(flags - SIMK_RESUME_VALID);
+                        reject_last = (f & 1u) != 0;
+                        reject_more = (f & 2u) != 0;
+                        rr.st(4, 0.0);
+                        pending = applied < count;
+                        const double next_pulse = simk_fma(applied, period, first);
+                        seg_end = pending ? simk_min(next_pulse, A.duration) : A.duration;
+                        phase = SIMK_PH_CHECK;
+                    }
+                }
             } else if (phase == SIMK_PH_SEGMENT) {
                 pending = applied < count;
                 const double next_pulse = simk_fma(applied, period, first);
@@ -412,7 +456,36 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
                 it = 0;
                 phase = ((t < seg_end) && !(failed > 0.0)) ? SIMK_PH_CHECK : SIMK_PH_END;
             } else if (phase == SIMK_PH_CHECK) {
-                if (++it > ctl.max_steps) {
+                if (SLICED && expired) {
+#pragma unroll
+                    for (simk_u32 s = 0; s < n; ++s) ys.st(s, y.get(s));
+                    hs_.st(0, h);
+                    fs.st(0, failed);
+                    SimkRows acc;
+                    acc.init(A.steps, cell * 8u, 0, row_bytes);
+                    acc.st(0, acc.ld(0) + steps);
+                    acc.init(A.rejects, cell * 8u, 0, row_bytes);
+                    acc.st(0, acc.ld(0) + rejects);
+                    acc.init(A.pulses, cell * 8u, 0, row_bytes);
+                    acc.st(0, acc.ld(0) + pulses);
+                    SimkRows rr;
+                    rr.init(A.resume, cell * 8u, row_bytes, 5u * row_bytes);
+                    rr.st(0, t);
+                    rr.st(1, applied);
+                    rr.st(2, (double)rep);
+                    rr.st(3, (double)it);
+                    rr.st(4, SIMK_RESUME_VALID + (reject_last ? 1.0 : 0.0) + (reject_more ? 2.0 : 0.0));
+                    // Wave-aggregated append to the carry list.
+                    const simk_u64 who = __builtin_amdgcn_read_exec();
+                    const simk_u32 rank = __builtin_amdgcn_mbcnt_hi((simk_u32)(who >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((simk_u32)who, 0u));
+                    simk_u32 base = 0;
+                    if (rank == 0) base = atomicAdd(A.carry_count, (simk_u32)__builtin_popcountll(who));
+                    const simk_u32 leader = (simk_u32)__builtin_ctzll(who);
+                    base = (simk_u32)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)base);
+                    A.carry[base + rank] = cell;
+                    phase = SIMK_PH_EXIT;
+                } else if (++it > ctl.max_steps) {
                     failed = 1.0;
                     phase = SIMK_PH_END;
                 } else {

@@ -769,7 +769,7 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
 }
 
 hipError_t enqueue_multi(const MultiPlan &P) {
-    hipLaunchKernelGGL(simk_queues_zero_kernel, dim3(1), dim3(64), 0, g.stream, P.queue, P.MA.count);
+    hipLaunchKernelGGL(simk_queues_zero_kernel, dim3(1), dim3(64), 0, g.stream, P.queue, 8u);
     SimkMultiArgs MA = P.MA;
     size_t size = sizeof(MA);
     void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &MA, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
@@ -799,7 +799,7 @@ SimkMultiShape simk_multi_shape() {
 }
 
 hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *lanes, const u32 *const *orders,
-                                  u32 repeats, double *ws, u32 *queue, u32 waves) {
+                                  u32 repeats, double *ws, u32 *queue, u32 waves, const SimkSlice *slice) {
     if (!g.multi_ok || count != g.multi_mechs.size() || !Ls || !lanes || !ws || !queue || waves == 0 ||
         repeats == 0)
         return hipErrorInvalidValue;
@@ -825,6 +825,13 @@ hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *
         P.MA.a[i] = make_args(Ls + i, lanes[i], g.multi_mechs[i], g.multi_ctl[i], P.slots, repeats, queue + i, 1);
         P.MA.a[i].ws = ws;
         P.MA.a[i].order = orders ? orders[i] : nullptr;
+        if (slice && slice->ticks) {
+            if (!slice->resume[i] || !slice->carry[i]) return hipErrorInvalidValue;
+            P.MA.a[i].slice = slice->ticks;
+            P.MA.a[i].resume = slice->resume[i];
+            P.MA.a[i].carry = slice->carry[i];
+            P.MA.a[i].carry_count = queue + 4 + i;
+        }
         P.MA.start[i] = (u32)(acc / (wsum > 0 ? wsum : 1.0) * P.waves);
         acc += weight[i];
     }

@@ -36,12 +36,21 @@ struct SimkMultiShape {
 };
 SimkMultiShape simk_multi_shape();
 
+// Time slicing of a multi-mechanism launch (see SimkKinArgs::slice).
+struct SimkSlice {
+    u64 ticks;               // 100 MHz real-time counter ticks per wave (0: off)
+    double *resume[4];       // per mechanism: [5][padded] resume rows
+    u32 *carry[4];           // per mechanism: stopped cells, waves * 64 entries
+};
+
 // Enqueues the queue reset and one multi-mechanism persistent launch on the
 // module stream. Ls[i]: mechanism i's block (duration, padded, pointers);
 // lanes[i]: queue positions of mechanism i (may be 0); orders[i]: device
-// position -> lane map (null: identity). ws: waves * slots * 64 f64, queue: 4
-// u32, both owned by the caller.
+// position -> lane map (null: identity). ws: waves * slots * 64 f64; queue:
+// 8 u32 (queue heads [0, 4), carry counts [4, 8)), both owned by the caller.
+// After the launch queue[i] (clamped to lanes[i]) is the number of queue
+// positions taken and queue[4 + i] the number of carried cells.
 hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *lanes, const u32 *const *orders,
-                                  u32 repeats, double *ws, u32 *queue, u32 waves);
+                                  u32 repeats, double *ws, u32 *queue, u32 waves, const SimkSlice *slice);
 
 #endif  // SIMK_MODULE_INTERNAL_H

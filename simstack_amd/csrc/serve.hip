@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <deque>
 #include <map>
 #include <unordered_map>
@@ -26,25 +27,35 @@ namespace {
 
 constexpr u32 kMaxSpecies = 64;
 constexpr u32 kMaxNodes = 8192;
+constexpr u32 kResumeRows = 5;
 
 struct Slab {
-    double *d = nullptr;     // zone block rows + courant row, row stride cap
+    double *d = nullptr;     // zone block rows, courant row, resume rows; row stride cap
     u32 cap = 0;
     u32 n = 0, R = 0;
     u64 zone_rows = 0;       // 3n + R + 8
     std::map<u32, u32> free_ranges;   // offset -> length
+    std::vector<u32> owner;  // lane -> ticket of the request holding it (0: none)
     u32 *d_order = nullptr;  // queue position -> lane
     u32 *h_order = nullptr;  // pinned staging
     u32 lanes = 0;           // current order length
+    u32 *d_carry = nullptr;  // cells stopped by the slice end
+    u32 *h_carry = nullptr;
+    double *courant() const { return d + zone_rows * cap; }
+    double *resume() const { return d + (zone_rows + 1) * cap; }
 };
+
+enum { kAdvect = 0, kChem = 1 };
 
 struct Request {
     SimkServeRequest r;
     u32 ticket = 0;
     u32 off = 0;
     u32 steps_done = 0;
+    u32 phase = kAdvect;
     u64 bytes = 0;
-    u32 slot = 0;            // inflow slot
+    u32 slot = 0;                // inflow slot
+    std::vector<u32> remaining;  // lanes whose chemistry step is pending: stopped cells first
 };
 
 struct Server {
@@ -54,16 +65,14 @@ struct Server {
     std::vector<Slab> slabs;
     std::unordered_map<u32, Request> reqs;
     std::deque<u32> queued;          // tickets in arrival order
-    std::vector<u32> active;         // tickets in admission order
+    std::vector<u32> active;         // admitted tickets
     std::vector<u32> free_slots;
     std::vector<u32> pending_done;   // finished, not yet reported
     u32 next_ticket = 1;
     double *d_ws = nullptr;
-    u32 *d_queue = nullptr;
+    u32 *d_queue = nullptr, *h_queue = nullptr;
     double *d_inflow = nullptr;      // [max_requests][kMaxSpecies]
     SimkUpwindSeg *d_segs = nullptr, *h_segs = nullptr;
-    u32 seg_count = 0, seg_rows = 0, seg_max_nodes = 0;
-    bool dirty = true;               // membership changed since the order/segments were built
     hipEvent_t ev[5] = {};
     SimkServeStats st;
 };
@@ -117,10 +126,13 @@ void release_all() {
         if (sl.d) (void)hipFree(sl.d);
         if (sl.d_order) (void)hipFree(sl.d_order);
         if (sl.h_order) (void)hipHostFree(sl.h_order);
+        if (sl.d_carry) (void)hipFree(sl.d_carry);
+        if (sl.h_carry) (void)hipHostFree(sl.h_carry);
     }
     S.slabs.clear();
     if (S.d_ws) (void)hipFree(S.d_ws);
     if (S.d_queue) (void)hipFree(S.d_queue);
+    if (S.h_queue) (void)hipHostFree(S.h_queue);
     if (S.d_inflow) (void)hipFree(S.d_inflow);
     if (S.d_segs) (void)hipFree(S.d_segs);
     if (S.h_segs) (void)hipHostFree(S.h_segs);
@@ -150,34 +162,34 @@ hipError_t admit(hipStream_t st) {
         hipError_t e = hipMemcpy2DAsync(sl.d + off, cap * 8, q.r.block, (u64)q.r.padded * 8, (u64)q.r.cells * 8,
                                         sl.zone_rows, hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(sl.d + sl.zone_rows * cap + off, q.r.courant, (u64)q.r.cells * 8,
-                               hipMemcpyHostToDevice, st);
+            e = hipMemcpyAsync(sl.courant() + off, q.r.courant, (u64)q.r.cells * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess && q.r.inflow)
             e = hipMemcpyAsync(S.d_inflow + (u64)q.slot * kMaxSpecies, q.r.inflow, (u64)sl.n * 8,
                                hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
+        for (u32 c = 0; c < q.r.cells; ++c) sl.owner[off + c] = q.ticket;
+        q.phase = kAdvect;
         S.st.admitted_bytes += q.bytes;
         if (S.st.admitted_bytes > S.st.peak_admitted_bytes) S.st.peak_admitted_bytes = S.st.admitted_bytes;
         S.st.admitted++;
         S.active.push_back(q.ticket);
         S.queued.pop_front();
-        S.dirty = true;
     }
     return hipSuccess;
 }
 
-// Rebuilds the advection segments and the per-slab order arrays of the
-// admitted requests and enqueues their uploads.
-hipError_t rebuild(hipStream_t st) {
-    for (Slab &sl : S.slabs) sl.lanes = 0;
+// Advects every request whose next step starts now; they enter the chemistry
+// phase with all their lanes pending.
+hipError_t advect(hipStream_t st) {
     u32 rows = 0, max_nodes = 0, count = 0;
     for (u32 t : S.active) {
-        const Request &q = S.reqs[t];
+        Request &q = S.reqs[t];
+        if (q.phase != kAdvect) continue;
         Slab &sl = S.slabs[q.r.mech];
         SimkUpwindSeg &g = S.h_segs[count++];
         g.x = sl.d + q.off;
         g.padded = sl.cap;
-        g.courant = sl.d + sl.zone_rows * sl.cap + q.off;
+        g.courant = sl.courant() + q.off;
         g.inflow = q.r.inflow ? S.d_inflow + (u64)q.slot * kMaxSpecies : nullptr;
         g.fields = sl.n;
         g.channels = q.r.channels;
@@ -185,18 +197,68 @@ hipError_t rebuild(hipStream_t st) {
         g.row_begin = rows;
         rows += sl.n * q.r.channels;
         if (q.r.nodes > max_nodes) max_nodes = q.r.nodes;
-        for (u32 c = 0; c < q.r.cells; ++c) sl.h_order[sl.lanes++] = q.off + c;
+        q.phase = kChem;
+        q.remaining.resize(q.r.cells);
+        for (u32 c = 0; c < q.r.cells; ++c) q.remaining[c] = q.off + c;
     }
-    S.seg_count = count;
-    S.seg_rows = rows;
-    S.seg_max_nodes = max_nodes;
+    if (!count) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(S.d_segs, S.h_segs, count * sizeof(SimkUpwindSeg), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = simk_enqueue_upwind_batch(st, S.d_segs, count, rows, max_nodes, S.cfg.substeps);
+    return e;
+}
+
+// Queue order of every slab: requests with chemistry pending, smallest first
+// (then oldest), each with its stopped cells before its untouched ones.
+hipError_t build_orders(hipStream_t st, u64 &waves) {
+    std::vector<const Request *> order;
+    for (u32 t : S.active) {
+        const Request &q = S.reqs[t];
+        if (q.phase == kChem) order.push_back(&q);
+    }
+    std::sort(order.begin(), order.end(), [](const Request *a, const Request *b) {
+        return a->r.cells != b->r.cells ? a->r.cells < b->r.cells : a->ticket < b->ticket;
+    });
+    for (Slab &sl : S.slabs) sl.lanes = 0;
+    for (const Request *q : order) {
+        Slab &sl = S.slabs[q->r.mech];
+        memcpy(sl.h_order + sl.lanes, q->remaining.data(), q->remaining.size() * 4);
+        sl.lanes += (u32)q->remaining.size();
+    }
+    waves = 0;
     hipError_t e = hipSuccess;
-    if (count) e = hipMemcpyAsync(S.d_segs, S.h_segs, count * sizeof(SimkUpwindSeg), hipMemcpyHostToDevice, st);
-    for (Slab &sl : S.slabs)
+    for (Slab &sl : S.slabs) {
+        waves += (sl.lanes + 63) / 64;
         if (e == hipSuccess && sl.lanes)
             e = hipMemcpyAsync(sl.d_order, sl.h_order, (u64)sl.lanes * 4, hipMemcpyHostToDevice, st);
-    S.dirty = false;
+    }
     return e;
+}
+
+// After a launch: the cells each request still has pending are the ones the
+// slice stopped (carry lists) and the queue positions no lane took.
+void collect(std::vector<u32> &step_done) {
+    for (u32 t : S.active) {
+        Request &q = S.reqs[t];
+        if (q.phase == kChem) q.remaining.clear();
+    }
+    for (u32 i = 0; i < S.shape.count; ++i) {
+        Slab &sl = S.slabs[i];
+        const u32 head = S.h_queue[i] < sl.lanes ? S.h_queue[i] : sl.lanes;
+        const u32 carried = S.h_queue[4 + i];
+        S.st.carried += carried;
+        for (u32 j = 0; j < carried; ++j) {
+            const u32 cell = sl.h_carry[j];
+            S.reqs[sl.owner[cell]].remaining.push_back(cell);
+        }
+        for (u32 pos = head; pos < sl.lanes; ++pos) {
+            const u32 cell = sl.h_order[pos];
+            S.reqs[sl.owner[cell]].remaining.push_back(cell);
+        }
+    }
+    for (u32 t : S.active) {
+        Request &q = S.reqs[t];
+        if (q.phase == kChem && q.remaining.empty()) step_done.push_back(t);
+    }
 }
 
 }  // namespace
@@ -217,13 +279,15 @@ GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
     if (!c.max_requests) c.max_requests = 4096;
     if (!c.steps) c.steps = 20;
     if (!c.substeps) c.substeps = 4;
+    if (!c.slice_us) c.slice_us = 20000;
     if (!(c.dt > 0.0)) c.dt = 1e-3;
     for (u32 i = 0; i < shape.count; ++i)
         if (shape.n[i] > kMaxSpecies) return code(hipErrorInvalidValue);
     S.cfg = c;
     S.shape = shape;
-    S.st = SimkServeStats();
     memset(&S.st, 0, sizeof(S.st));
+    hipStream_t st = simk_module_stream();
+    const u64 carry_cap = (u64)shape.resident_waves * 64;
     hipError_t e = hipSuccess;
     S.slabs.resize(shape.count);
     for (u32 i = 0; i < shape.count && e == hipSuccess; ++i) {
@@ -231,28 +295,37 @@ GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
         sl.n = shape.n[i];
         sl.R = shape.R[i];
         sl.zone_rows = zone_rows(sl.n, sl.R);
+        const u64 rows = sl.zone_rows + 1 + kResumeRows;
         const u64 widest = sl.R > sl.n ? sl.R : sl.n;   // row group whose byte offsets must stay below 2^31
-        u64 cap = c.hbm_budget / ((sl.zone_rows + 1) * 8);
-        const u64 lim = ((1ull << 31) - 1) / (widest * 8);
+        // Rounded up, so any request within the budget fits an empty slab.
+        u64 cap = (c.hbm_budget / (rows * 8) + 63) & ~63ull;
+        const u64 lim = (((1ull << 31) - 1) / (widest * 8)) & ~63ull;
         if (cap > lim) cap = lim;
-        cap &= ~63ull;
         if (cap < 64) {
             e = hipErrorInvalidValue;
             break;
         }
         sl.cap = (u32)cap;
         sl.free_ranges[0] = sl.cap;
-        e = hipMalloc(&sl.d, (sl.zone_rows + 1) * cap * 8);
+        sl.owner.assign(cap, 0);
+        e = hipMalloc(&sl.d, rows * cap * 8);
+        // Resume flags start clear; a lane's flag is set only while its cell
+        // is stopped mid-step, and cleared when the cell continues.
+        if (e == hipSuccess) e = hipMemsetAsync(sl.resume(), 0, kResumeRows * cap * 8, st);
         if (e == hipSuccess) e = hipMalloc(&sl.d_order, cap * 4);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_order, cap * 4, 0);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_carry, carry_cap * 4);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.h_carry, carry_cap * 4, 0);
     }
     if (e == hipSuccess) e = hipMalloc(&S.d_ws, (u64)shape.resident_waves * shape.slots * 64 * 8);
     if (e == hipSuccess) e = hipMalloc(&S.d_queue, 256);
+    if (e == hipSuccess) e = hipHostMalloc(&S.h_queue, 256, 0);
     if (e == hipSuccess) e = hipMalloc(&S.d_inflow, (u64)c.max_requests * kMaxSpecies * 8);
     if (e == hipSuccess) e = hipMalloc(&S.d_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg));
     if (e == hipSuccess) e = hipHostMalloc(&S.h_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg), 0);
     for (hipEvent_t &ev : S.ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         release_all();
         return code(e);
@@ -271,7 +344,7 @@ GPU_API s32 simk_serve_submit(const SimkServeRequest *request, u32 *ticket) {
         (u64)r.channels * r.nodes != r.cells || r.padded < r.cells || !r.block || !r.courant || !r.result)
         return code(hipErrorInvalidValue);
     const Slab &sl = S.slabs[r.mech];
-    const u64 bytes = (u64)r.cells * (sl.zone_rows + 1) * 8;
+    const u64 bytes = (u64)r.cells * (sl.zone_rows + 1 + kResumeRows) * 8;
     if (bytes > S.cfg.hbm_budget || r.cells > sl.cap) return code(hipErrorInvalidValue);
     if (S.free_slots.empty()) return code(hipErrorOutOfMemory);
     Request q;
@@ -296,19 +369,22 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
     hipError_t e = hipEventRecord(S.ev[0], st);
     if (e == hipSuccess) e = admit(st);
     if (e == hipSuccess) e = hipEventRecord(S.ev[1], st);
+    if (e == hipSuccess) e = advect(st);
+    if (e == hipSuccess) e = hipEventRecord(S.ev[2], st);
     if (e != hipSuccess) return code(e);
-    std::vector<u32> finished;
-    const bool ran = !S.active.empty();
+    u64 waves = 0;
+    e = build_orders(st, waves);
+    if (e != hipSuccess) return code(e);
+    const bool ran = waves > 0;
+    std::vector<u32> step_done, finished;
     if (ran) {
-        if (S.dirty) e = rebuild(st);
-        if (e == hipSuccess)
-            e = simk_enqueue_upwind_batch(st, S.d_segs, S.seg_count, S.seg_rows, S.seg_max_nodes, S.cfg.substeps);
-        if (e == hipSuccess) e = hipEventRecord(S.ev[2], st);
         GpuKinLaunch Ls[4];
         u32 lanes[4];
         const u32 *orders[4];
-        u64 waves = 0;
-        for (u32 i = 0; i < S.shape.count && e == hipSuccess; ++i) {
+        SimkSlice slice;
+        memset(&slice, 0, sizeof(slice));
+        slice.ticks = S.cfg.slice_us == 0xFFFFFFFFu ? 0 : (u64)S.cfg.slice_us * 100;   // 100 MHz counter
+        for (u32 i = 0; i < S.shape.count; ++i) {
             Slab &sl = S.slabs[i];
             u64 off[12];
             zone_offsets(sl.n, sl.R, sl.cap, off);
@@ -325,44 +401,55 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
             L.duration = S.cfg.dt;
             lanes[i] = sl.lanes;
             orders[i] = sl.d_order;
-            waves += (sl.lanes + 63) / 64;
+            slice.resume[i] = sl.resume();
+            slice.carry[i] = sl.d_carry;
         }
         if (waves > S.shape.resident_waves) waves = S.shape.resident_waves;
-        if (waves == 0) waves = 1;
-        if (e == hipSuccess)
-            e = simk_enqueue_multi_ext(S.shape.count, Ls, lanes, orders, 1, S.d_ws, S.d_queue, (u32)waves);
+        e = simk_enqueue_multi_ext(S.shape.count, Ls, lanes, orders, 1, S.d_ws, S.d_queue, (u32)waves, &slice);
         if (e == hipSuccess) e = hipEventRecord(S.ev[3], st);
-        // Requests whose last step was just enqueued: copy their solver rows back.
-        for (size_t i = 0; i < S.active.size() && e == hipSuccess;) {
-            Request &q = S.reqs[S.active[i]];
+        if (e == hipSuccess) e = hipMemcpyAsync(S.h_queue, S.d_queue, 8 * 4, hipMemcpyDeviceToHost, st);
+        for (u32 i = 0; i < S.shape.count && e == hipSuccess; ++i)
+            if (slice.ticks)
+                e = hipMemcpyAsync(S.slabs[i].h_carry, S.slabs[i].d_carry, (u64)waves * 64 * 4,
+                                   hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return code(e);
+        if (!slice.ticks)
+            for (u32 i = 0; i < S.shape.count; ++i) S.h_queue[4 + i] = 0;
+        collect(step_done);
+        // Requests whose last step just ended: copy their solver rows back.
+        for (u32 t : step_done) {
+            Request &q = S.reqs[t];
             if (++q.steps_done < S.cfg.steps) {
-                ++i;
+                q.phase = kAdvect;
                 continue;
             }
             Slab &sl = S.slabs[q.r.mech];
             e = hipMemcpy2DAsync(q.r.result, (u64)q.r.cells * 8, sl.d + q.off, (u64)sl.cap * 8, (u64)q.r.cells * 8,
                                  (u64)sl.n + 5, hipMemcpyDeviceToHost, st);
-            finished.push_back(q.ticket);
-            S.active.erase(S.active.begin() + (long)i);
-            S.dirty = true;
+            if (e != hipSuccess) return code(e);
+            finished.push_back(t);
         }
-        if (e == hipSuccess) e = hipEventRecord(S.ev[4], st);
     }
+    if (e == hipSuccess) e = hipEventRecord(S.ev[4], st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return code(e);
     S.st.iterations++;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, S.ev[0], S.ev[1]) == hipSuccess) S.st.copy_ms += ms;
+    if (hipEventElapsedTime(&ms, S.ev[1], S.ev[2]) == hipSuccess) S.st.advect_ms += ms;
     if (ran) {
-        if (hipEventElapsedTime(&ms, S.ev[1], S.ev[2]) == hipSuccess) S.st.advect_ms += ms;
         if (hipEventElapsedTime(&ms, S.ev[2], S.ev[3]) == hipSuccess) S.st.chem_ms += ms;
         if (hipEventElapsedTime(&ms, S.ev[3], S.ev[4]) == hipSuccess) S.st.copy_ms += ms;
     }
     for (u32 t : finished) {
         Request &q = S.reqs[t];
-        give_range(S.slabs[q.r.mech], q.off, q.r.cells);
+        Slab &sl = S.slabs[q.r.mech];
+        give_range(sl, q.off, q.r.cells);
+        for (u32 c = 0; c < q.r.cells; ++c) sl.owner[q.off + c] = 0;
         S.st.admitted_bytes -= q.bytes;
         S.free_slots.push_back(q.slot);
+        S.active.erase(std::find(S.active.begin(), S.active.end(), t));
         S.reqs.erase(t);
         S.st.completed++;
         S.pending_done.push_back(t);

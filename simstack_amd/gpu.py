@@ -30,7 +30,7 @@ class SimkKinInfo(C.Structure):
 
 class SimkServeConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_requests", C.c_uint32), ("steps", C.c_uint32),
-                ("substeps", C.c_uint32), ("reserved", C.c_uint32), ("dt", C.c_double)]
+                ("substeps", C.c_uint32), ("slice_us", C.c_uint32), ("dt", C.c_double)]
 
 
 class SimkServeRequest(C.Structure):
@@ -43,7 +43,7 @@ class SimkServeStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
                 ("queued", C.c_uint32), ("active", C.c_uint32), ("iterations", C.c_uint64),
                 ("admitted_bytes", C.c_uint64), ("peak_admitted_bytes", C.c_uint64), ("advect_ms", C.c_double),
-                ("chem_ms", C.c_double), ("copy_ms", C.c_double)]
+                ("chem_ms", C.c_double), ("copy_ms", C.c_double), ("carried", C.c_uint64)]
 
 
 # Every export the reference's loader resolves (src/gpu/gpu_api.h:112-115).
@@ -195,8 +195,8 @@ class GpuModule:
         return ms.value
 
     # ---- serving loop (simk_serve_*) ----
-    def serve_open(self, hbm_budget=0, max_requests=0, steps=0, substeps=0, dt=0.0):
-        cfg = SimkServeConfig(hbm_budget, max_requests, steps, substeps, 0, dt)
+    def serve_open(self, hbm_budget=0, max_requests=0, steps=0, substeps=0, dt=0.0, slice_us=0):
+        cfg = SimkServeConfig(hbm_budget, max_requests, steps, substeps, slice_us, dt)
         self._check(self.lib.simk_serve_open(C.byref(cfg)), "simk_serve_open")
         self._serve_keep = {}
 

@@ -24,12 +24,12 @@ def pools():
     return serving.Pools(cells=6000)
 
 
-def _serve(gm, pools, reqs, budget=8 << 30, submit_after=None):
+def _serve(gm, pools, reqs, budget=8 << 30, submit_after=None, slice_us=0):
     """reqs: list of (mech, first, channels, nodes). submit_after[i] = iterations
     to run before submitting request i (default 0). Returns the results."""
     meth, ctl = problem.method(2), problem.control()
     gm.set_mechs(pools.mechs, meth, ctl)
-    gm.serve_open(hbm_budget=budget, max_requests=16, steps=STEPS, substeps=SUBSTEPS, dt=DT)
+    gm.serve_open(hbm_budget=budget, max_requests=16, steps=STEPS, substeps=SUBSTEPS, dt=DT, slice_us=slice_us)
     submit_after = submit_after or [0] * len(reqs)
     results, ticket_of = {}, {}
     it = 0
@@ -106,11 +106,24 @@ def test_request_same_alone_in_mix_and_oracle(gm, pools):
         assert compare_bits(got, want)["y"] == 0
 
 
+def test_time_slices_keep_bits(gm, pools):
+    """Slices of 0.3 ms stop cells mid-step many times (their state saved to
+    the block and resume rows, continued in later launches); the results equal
+    the unsliced run's bit for bit."""
+    reqs = [(0, 0, 1, 2000), (2, 100, 1, 300), (1, 0, 2, 400)]
+    whole, st_whole = _serve(gm, pools, reqs, slice_us=0xFFFFFFFF)
+    sliced, st = _serve(gm, pools, reqs, slice_us=300)
+    assert st_whole.carried == 0
+    assert st.carried > 0 and st.iterations > st_whole.iterations
+    for i in range(len(reqs)):
+        assert whole[i].tobytes() == sliced[i].tobytes()
+
+
 def test_budget_admission_serialises_and_keeps_results(gm, pools):
     reqs = [(1, 0, 2, 500), (1, 1000, 2, 500), (0, 3000, 1, 1000)]
     free, st_free = _serve(gm, pools, reqs)
-    rows1 = 3 * pools.mechs[1].n + pools.mechs[1].R + 9
-    rows0 = 3 * pools.mechs[0].n + pools.mechs[0].R + 9
+    rows1 = 3 * pools.mechs[1].n + pools.mechs[1].R + 14     # zone rows + courant + 5 resume rows
+    rows0 = 3 * pools.mechs[0].n + pools.mechs[0].R + 14
     budget = max(1000 * rows1 * 8, 1000 * rows0 * 8)   # one request at a time
     tight, st = _serve(gm, pools, reqs, budget=budget)
     assert st.peak_admitted_bytes <= budget

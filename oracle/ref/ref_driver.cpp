@@ -125,7 +125,7 @@ struct RefLaunch {
     f64 *y, *h, *steps, *rejects, *pulses, *failed;
     const f64 *k, *src, *pulse, *period, *pulse_first, *pulse_count;
     u32 padded;
-    u32 lane_width;   // 1: scalar f64 lanes, 4: f64x4 (AVX2)
+    u32 lane_width;   // 1: scalar f64 lanes, 4: f64x4 (AVX2); f32 delta lanes: 32 scalar, 8 f32x8 (AVX2)
     f64 duration;
 };
 
@@ -147,6 +147,12 @@ static void ref_run(const RefWork *w) {
     if (L->lane_width == 4) {
         for (u32 i0 = w->first; i0 < w->end; i0 += 4)
             dbd_advance_group<f64x4>(w->m, w->method, w->ctl, a, i0, L->duration);
+    } else if (L->lane_width == 8) {
+        for (u32 i0 = w->first; i0 < w->end; i0 += 8)
+            dbd_advance_group<f32x8, KinOpsDefault, true>(w->m, w->method, w->ctl, a, i0, L->duration);
+    } else if (L->lane_width == 32) {
+        for (u32 i0 = w->first; i0 < w->end; ++i0)
+            dbd_advance_group<f32, KinOpsDefault, true>(w->m, w->method, w->ctl, a, i0, L->duration);
     } else {
         for (u32 i0 = w->first; i0 < w->end; ++i0) dbd_advance_group<f64>(w->m, w->method, w->ctl, a, i0, L->duration);
     }
@@ -173,7 +179,7 @@ static void *ref_worker(void *arg) {
 // to a pool, as dbd_advance_threads does; results do not depend on threads.
 REF_API s32 ref_advance(const void *mech, const void *method, const void *ctl, const RefLaunch *L, u32 lanes,
                         u32 threads) {
-    const u32 w = L->lane_width == 4 ? 4 : 1;
+    const u32 w = L->lane_width == 4 ? 4 : (L->lane_width == 8 ? 8 : 1);
     if (lanes % w) return 1;
     u32 groups = lanes / w;
     if (groups == 0) return 0;

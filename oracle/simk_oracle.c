@@ -525,6 +525,308 @@ ORACLE_API s32 oracle_advance(const void *mech, const SimkCaps *caps, const KinM
     return 0;
 }
 
+/* ---- f32 delta lanes (GpuKinLaunch precision 1) ----------------------------
+ * dbd_advance_group<f32, KinOpsDefault, Delta = true> (src/sim/dbd_lanes.h:28-78)
+ * around kin_advance / kin_integrate<f32, Delta> (kin_solver.h:189-347), one
+ * scalar lane:
+ *  - every input is rounded to f32 on load (lane_load_f64, base_lane.h:101);
+ *    method and control constants are rounded by lane_splat (base_lane.h:98);
+ *  - y holds the increment over base = f32(y_f64), starting at 0, and every
+ *    rate and Jacobian is evaluated at base + y (kin_solver.h:221-229, 245-250);
+ *  - the error scale uses |base + y| and |base + y_new| (:268-270) and the
+ *    clamp keeps base + y >= 0 (:288-291);
+ *  - the increment is added to the f64 storage, which is then clamped at 0
+ *    (dbd_lanes.h:58-67); h and failed are stored widened, the per-call
+ *    counters added. */
+static inline f32 of_min(f32 a, f32 b) { return a < b ? a : b; }
+static inline f32 of_max(f32 a, f32 b) { return a > b ? a : b; }
+static inline f32 of_abs(f32 a) { return a < 0.0f ? -a : a; }
+static inline int of_is_finite(f32 a) { f32 d = a - a; return d == d; }
+
+static void of_fun(const OMech *m, const f32 *k, const f32 *src, const f32 *y, f32 *f) {
+    for (u32 s = 0; s < m->n; ++s) f[s] = src[s];
+    for (u32 r = 0; r < m->R; ++r) {
+        const KinReaction *rx = m->rx + r;
+        f32 rate = k[r];
+        for (u32 p = 0; p < rx->reactant_count; ++p) rate = rate * y[rx->reactant[p]];
+        for (u32 e = 0; e < rx->net_count; ++e) {
+            u32 s = rx->net_species[e];
+            f[s] = fmaf((f32)(f64)rx->net_coef[e], rate, f[s]);
+        }
+    }
+}
+
+static void of_neg_jac(const OMech *m, const f32 *k, const f32 *y, f32 *a) {
+    for (u32 i = 0; i < m->nnz; ++i) a[i] = 0.0f;
+    for (u32 r = 0; r < m->R; ++r) {
+        const KinReaction *rx = m->rx + r;
+        for (u32 p = 0; p < rx->reactant_count; ++p) {
+            f32 d = k[r];
+            for (u32 q = 0; q < rx->reactant_count; ++q)
+                if (q != p) d = d * y[rx->reactant[q]];
+            for (u32 e = 0; e < rx->net_count; ++e) {
+                u32 idx = rx->jac_index[p][e];
+                a[idx] = fmaf((f32)(-(f64)rx->net_coef[e]), d, a[idx]);
+            }
+        }
+    }
+}
+
+static void of_factor(const OMech *m, f32 ghinv, f32 *a) {
+    for (u32 i = 0; i < m->n; ++i) {
+        for (u32 d = m->div_begin[i]; d < m->div_begin[i + 1]; ++d) {
+            KinLuDiv div = m->lu_div[d];
+            f32 l = a[div.ik] * a[div.kk];
+            a[div.ik] = l;
+            for (u32 s = div.sub_begin; s < div.sub_end; ++s) {
+                KinLuSub sub = m->lu_sub[s];
+                a[sub.ij] = fmaf(-l, a[sub.kj], a[sub.ij]);
+            }
+        }
+        u32 ii = m->diag[i];
+        a[ii] = 1.0f / (a[ii] + ghinv);
+    }
+}
+
+static void of_solve(const OMech *m, const f32 *a, f32 *x) {
+    const u32 n = m->n;
+    for (u32 i = 0; i < n; ++i)
+        for (u32 e = m->lower_begin[i]; e < m->lower_begin[i + 1]; ++e)
+            x[i] = fmaf(-a[m->lower[e].index], x[m->lower[e].col], x[i]);
+    for (u32 ii = n; ii-- > 0;) {
+        for (u32 e = m->upper_begin[ii]; e < m->upper_begin[ii + 1]; ++e)
+            x[ii] = fmaf(-a[m->upper[e].index], x[m->upper[e].col], x[ii]);
+        x[ii] = x[ii] * a[m->diag[ii]];
+    }
+}
+
+/* lane_cbrt_clamped<f32> and kin_step_factor<f32>: the f64 forms with every
+ * constant rounded to f32 (base_lane.h:123-135, kin_solver.h:169-180). */
+ORACLE_API f32 oracle_cbrt_clamped_f32(f32 x) {
+    x = of_min(of_max(x, (f32)1e-4), (f32)1e4);
+    f32 r4 = sqrtf(sqrtf(x));
+    f32 r16 = sqrtf(sqrtf(r4));
+    f32 r64 = sqrtf(sqrtf(r16));
+    f32 g = r4 * r16 * r64;
+    const f32 two = 2.0f, third = (f32)(1.0 / 3.0);
+    g = (two * g + x / (g * g)) * third;
+    g = (two * g + x / (g * g)) * third;
+    return g;
+}
+
+ORACLE_API f32 oracle_step_factor_f32(f32 err, u32 order) {
+    f32 e = of_min(of_max(err, (f32)1e-4), (f32)1e4);
+    f32 root;
+    if (order == 2) root = sqrtf(e);
+    else if (order == 4) root = sqrtf(sqrtf(e));
+    else root = oracle_cbrt_clamped_f32(e);
+    f32 fac = (f32)0.9 / root;
+    return of_min(of_max(fac, (f32)0.2), (f32)6.0);
+}
+
+typedef struct OWork32 {
+    f32 *lu, *kstage, *fcn, *ytmp, *yeval; /* nnz, 4*n, n, n, n */
+} OWork32;
+
+static void of_integrate(const OMech *m, const KinMethod *method, const KinControl *ctl, const f32 *k, const f32 *src,
+                         const f32 *base, f32 *y, f32 *t_io, f32 t_end, f32 *h_io, f32 *stats, OWork32 *w) {
+    const u32 n = m->n;
+    const f32 one = 1.0f, zero = 0.0f;
+    const f32 h_min = (f32)ctl->h_min, rtol = (f32)ctl->rtol, atol = (f32)ctl->atol;
+    const f32 inv_n = (f32)(1.0 / (f64)n);
+    f32 t = *t_io, h = *h_io;
+    f32 *lu = w->lu, *fcn = w->fcn, *ytmp = w->ytmp, *yeval = w->yeval;
+#define KST(st, s) w->kstage[(st) * n + (s)]
+
+    int failed = stats[3] > zero;
+    int active = (t < t_end) && !failed;
+    int reject_last = 0, reject_more = 0;
+    h = h > zero ? h : (f32)ctl->h_start;
+
+    u32 iterations = 0;
+    while (active) {
+        if (++iterations > ctl->max_steps) {
+            stats[3] = one;
+            break;
+        }
+        f32 remaining = t_end - t;
+        f32 h_cap = remaining;
+        if (ctl->h_max > 0.0) h_cap = of_min(h_cap, (f32)ctl->h_max);
+        f32 h_wanted = of_max(h, h_min);
+        int last = h_wanted >= remaining;
+        f32 hs = last ? remaining : of_min(h_wanted, h_cap);
+
+        for (u32 s = 0; s < n; ++s) yeval[s] = base[s] + y[s];
+        of_fun(m, k, src, yeval, fcn);
+        of_neg_jac(m, k, yeval, lu);
+        f32 ghinv = one / (hs * (f32)method->gamma0);
+        of_factor(m, ghinv, lu);
+        f32 hinv = one / hs;
+
+        for (u32 st = 0; st < method->stages; ++st) {
+            u32 packed = st * (st - 1) / 2;
+            if (st > 0 && method->new_f[st]) {
+                for (u32 s = 0; s < n; ++s) ytmp[s] = y[s];
+                for (u32 j = 0; j < st; ++j) {
+                    f64 a = method->a[packed + j];
+                    if (a == 0.0) continue;
+                    for (u32 s = 0; s < n; ++s) ytmp[s] = fmaf((f32)a, KST(j, s), ytmp[s]);
+                }
+                for (u32 s = 0; s < n; ++s) yeval[s] = base[s] + ytmp[s];
+                of_fun(m, k, src, yeval, fcn);
+            }
+            for (u32 s = 0; s < n; ++s) KST(st, s) = fcn[s];
+            for (u32 j = 0; j < st; ++j) {
+                f64 c = method->c[packed + j];
+                if (c == 0.0) continue;
+                f32 hc = (f32)c * hinv;
+                for (u32 s = 0; s < n; ++s) KST(st, s) = fmaf(hc, KST(j, s), KST(st, s));
+            }
+            of_solve(m, lu, &KST(st, 0));
+        }
+
+        f32 err = zero;
+        for (u32 s = 0; s < n; ++s) {
+            f32 ynew = y[s], yerr = zero;
+            for (u32 st = 0; st < method->stages; ++st) {
+                if (method->m[st] != 0.0) ynew = fmaf((f32)method->m[st], KST(st, s), ynew);
+                if (method->e[st] != 0.0) yerr = fmaf((f32)method->e[st], KST(st, s), yerr);
+            }
+            f32 scale = fmaf(rtol, of_max(of_abs(base[s] + y[s]), of_abs(base[s] + ynew)), atol);
+            f32 q = yerr / scale;
+            err = fmaf(q, q, err);
+            ytmp[s] = ynew;
+        }
+        err = sqrtf(err * inv_n);
+        if (!of_is_finite(err)) err = (f32)1e10;
+
+        f32 fac = oracle_step_factor_f32(err, method->order);
+        int ok = (err <= one) || (hs <= h_min);
+        if (ctl->fixed_step) {
+            ok = 1;
+            fac = one;
+        }
+        int accept = ok, reject = !ok;
+
+        if (accept) {
+            for (u32 s = 0; s < n; ++s) {
+                f32 ynew = ytmp[s];
+                if (ctl->clamp_negative) ynew = of_max(ynew, -base[s]);
+                y[s] = ynew;
+            }
+            t = last ? t_end : t + hs;
+        }
+        f32 h_acc = hs * fac;
+        if (reject_last) h_acc = of_min(h_acc, hs);
+        if (last && fac >= one) h_acc = of_max(h_acc, h_wanted);
+        f32 h_rej = reject_more ? hs * (f32)0.1 : of_min(hs * fac, hs);
+        if (ctl->fixed_step) {
+            h_acc = h_wanted;
+            h_rej = h_wanted;
+        }
+        h = accept ? h_acc : h_rej;
+        reject_more = reject && reject_last;
+        reject_last = reject;
+        stats[0] = stats[0] + (accept ? one : zero);
+        stats[1] = stats[1] + (reject ? one : zero);
+        active = t < t_end;
+    }
+#undef KST
+    *t_io = t;
+    *h_io = h;
+}
+
+static void of_advance_lane(const OMech *m, const KinMethod *method, const KinControl *ctl, const OracleLaunch *L,
+                            u32 i, OWork32 *w, f32 *buf) {
+    const u32 n = m->n, p = L->padded;
+    f32 *base = buf, *y = base + n, *src = y + n, *pulse = src + n, *k = pulse + n;
+    for (u32 s = 0; s < n; ++s) {
+        base[s] = (f32)L->y[(size_t)s * p + i];
+        y[s] = 0.0f;
+        src[s] = (f32)L->src[(size_t)s * p + i];
+        pulse[s] = (f32)L->pulse[(size_t)s * p + i];
+    }
+    for (u32 r = 0; r < m->R; ++r) k[r] = (f32)L->k[(size_t)r * p + i];
+    f32 h = (f32)L->h[i];
+    const f32 period = (f32)L->period[i], first = (f32)L->pulse_first[i], count = (f32)L->pulse_count[i];
+    f32 stats[4] = {0.0f, 0.0f, 0.0f, (f32)L->failed[i]};
+    const f32 duration = (f32)L->duration;
+
+    f32 t = 0.0f, applied = 0.0f;
+    for (;;) {
+        int pending = applied < count;
+        f32 next_pulse = fmaf(applied, period, first);
+        f32 seg_end = pending ? of_min(next_pulse, duration) : duration;
+        of_integrate(m, method, ctl, k, src, base, y, &t, seg_end, &h, stats, w);
+        int alive = !(stats[3] > 0.0f);
+        int hit = pending && alive && (t >= seg_end);
+        if (hit) {
+            for (u32 s = 0; s < n; ++s) y[s] = y[s] + pulse[s];
+            applied = applied + 1.0f;
+            stats[2] = stats[2] + 1.0f;
+            h = (f32)ctl->h_start;
+        }
+        int done = ((t >= duration) && (applied >= count)) || !alive;
+        if (done) break;
+    }
+    for (u32 s = 0; s < n; ++s) {
+        f64 *dst = L->y + (size_t)s * p + i;
+        *dst += (f64)y[s];
+        if (ctl->clamp_negative) *dst = *dst > 0.0 ? *dst : 0.0;
+    }
+    L->h[i] = (f64)h;
+    L->failed[i] = (f64)stats[3];
+    L->steps[i] += (f64)stats[0];
+    L->rejects[i] += (f64)stats[1];
+    L->pulses[i] += (f64)stats[2];
+}
+
+typedef struct OPool32 {
+    const OMech *m;
+    const KinMethod *method;
+    const KinControl *ctl;
+    const OracleLaunch *L;
+    const u32 *lanes;
+    u32 count;
+    u32 next;
+} OPool32;
+
+static void *of_worker(void *arg) {
+    OPool32 *pool = (OPool32 *)arg;
+    const OMech *m = pool->m;
+    OWork32 w;
+    w.lu = (f32 *)malloc(sizeof(f32) * (m->nnz + 8 * (size_t)m->n));
+    w.kstage = w.lu + m->nnz;
+    w.fcn = w.kstage + 4 * (size_t)m->n;
+    w.ytmp = w.fcn + m->n;
+    w.yeval = w.ytmp + m->n;
+    f32 *buf = (f32 *)malloc(sizeof(f32) * (4 * (size_t)m->n + m->R + 1));
+    for (;;) {
+        u32 i = __atomic_fetch_add(&pool->next, 1u, __ATOMIC_RELAXED);
+        if (i >= pool->count) break;
+        of_advance_lane(m, pool->method, pool->ctl, pool->L, pool->lanes ? pool->lanes[i] : i, &w, buf);
+    }
+    free(buf);
+    free(w.lu);
+    return 0;
+}
+
+ORACLE_API s32 oracle_advance_delta32(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                      const KinControl *ctl, const OracleLaunch *L, const u32 *lanes, u32 lane_count,
+                                      u32 threads) {
+    OMech m = o_view(mech, caps);
+    OPool32 pool = {&m, method, ctl, L, lanes, lane_count, 0};
+    if (threads <= 1) {
+        of_worker(&pool);
+        return 0;
+    }
+    pthread_t *tid = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    for (u32 t = 0; t < threads; ++t) pthread_create(&tid[t], 0, of_worker, &pool);
+    for (u32 t = 0; t < threads; ++t) pthread_join(tid[t], 0);
+    free(tid);
+    return 0;
+}
+
 /* ---- operator-split upwind advection -------------------------------------
  * nft_transport's flux-form row update (src/sim/nft.cpp:1055-1081): per
  * substep, node j takes x_j + c_j (x_{j-1} - x_j) from the OLD neighbour

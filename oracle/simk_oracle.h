@@ -34,6 +34,11 @@ ORACLE_API void oracle_integrate(const void *mech, const SimkCaps *caps, const K
                                  const f64 *k, const f64 *src, f64 *y, f64 *t, f64 t_end, f64 *h, f64 *stats);
 ORACLE_API s32 oracle_advance(const void *mech, const SimkCaps *caps, const KinMethod *method, const KinControl *ctl,
                               const OracleLaunch *L, const u32 *lanes, u32 lane_count, u32 threads);
+ORACLE_API s32 oracle_advance_delta32(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                      const KinControl *ctl, const OracleLaunch *L, const u32 *lanes, u32 lane_count,
+                                      u32 threads);
+ORACLE_API f32 oracle_cbrt_clamped_f32(f32 x);
+ORACLE_API f32 oracle_step_factor_f32(f32 err, u32 order);
 ORACLE_API void oracle_upwind(f32 *x, const f32 *c, u32 nodes, u32 fields, u32 stride, u32 substeps, f32 *outlet_sum);
 ORACLE_API void oracle_upwind_f64(f64 *x, const f64 *c, u32 nodes, u32 fields, u32 stride, u32 substeps,
                                   const f64 *inflow);

@@ -47,8 +47,9 @@ def ref_robertson():
     return out
 
 
-def ref_zones(name, cells, durations, mid=2, pulses=False, ctl_kw=None, seed_pulses=11):
-    """A seeded workload advanced by the reference's dbd_advance_group<f64>."""
+def ref_zones(name, cells, durations, mid=2, pulses=False, ctl_kw=None, seed_pulses=11, delta32=False):
+    """A seeded workload advanced by the reference's dbd_advance_group<f64>, or
+    with delta32 by its f32 delta lanes dbd_advance_group<f32, ..., true>."""
     from helpers import add_pulses
     sp = problem.spec(name)
     mech = problem.Mechanism.synthetic(sp)
@@ -60,14 +61,15 @@ def ref_zones(name, cells, durations, mid=2, pulses=False, ctl_kw=None, seed_pul
     mb = oracle_ffi.ref_mech(mech)
     meth_b = oracle_ffi.ref_method_bytes(mid)
     for d in durations:
-        oracle_ffi.ref_advance(mb, meth_b, ctl, z, d, z.padded, width=1)
+        oracle_ffi.ref_advance(mb, meth_b, ctl, z, d, z.padded, width=32 if delta32 else 1)
     out = {"reactants": mech.reactants, "products": mech.products, "block_in": z_in.block,
            "block_out": z.block, "padded": np.array([z.padded]), "count": np.array([cells]),
            "n": np.array([mech.n]), "R": np.array([mech.R]), "durations": np.array(durations),
            "method": np.array([mid]),
            "ctl": np.array([ctl.rtol, ctl.atol, ctl.h_min, ctl.h_max, ctl.h_start, ctl.max_steps, ctl.fixed_step,
                             ctl.clamp_negative], np.float64),
-           "mech_bytes_ref_caps": mb, "ref_caps": np.array(oracle_ffi.ref_caps(), np.uint32)}
+           "mech_bytes_ref_caps": mb, "ref_caps": np.array(oracle_ffi.ref_caps(), np.uint32),
+           "delta32": np.array([1 if delta32 else 0])}
     return out
 
 
@@ -84,6 +86,15 @@ def main():
                         **ref_zones("c3c", 8, [0.02], ctl_kw=dict(fixed_step=1, max_steps=1000, h_start=1e-3)))
     np.savez_compressed(os.path.join(HERE, "c2_8cells_failure.npz"),
                         **ref_zones("c2", 8, [1.0], ctl_kw=dict(max_steps=40)))
+    # f32 delta lanes (the reference's default f32 precision)
+    np.savez_compressed(os.path.join(HERE, "c1_d32_24cells_pulses.npz"),
+                        **ref_zones("c1", 24, [0.05, 0.05], mid=2, pulses=True, delta32=True,
+                                    ctl_kw=dict(rtol=1e-3, atol=1e-10)))
+    for mid in (0, 1):
+        np.savez_compressed(os.path.join(HERE, f"c3c_d32_16cells_m{mid}.npz"),
+                            **ref_zones("c3c", 16, [0.1], mid=mid, delta32=True, ctl_kw=dict(rtol=1e-4, atol=1e-11)))
+    np.savez_compressed(os.path.join(HERE, "c2_d32_8cells_failure.npz"),
+                        **ref_zones("c2", 8, [1.0], delta32=True, ctl_kw=dict(rtol=1e-5, atol=1e-12, max_steps=60)))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

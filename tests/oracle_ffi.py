@@ -58,6 +58,10 @@ def oracle():
              f64p, f64p, f64p, f64p, C.c_double, f64p, f64p)
         _sig(lib, "oracle_advance", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl),
              P(OracleLaunch), V, C.c_uint32, C.c_uint32)
+        _sig(lib, "oracle_advance_delta32", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl),
+             V, V, C.c_uint32, C.c_uint32)
+        _sig(lib, "oracle_cbrt_clamped_f32", C.c_float, C.c_float)
+        _sig(lib, "oracle_step_factor_f32", C.c_float, C.c_float, C.c_uint32)
         _sig(lib, "oracle_upwind", None, V, V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, V)
         _sig(lib, "oracle_upwind_f64", None, V, V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, V)
         _oracle = lib
@@ -117,6 +121,18 @@ def oracle_advance(mech, meth, ctl, z, duration, lanes=None, threads=8):
         n = len(arr)
     oracle().oracle_advance(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(ctl), C.byref(L),
                             None if arr is None else arr.ctypes.data, n, threads)
+
+
+def oracle_advance_delta32(mech, meth, ctl, z, duration, lanes=None, threads=8):
+    """The f32 delta-lane advance (GpuKinLaunch precision 1) of the oracle, in place."""
+    L = _launch(OracleLaunch, z, duration, {})
+    if lanes is None:
+        arr, n = None, z.count
+    else:
+        arr = np.ascontiguousarray(lanes, dtype=np.uint32)
+        n = len(arr)
+    oracle().oracle_advance_delta32(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(ctl),
+                                    C.byref(L), None if arr is None else arr.ctypes.data, n, threads)
 
 
 def ref_advance(mech_bytes, meth_bytes, ctl, z, duration, lanes, width=1, threads=8):

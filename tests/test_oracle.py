@@ -248,7 +248,8 @@ def test_methods_bitwise():
 
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "c*.npz"))), ids=os.path.basename)
 def test_oracle_reproduces_reference_fixture(path):
-    """The oracle advances each fixture's inputs to the reference's output bits."""
+    """The oracle advances each fixture's inputs to the reference's output bits
+    (FP64 lanes, or the f32 delta lanes for the *_d32_* fixtures)."""
     g = np.load(path)
     n, R, count = int(g["n"][0]), int(g["R"][0]), int(g["count"][0])
     mech = problem.Mechanism(n, R, g["reactants"], g["products"])
@@ -258,8 +259,10 @@ def test_oracle_reproduces_reference_fixture(path):
     z = problem.Zones(n, R, count, padded=int(g["padded"][0]))
     z.block[:] = g["block_in"]
     meth = problem.method(int(g["method"][0]))
+    delta32 = "delta32" in g.files and int(g["delta32"][0]) == 1
+    advance = oracle_ffi.oracle_advance_delta32 if delta32 else oracle_ffi.oracle_advance
     for d in g["durations"]:
-        oracle_ffi.oracle_advance(mech, meth, ctl, z, float(d), threads=4)
+        advance(mech, meth, ctl, z, float(d), threads=4)
     want = problem.Zones(n, R, count, padded=int(g["padded"][0]))
     want.block[:] = g["block_out"]
     from helpers import compare_bits
@@ -341,3 +344,71 @@ def test_oracle_vs_reference_live(name, cells, width):
     from helpers import compare_bits
     assert compare_bits(a, b) == {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
     assert mb.tobytes() == oracle_ffi.ref_mech(mech).tobytes()
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not oracle_ffi.have_ref(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("name,cells,width,rtol", [("c1", 48, 32, 1e-3), ("c1", 64, 8, 1e-3), ("c3c", 64, 32, 1e-4),
+                                                   ("c2", 16, 8, 1e-3)])
+def test_delta32_vs_reference_live(name, cells, width, rtol):
+    """f32 delta lanes: oracle == the reference's dbd_advance_group<f32, .., Delta>
+    (scalar f32 and AVX2 f32x8) with pulses, bit for bit."""
+    from helpers import add_pulses, compare_bits
+    sp = problem.spec(name)
+    mech = problem.Mechanism.synthetic(sp)
+    z = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params, first_cell=1000)
+    add_pulses(z, seed=5, duration=0.2)
+    meth, ctl = problem.method(2), problem.control(rtol=rtol, atol=1e-9)
+    a, b = z.copy(), z.copy()
+    oracle_ffi.oracle_advance_delta32(mech, meth, ctl, a, 0.2, threads=8)
+    mb = mech.rebuild(oracle_ffi.ref_caps()).bytes
+    oracle_ffi.ref_advance(mb, oracle_ffi.ref_method_bytes(2), ctl, b, 0.2, z.padded, width=width)
+    assert compare_bits(a, b) == {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+
+
+def test_cbrt_f32_closed_form():
+    """The f32 x^(1/3) (all constants rounded to f32) against the exact cube root."""
+    xs = np.geomspace(1e-4, 1e4, 301).astype(np.float32)
+    got = np.array([oracle().oracle_cbrt_clamped_f32(float(x)) for x in xs])
+    assert np.max(np.abs(got / np.cbrt(xs.astype(np.float64)) - 1.0)) < 1e-5   # two Newton steps: ~1e-6 + f32 rounding
+    assert oracle().oracle_cbrt_clamped_f32(1e-9) == oracle().oracle_cbrt_clamped_f32(1e-4)
+    for order, root in ((2, np.sqrt), (4, lambda e: np.sqrt(np.sqrt(e)))):
+        for e in (1e-3, 0.5, 2.0, 900.0):
+            want = min(max(0.9 / root(e), 0.2), 6.0)
+            assert abs(oracle().oracle_step_factor_f32(e, order) / want - 1.0) < 1e-6
+
+
+def test_delta32_tracks_fp64_within_its_tolerance():
+    """Delta lanes in f32 at rtol 1e-4, on the cells that converge (f32 fails
+    some stiff cells of this mechanism, as the reference's f32 lanes do), stay
+    within 2e-3 of the FP64 path on the species above 1e-4 mol/L."""
+    sp = problem.spec("c1")
+    mech = problem.Mechanism.synthetic(sp)
+    z = problem.Zones(mech.n, mech.R, 32).fill_synthetic(sp, mech.params)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-4, atol=1e-12)
+    a, b = z.copy(), z.copy()
+    oracle_ffi.oracle_advance_delta32(mech, meth, ctl, a, 0.1, threads=4)
+    oracle_ffi.oracle_advance(mech, meth, ctl, b, 0.1, threads=4)
+    ok = a.failed[:32] == 0
+    assert ok.sum() >= 16 and b.failed[:32].sum() == 0
+    ya, yb = a.y[:, :32][:, ok], b.y[:, :32][:, ok]
+    mask = yb > 1e-4
+    assert np.max(np.abs(ya[mask] - yb[mask]) / yb[mask]) < 2e-3
+
+
+def test_delta32_keeps_increments_below_f32_resolution():
+    """Robertson (A -> B, 2B -> C + B, B + C -> A + C) over 1 us from A = 1: A
+    loses 4e-8, below half an f32 ulp of 1. The delta form integrates the
+    increment and adds it to the f64 storage, so A moves by -0.04 * 1e-6 to
+    within the f32 precision of the increment."""
+    reac = np.array([[0, 255, 255], [1, 1, 255], [1, 2, 255]], np.uint8)
+    prod = np.array([[1, 255, 255], [2, 1, 255], [0, 2, 255]], np.uint8)
+    mech = problem.Mechanism(3, 3, reac, prod)
+    z = problem.Zones(3, 3, 1)
+    z.k[:, 0] = [0.04, 3.0e7, 1.0e4]
+    z.y[:, 0] = [1.0, 0.0, 0.0]
+    meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-14, h_start=1e-7)
+    oracle_ffi.oracle_advance_delta32(mech, meth, ctl, z, 1e-6, lanes=[0], threads=1)
+    dA = z.y[0, 0] - 1.0
+    assert dA != 0.0 and abs(dA / -4e-8 - 1.0) < 1e-3
+    assert abs(z.y[1, 0] / 4e-8 - 1.0) < 1e-3

@@ -108,7 +108,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
 /* Statistics of the current mechanism and the last advance. */
 typedef struct SimkKinInfo {
-    u32 kernel;          /* 1 generated, 2 table */
+    u32 kernel;          /* last launch: 1 generated FP64, 2 table FP64, 3 mixed precision, 4 multi-mechanism, 5 f32 delta */
     u32 ws_slots;        /* workspace f64 slots per lane */
     u32 waves;           /* persistent waves of the last launch */
     u32 pad_;

@@ -20,6 +20,7 @@
 #include "simk_gpu_api.h"
 #include "module_internal.h"
 
+#include "device_delta32.h"
 #include "device_kin_src.inc"   // kDeviceKinSource: device_kin.h as text, for hipRTC
 
 using namespace simk;
@@ -148,6 +149,11 @@ extern "C" __global__ void __launch_bounds__(SIMK_WAVE) simk_table_kernel(SimkKi
     simk_persistent_body<TableOps, ValueMethod>(A, me);
 }
 
+// f32 delta lanes (GpuKinLaunch precision 1), table walk, any mechanism.
+extern "C" __global__ void __launch_bounds__(SIMK_WAVE) simk_delta32_kernel(SimkKinArgs A, SimkD32Consts C) {
+    simk_d32_body(A, C);
+}
+
 // The reference's M1 point demo (a spring/swirl particle step), kept so every
 // reference export exists.
 __global__ void simk_step_points_kernel(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy, f32 *vz, u32 count, f32 dt) {
@@ -247,7 +253,7 @@ struct State {
     u32 *queue = nullptr;
     u32 cus = 0;
     u32 grid_override = 0;     // persistent waves per launch (0: resident waves)
-    int blocks_per_cu_gen = 0, blocks_per_cu_table = 0;
+    int blocks_per_cu_table = 0, blocks_per_cu_d32 = 0;
     const u32 *order = nullptr;
     // last call
     u32 last_kernel = 0;
@@ -566,6 +572,90 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     return 0;
 }
 
+// Method and control rounded to f32 as lane_splat rounds them (base_lane.h:98).
+SimkD32Consts d32_consts() {
+    SimkD32Consts C;
+    memset(&C, 0, sizeof(C));
+    const KinMethod &me = g.method;
+    for (int i = 0; i < 6; ++i) {
+        C.a[i] = (f32)me.a[i];
+        C.c[i] = (f32)me.c[i];
+        if (me.a[i] != 0.0) C.a_nz |= 1u << i;
+        if (me.c[i] != 0.0) C.c_nz |= 1u << i;
+    }
+    for (int i = 0; i < 4; ++i) {
+        C.m[i] = (f32)me.m[i];
+        C.e[i] = (f32)me.e[i];
+        if (me.m[i] != 0.0) C.m_nz |= 1u << i;
+        if (me.e[i] != 0.0) C.e_nz |= 1u << i;
+        if (me.new_f[i]) C.new_f |= 1u << i;
+    }
+    C.gamma0 = (f32)me.gamma0;
+    C.stages = me.stages;
+    C.order = me.order;
+    C.rtol = (f32)g.ctl.rtol;
+    C.atol = (f32)g.ctl.atol;
+    C.h_min = (f32)g.ctl.h_min;
+    C.h_max = (f32)g.ctl.h_max;
+    C.h_start = (f32)g.ctl.h_start;
+    C.inv_n = (f32)g.ctl.inv_n;   // (f32)(1.0 / (f64)n), kin_solver.h:197
+    C.h_max_pos = g.ctl.h_max > 0.0 ? 1u : 0u;
+    C.max_steps = g.ctl.max_steps;
+    C.fixed_step = g.ctl.fixed_step;
+    C.clamp_negative = g.ctl.clamp_negative;
+    return C;
+}
+
+hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
+    const Mech &m = g.mech;
+    const u32 slots = m.nnz + 11 * m.n + m.R;   // f32 slots (simk_d32_slots)
+    const u64 rows = m.R > m.n ? m.R : m.n;
+    if (rows * L->padded * 8ull >= (1ull << 31) || (u64)slots * SIMK_WAVE * 4ull >= (1ull << 31))
+        return hipErrorInvalidValue;
+    if (!g.blocks_per_cu_d32) {
+        int blocks = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_delta32_kernel, SIMK_WAVE, 0) != hipSuccess)
+            blocks = 1;
+        g.blocks_per_cu_d32 = blocks > 0 ? blocks : 1;
+    }
+    const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
+    u32 resident = (u32)g.blocks_per_cu_d32 * (g.cus ? g.cus : 1);
+    if (g.grid_override) resident = g.grid_override;
+    const u32 waves = needed < resident ? needed : resident;
+    hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(float));
+    if (e != hipSuccess) return e;
+    if (!g.queue) {
+        e = hipMalloc(&g.queue, 256);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, g.queue, waves * SIMK_WAVE);
+    SimkKinArgs A;
+    memset(&A, 0, sizeof(A));
+    A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
+    A.k = L->k; A.src = L->src; A.pulse = L->pulse; A.period = L->period; A.pulse_first = L->pulse_first;
+    A.pulse_count = L->pulse_count;
+    A.padded = L->padded;
+    A.lanes = lanes;
+    A.duration = L->duration;
+    A.ws = g.ws;
+    A.ws_slots = slots;
+    A.n_species = m.n;
+    A.n_reactions = m.R;
+    A.repeats = repeats;
+    A.queue = g.queue;
+    A.order = g.order;
+    A.tables = (const SimkDevTables *)g.d_tables;
+    A.ctl = g.ctl;
+    A.method = g.dev_method;
+    const SimkD32Consts C = d32_consts();
+    g.last_kernel = 5;
+    g.last_name = "simk_delta32_kernel";
+    g.last_slots = slots;
+    g.last_waves = waves;
+    hipLaunchKernelGGL(simk_delta32_kernel, dim3(waves), dim3(SIMK_WAVE), 0, g.stream, A, C);
+    return hipGetLastError();
+}
+
 // Launches `repeats` consecutive advances of queue positions [0, lanes) on the
 // module stream with the persistent kernel: grid = resident waves (or fewer
 // when there are fewer cells), lanes refill from the queue.
@@ -576,8 +666,9 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     if (!L->y || !L->h || !L->steps || !L->rejects || !L->pulses || !L->failed || !L->k || !L->src || !L->pulse ||
         !L->period || !L->pulse_first || !L->pulse_count)
         return hipErrorInvalidValue;
-    if (L->precision != 0 && L->precision != 2) return hipErrorNotSupported;
+    if (L->precision > 2) return hipErrorNotSupported;
     if (g.mode != 0) return hipErrorNotSupported;
+    if (L->precision == 1) return launch_delta32(L, lanes, repeats);
     const JitKernel *jk = nullptr;
     if (L->precision == 2) {
         // Mixed precision exists only as a generated kernel.

@@ -1,0 +1,96 @@
+"""f32 delta lanes on the device (GpuKinLaunch precision 1) vs the oracle's
+restatement of the reference's dbd_advance_group<f32, .., Delta = true>
+(src/sim/dbd_lanes.h:28-78, src/sim/kinetics/kin_solver.h:189-347): bit for
+bit on y (f64 storage after the increment is added and clamped), h, the
+counters and the failure flag. This is the reference's default chemistry
+precision (--precision f32)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_ffi
+from simstack_amd import _ffi, gpu, problem
+from helpers import add_pulses, compare_bits, make_problem
+
+pytestmark = pytest.mark.gpu
+
+ZERO = {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+
+
+def _gpu(gm, mech, meth, ctl, z, durations, repeats=1, default_abi=False):
+    gm.arena_reset()
+    if default_abi:
+        gm._check(gm.lib.gpu_kin_set_mech(mech.bytes.ctypes.data, C.byref(meth), C.byref(ctl)), "set_mech")
+    else:
+        gm.set_mech(mech, meth, ctl)
+    dz = gpu.DeviceZones(gm, z)
+    dz.upload()
+    for d in durations:
+        dz.advance(d, precision=1, repeats=repeats)
+    dz.download(mutable_only=False)
+    return gm.kin_info()
+
+
+def _oracle(mech, meth, ctl, z, durations, lanes=None):
+    for d in durations:
+        oracle_ffi.oracle_advance_delta32(mech, meth, ctl, z, d, lanes=lanes, threads=16)
+
+
+@pytest.mark.parametrize("mid", [0, 1, 2], ids=["ros2", "ros3", "rodas3"])
+def test_delta32_bit_exact_with_pulses(gm, mid):
+    """Every method, pulses on half the cells, two calls carrying h. max_steps
+    bounds the cells f32 cannot converge (the reference's f32 lanes fail them
+    too); the second-order method runs at rtol 1e-2."""
+    sp, mech, z = make_problem("c1", 256, caps=_ffi.DEFAULT_CAPS)
+    add_pulses(z, seed=mid + 3, duration=0.01)
+    meth = problem.method(mid)
+    ctl = problem.control(rtol=1e-2 if mid == 0 else 1e-3, atol=1e-10, max_steps=1000)
+    a, b = z.copy(), z.copy()
+    info = _gpu(gm, mech, meth, ctl, a, [0.01, 0.01], default_abi=True)
+    _oracle(mech, meth, ctl, b, [0.01, 0.01])
+    assert info.kernel == 5
+    assert compare_bits(a, b) == ZERO
+    assert b.pulses[:256].sum() > 0 and b.steps[:256].min() > 2
+
+
+def test_delta32_c2_sampled(gm):
+    """48/150 with pH spread: 1,024 cells, 128 checked by the oracle."""
+    sp, mech, z = make_problem("c2", 1024)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
+    a = z.copy()
+    _gpu(gm, mech, meth, ctl, a, [0.02])
+    lanes = np.random.default_rng(4).choice(1024, 128, replace=False)
+    b = z.subset(lanes)
+    _oracle(mech, meth, ctl, b, [0.02])
+    assert compare_bits(a.subset(lanes), b) == ZERO
+
+
+def test_delta32_fused_repeats_equal_separate_calls(gm):
+    """simk_kin_advance_n(3) in delta lanes == three calls: each call restarts
+    from the f64 storage (base re-rounded, counters added)."""
+    sp, mech, z = make_problem("c3c", 500)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
+    a, b, c = z.copy(), z.copy(), z.copy()
+    _gpu(gm, mech, meth, ctl, a, [0.005], repeats=3)
+    _gpu(gm, mech, meth, ctl, b, [0.005, 0.005, 0.005])
+    _oracle(mech, meth, ctl, c, [0.005, 0.005, 0.005])
+    assert compare_bits(a, b) == ZERO
+    assert compare_bits(a, c) == ZERO
+
+
+def test_delta32_refill_with_few_waves(gm):
+    """Three resident waves for 400 cells of the 40/130 mechanism (lanes refill
+    many times). max_steps bounds the cells f32 cannot converge: about 37,000
+    attempts in all, at most 1,000 for one cell."""
+    sp, mech, z = make_problem("c3b", 400)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
+    a, b = z.copy(), z.copy()
+    try:
+        gm.set_grid(3)
+        _gpu(gm, mech, meth, ctl, a, [0.01])
+    finally:
+        gm.set_grid(0)
+    _oracle(mech, meth, ctl, b, [0.01])
+    assert compare_bits(a, b) == ZERO
+    assert b.failed[:400].sum() > 0

@@ -20,15 +20,18 @@
 
 #include "device_kin.h"
 
-// Method and control rounded to f32 on the host, plus the f64 tests the
-// reference makes on the unrounded values (a == 0, c == 0, h_max > 0, ...).
-struct SimkD32Consts {
-    float a[6], c[6], m[4], e[4], gamma0;
+// Method and control in the lane type T (lane_splat's rounding for f32),
+// plus the f64 tests the reference makes on the unrounded values (a == 0,
+// c == 0, h_max > 0, ...).
+template <class T>
+struct SimkTConsts {
+    T a[6], c[6], m[4], e[4], gamma0;
     simk_u32 a_nz, c_nz, m_nz, e_nz;   // bit i: coefficient i is non-zero
     simk_u32 stages, order, new_f;     // new_f: bit st
-    float rtol, atol, h_min, h_max, h_start, inv_n;
+    T rtol, atol, h_min, h_max, h_start, inv_n;
     simk_u32 h_max_pos, max_steps, fixed_step, clamp_negative;
 };
+typedef SimkTConsts<float> SimkD32Consts;
 
 SIMK_DEV float simk_fminf_(float a, float b) { return a < b ? a : b; }
 SIMK_DEV float simk_fmaxf_(float a, float b) { return a > b ? a : b; }

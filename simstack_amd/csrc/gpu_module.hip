@@ -21,6 +21,7 @@
 #include "module_internal.h"
 
 #include "device_delta32.h"
+#include "device_wave.h"
 #include "device_kin_src.inc"   // kDeviceKinSource: device_kin.h as text, for hipRTC
 
 using namespace simk;
@@ -154,6 +155,16 @@ extern "C" __global__ void __launch_bounds__(SIMK_WAVE) simk_delta32_kernel(Simk
     simk_d32_body(A, C);
 }
 
+// Cooperative mode: one wave per cell, cell state in LDS.
+template <class T, bool DELTA>
+__global__ void __launch_bounds__(SIMK_WAVE) simk_wave_kernel(SimkKinArgs A, SimkTConsts<T> C,
+                                                             const SimkWaveTables *W) {
+    extern __shared__ __align__(16) unsigned char simk_wave_lds[];
+    const simk_u32 elems = simk_wave_lds_elems(A.tables->n, A.tables->R, A.tables->nnz);
+    double *y64 = (double *)(simk_wave_lds + ((elems * (simk_u32)sizeof(T) + 7u) & ~7u));
+    simk_wave_body<T, DELTA>(A, C, W, (T *)simk_wave_lds, y64);
+}
+
 // The reference's M1 point demo (a spring/swirl particle step), kept so every
 // reference export exists.
 __global__ void simk_step_points_kernel(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy, f32 *vz, u32 count, f32 dt) {
@@ -226,6 +237,7 @@ struct State {
     SimkDevCtl ctl{};
     SimkDevMethod dev_method{};
     void *d_tables = nullptr;   // SimkDevTables followed by its arrays
+    void *d_wave = nullptr;     // SimkWaveTables followed by its lists (cooperative mode)
     JitKernel jit;
     bool jit_ok = false;
     JitKernel jit_mixed;       // compiled on the first mixed-precision launch
@@ -365,6 +377,8 @@ bool jit_code(const GenResult &gr, std::vector<char> &code, bool &cached, std::s
 void free_mech_resources() {
     if (g.d_tables) hipFree(g.d_tables);
     g.d_tables = nullptr;
+    if (g.d_wave) hipFree(g.d_wave);
+    g.d_wave = nullptr;
     if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
     g.jit = JitKernel();
     g.jit_ok = false;
@@ -467,6 +481,103 @@ hipError_t upload_tables(const Mech &m) {
     return hipSuccess;
 }
 
+// Contribution lists of the cooperative kernel, in the reference's
+// accumulation order (fun: reaction order per species; -J: (r, p, e) order per
+// LU entry; solves: by column).
+hipError_t upload_wave_tables(const Mech &m) {
+    std::vector<u16> fb(m.n + 1), fr, jb(m.nnz + 1), jrp, lb(m.n + 1), lr, li, ub(m.n + 1), ur, ui;
+    std::vector<f64> fc, jc;
+    for (u32 sp = 0; sp < m.n; ++sp) {
+        fb[sp] = (u16)fr.size();
+        for (u32 r = 0; r < m.R; ++r)
+            for (u32 e = 0; e < m.rx[r].net_count; ++e)
+                if (m.rx[r].net_species[e] == sp) {
+                    fr.push_back((u16)r);
+                    fc.push_back((f64)m.rx[r].net_coef[e]);
+                }
+    }
+    fb[m.n] = (u16)fr.size();
+    std::vector<std::vector<std::pair<u16, f64>>> per(m.nnz);
+    for (u32 r = 0; r < m.R; ++r)
+        for (u32 p = 0; p < m.rx[r].reactant_count; ++p)
+            for (u32 e = 0; e < m.rx[r].net_count; ++e)
+                per[m.rx[r].jac_index[p][e]].push_back({(u16)(r * 3 + p), -(f64)m.rx[r].net_coef[e]});
+    for (u32 idx = 0; idx < m.nnz; ++idx) {
+        jb[idx] = (u16)jrp.size();
+        for (const auto &c : per[idx]) {
+            jrp.push_back(c.first);
+            jc.push_back(c.second);
+        }
+    }
+    jb[m.nnz] = (u16)jrp.size();
+    std::vector<std::vector<std::pair<u16, u16>>> lcol(m.n), ucol(m.n);
+    for (u32 i = 0; i < m.n; ++i) {
+        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e)
+            lcol[m.lower[e].col].push_back({(u16)i, m.lower[e].index});
+        for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e)
+            ucol[m.upper[e].col].push_back({(u16)i, m.upper[e].index});
+    }
+    for (u32 c = 0; c < m.n; ++c) {
+        lb[c] = (u16)lr.size();
+        for (const auto &x : lcol[c]) {
+            lr.push_back(x.first);
+            li.push_back(x.second);
+        }
+        ub[c] = (u16)ur.size();
+        for (const auto &x : ucol[c]) {
+            ur.push_back(x.first);
+            ui.push_back(x.second);
+        }
+    }
+    lb[m.n] = (u16)lr.size();
+    ub[m.n] = (u16)ur.size();
+    if (fr.size() > 0xFFFF || jrp.size() > 0xFFFF) return hipErrorInvalidValue;
+    std::vector<u8> blob(sizeof(SimkWaveTables), 0);
+    u64 off = sizeof(SimkWaveTables);
+    u64 o[12];
+    put(blob, off, fb.data(), fb.size()); o[0] = off - 2 * fb.size();
+    put(blob, off, fr.data(), fr.size()); o[1] = off - 2 * fr.size();
+    put(blob, off, fc.data(), fc.size()); o[2] = off - 8 * fc.size();
+    put(blob, off, jb.data(), jb.size()); o[3] = off - 2 * jb.size();
+    put(blob, off, jrp.data(), jrp.size()); o[4] = off - 2 * jrp.size();
+    put(blob, off, jc.data(), jc.size()); o[5] = off - 8 * jc.size();
+    put(blob, off, lb.data(), lb.size()); o[6] = off - 2 * lb.size();
+    put(blob, off, lr.data(), lr.size()); o[7] = off - 2 * lr.size();
+    put(blob, off, li.data(), li.size()); o[8] = off - 2 * li.size();
+    put(blob, off, ub.data(), ub.size()); o[9] = off - 2 * ub.size();
+    put(blob, off, ur.data(), ur.size()); o[10] = off - 2 * ur.size();
+    put(blob, off, ui.data(), ui.size()); o[11] = off - 2 * ui.size();
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, blob.size());
+    if (e != hipSuccess) return e;
+    const u8 *base = (const u8 *)d;
+    SimkWaveTables t;
+    memset(&t, 0, sizeof(t));
+    t.n = m.n;
+    t.R = m.R;
+    t.nnz = m.nnz;
+    t.fun_begin = (const unsigned short *)(base + o[0]);
+    t.fun_r = (const unsigned short *)(base + o[1]);
+    t.fun_coef = (const double *)(base + o[2]);
+    t.jac_begin = (const unsigned short *)(base + o[3]);
+    t.jac_rp = (const unsigned short *)(base + o[4]);
+    t.jac_coef = (const double *)(base + o[5]);
+    t.lcol_begin = (const unsigned short *)(base + o[6]);
+    t.lcol_row = (const unsigned short *)(base + o[7]);
+    t.lcol_idx = (const unsigned short *)(base + o[8]);
+    t.ucol_begin = (const unsigned short *)(base + o[9]);
+    t.ucol_row = (const unsigned short *)(base + o[10]);
+    t.ucol_idx = (const unsigned short *)(base + o[11]);
+    memcpy(blob.data(), &t, sizeof(t));
+    e = hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return e;
+    }
+    g.d_wave = d;
+    return hipSuccess;
+}
+
 void fill_dev_method(const KinMethod &me, SimkDevMethod &dm) {
     memset(&dm, 0, sizeof(dm));
     dm.stages = me.stages;
@@ -557,6 +668,7 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     g.ctl.fixed_step = control->fixed_step ? 1 : 0;
     g.ctl.clamp_negative = control->clamp_negative ? 1 : 0;
     e = upload_tables(m);
+    if (e == hipSuccess) e = upload_wave_tables(m);
     if (e != hipSuccess) return err_code(e);
     if (!g.blocks_per_cu_table) {
         int blocks = 0;
@@ -572,38 +684,98 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     return 0;
 }
 
-// Method and control rounded to f32 as lane_splat rounds them (base_lane.h:98).
-SimkD32Consts d32_consts() {
-    SimkD32Consts C;
+// Method and control in the lane type T (lane_splat's rounding for f32, base_lane.h:98).
+template <class T>
+SimkTConsts<T> t_consts() {
+    SimkTConsts<T> C;
     memset(&C, 0, sizeof(C));
     const KinMethod &me = g.method;
     for (int i = 0; i < 6; ++i) {
-        C.a[i] = (f32)me.a[i];
-        C.c[i] = (f32)me.c[i];
+        C.a[i] = (T)me.a[i];
+        C.c[i] = (T)me.c[i];
         if (me.a[i] != 0.0) C.a_nz |= 1u << i;
         if (me.c[i] != 0.0) C.c_nz |= 1u << i;
     }
     for (int i = 0; i < 4; ++i) {
-        C.m[i] = (f32)me.m[i];
-        C.e[i] = (f32)me.e[i];
+        C.m[i] = (T)me.m[i];
+        C.e[i] = (T)me.e[i];
         if (me.m[i] != 0.0) C.m_nz |= 1u << i;
         if (me.e[i] != 0.0) C.e_nz |= 1u << i;
         if (me.new_f[i]) C.new_f |= 1u << i;
     }
-    C.gamma0 = (f32)me.gamma0;
+    C.gamma0 = (T)me.gamma0;
     C.stages = me.stages;
     C.order = me.order;
-    C.rtol = (f32)g.ctl.rtol;
-    C.atol = (f32)g.ctl.atol;
-    C.h_min = (f32)g.ctl.h_min;
-    C.h_max = (f32)g.ctl.h_max;
-    C.h_start = (f32)g.ctl.h_start;
-    C.inv_n = (f32)g.ctl.inv_n;   // (f32)(1.0 / (f64)n), kin_solver.h:197
+    C.rtol = (T)g.ctl.rtol;
+    C.atol = (T)g.ctl.atol;
+    C.h_min = (T)g.ctl.h_min;
+    C.h_max = (T)g.ctl.h_max;
+    C.h_start = (T)g.ctl.h_start;
+    C.inv_n = (T)g.ctl.inv_n;   // (T)(1.0 / (f64)n), kin_solver.h:197
     C.h_max_pos = g.ctl.h_max > 0.0 ? 1u : 0u;
     C.max_steps = g.ctl.max_steps;
     C.fixed_step = g.ctl.fixed_step;
     C.clamp_negative = g.ctl.clamp_negative;
     return C;
+}
+
+// Cooperative mode: one wave per cell (gpu_kin_set_mode(1)); FP64 or f32 delta.
+template <class T, bool DELTA>
+hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
+    const Mech &m = g.mech;
+    const u64 elems = 13ull * m.n + 5ull * m.R + m.nnz + 2;
+    const u64 lds = ((elems * sizeof(T) + 7) & ~7ull) + 8ull * m.n;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, g.device);
+    if (e != hipSuccess) return e;
+    if (lds > prop.sharedMemPerBlock) return hipErrorInvalidValue;   // the cell does not fit one workgroup's LDS
+    const auto kernel = simk_wave_kernel<T, DELTA>;
+    if (lds > 65536) {
+        e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    int blocks = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, SIMK_WAVE, lds) != hipSuccess || blocks < 1)
+        blocks = 1;
+    u32 resident = (u32)blocks * (g.cus ? g.cus : 1);
+    if (g.grid_override) resident = g.grid_override;
+    const u32 waves = lanes < resident ? lanes : resident;
+    if (!g.queue) {
+        e = hipMalloc(&g.queue, 256);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, g.queue, waves);
+    SimkKinArgs A;
+    memset(&A, 0, sizeof(A));
+    A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
+    A.k = L->k; A.src = L->src; A.pulse = L->pulse; A.period = L->period; A.pulse_first = L->pulse_first;
+    A.pulse_count = L->pulse_count;
+    A.padded = L->padded;
+    A.lanes = lanes;
+    A.duration = L->duration;
+    A.n_species = m.n;
+    A.n_reactions = m.R;
+    A.repeats = repeats;
+    A.queue = g.queue;
+    A.order = g.order;
+    A.tables = (const SimkDevTables *)g.d_tables;
+    A.ctl = g.ctl;
+    A.method = g.dev_method;
+    const SimkTConsts<T> C = t_consts<T>();
+    g.last_kernel = DELTA ? 7 : 6;
+    g.last_name = DELTA ? "simk_wave_kernel<f32,delta>" : "simk_wave_kernel<f64>";
+    g.last_slots = 0;
+    g.last_waves = waves;
+    hipLaunchKernelGGL(kernel, dim3(waves), dim3(SIMK_WAVE), (u32)lds, g.stream, A, C,
+                       (const SimkWaveTables *)g.d_wave);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
+    if (!g.d_wave) return hipErrorInvalidValue;
+    if (L->precision == 0) return launch_wave_t<f64, false>(L, lanes, repeats);
+    if (L->precision == 1) return launch_wave_t<f32, true>(L, lanes, repeats);
+    return hipErrorNotSupported;   // mixed precision has no cooperative form
 }
 
 hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
@@ -647,7 +819,7 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
     A.method = g.dev_method;
-    const SimkD32Consts C = d32_consts();
+    const SimkD32Consts C = t_consts<f32>();
     g.last_kernel = 5;
     g.last_name = "simk_delta32_kernel";
     g.last_slots = slots;
@@ -667,7 +839,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
         !L->period || !L->pulse_first || !L->pulse_count)
         return hipErrorInvalidValue;
     if (L->precision > 2) return hipErrorNotSupported;
-    if (g.mode != 0) return hipErrorNotSupported;
+    if (g.mode == 1) return launch_wave(L, lanes, repeats);
     if (L->precision == 1) return launch_delta32(L, lanes, repeats);
     const JitKernel *jk = nullptr;
     if (L->precision == 2) {

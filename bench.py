@@ -55,7 +55,7 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration):
 
 
 def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
-                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0):
+                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0):
     """Advances `cells` cells of the workload `steps` x 1 s from the seeded initial
     state (after `warmup` untimed seconds on a copy). Returns timings and the block."""
     z0 = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
@@ -67,11 +67,17 @@ def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precis
     if need > gm.info.arena_bytes:
         gm = run_workload.gm = gpu.GpuModule(arena_bytes=need)
     gm.set_kernel(kernel)
+    gm.lib.gpu_kin_set_mode(0)
     gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps)
     gm.set_grid(grid)
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z0.copy())
     dz.upload()
+    tuned = None
+    if tune > 0.0 and precision == 0:
+        t0 = time.perf_counter()
+        tuned = dz.autotune(duration=tune).as_dict()
+        tuned["tuning_wall_s"] = time.perf_counter() - t0
     if warmup:
         dz.advance(1.0, repeats=warmup, precision=precision)
     dz.z.block[:] = z0.block
@@ -91,8 +97,10 @@ def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precis
     acc = float((z.steps[:cells] - z0.steps[:cells]).sum())
     att = acc + float((z.rejects[:cells] - z0.rejects[:cells]).sum())
     info = gm.kin_info()
+    gm.lib.gpu_kin_set_mode(0)
     return {"wall": wall, "kernel_ms": kernel_ms, "accepted": acc, "attempts": att, "zones": z,
-            "failed": int(z.failed[:cells].sum()), "kernel": info.kernel_name.decode(), "waves": info.waves}
+            "failed": int(z.failed[:cells].sum()), "kernel": info.kernel_name.decode(), "waves": info.waves,
+            "tuned": tuned}
 
 
 run_workload.gm = None
@@ -232,6 +240,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64 (bit-exact), 2 mixed precision")
     ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1; ~0u: none)")
+    ap.add_argument("--tune", type=float, default=0.05,
+                    help="autotune on the workload for this many simulated seconds before timing (0: defaults)")
     ap.add_argument("--no-ragged", action="store_true", help="skip the three-mechanism ragged batch")
     ap.add_argument("--ragged-steps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
@@ -251,7 +261,7 @@ def main():
     knobs = dict(kernel=args.kernel, grid=args.grid, reg_window=args.reg_window, waves_per_eu=args.waves_per_eu,
                  batch=args.batch)
     r = run_workload(gpu, problem, mech, sp, meth, ctl, args.cells, args.steps, args.warmup, args.precision,
-                     args.sweeps, args.separate, **knobs)
+                     args.sweeps, args.separate, tune=args.tune if not args.grid else 0.0, **knobs)
     kernel_ms = r["kernel_ms"]
     wall, acc, att = r["wall"], r["accepted"], r["attempts"]
     avg_ms = float(np.mean(kernel_ms))
@@ -284,7 +294,7 @@ def main():
         "config": {"workload": "48 species / 150 reactions, 65,536 cells, 10 x 1 s operator-split, Rodas3, "
                                "rtol 1e-6 atol 1e-12", "cells": args.cells, "species": mech.n,
                    "reactions": mech.R, "nnz": mech.nnz, "lu_div": mech.lu_div, "lu_sub": mech.lu_sub,
-                   "kernel": r["kernel"], "waves": r["waves"], "launches": len(kernel_ms),
+                   "kernel": r["kernel"], "waves": r["waves"], "launches": len(kernel_ms), "autotune": r["tuned"],
                    "parallelism": "1 GPU, one cell per lane, persistent lane-refilling kernel"},
         "attempts_per_s": att / wall,
         "accepted_steps": acc,

@@ -106,6 +106,32 @@ typedef struct SimkGenOptions {
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
+/* Persistent launches keep the workspace of all resident waves within this
+ * many bytes (default 224 MiB, below the MI355X's 256 MiB Infinity Cache; the
+ * environment variable SIMK_CACHE_BUDGET_MB sets it at gpu_init; 0 = launch
+ * as many waves as fit the register file). simk_kin_set_grid overrides it. */
+GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
+
+/* Autotuner: times one advance of `duration` seconds (0: L's duration) of
+ * cells [0, sample_cells) of L (all cells when 0; on a device copy, so L's
+ * block is untouched) under candidate settings and
+ * keeps the fastest for later FP64 advances of the current mechanism:
+ *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
+ *   2. at the best grid, the generator's load-pipeline depth (8, 24), register
+ *      window (4, 16) and unrolled stages, one knob at a time;
+ *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
+ * Every candidate computes the same bits; only time differs. */
+typedef struct SimkTuneResult {
+    u32 mode;            /* 0 one lane per cell, 1 one wave per cell */
+    u32 lanes_per_cell;  /* 1 or 64 */
+    u32 waves;           /* persistent grid (0: cache-aware default) */
+    u32 batch, reg_window, unroll_stages;
+    u32 candidates, sample_cells;
+    f32 best_ms;         /* kernel time of the chosen settings on the sample */
+    f32 default_ms;      /* the same under the defaults */
+} SimkTuneResult;
+GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out);
+
 /* Statistics of the current mechanism and the last advance. */
 typedef struct SimkKinInfo {
     u32 kernel;          /* last launch: 1 generated FP64, 2 table FP64, 3 mixed precision, 4 multi-mechanism, 5 f32 delta */

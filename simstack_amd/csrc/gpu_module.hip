@@ -238,6 +238,7 @@ struct State {
     SimkDevMethod dev_method{};
     void *d_tables = nullptr;   // SimkDevTables followed by its arrays
     void *d_wave = nullptr;     // SimkWaveTables followed by its lists (cooperative mode)
+    u64 cache_budget = 208ull << 20;   // workspace of all resident waves kept within this (0: no limit)
     JitKernel jit;
     bool jit_ok = false;
     JitKernel jit_mixed;       // compiled on the first mixed-precision launch
@@ -684,6 +685,23 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     return 0;
 }
 
+// Persistent waves to launch: the occupancy limit, lowered so the workspace
+// of all resident waves stays within the last-level cache budget (the 256 MiB
+// Infinity Cache of the MI355X). Every workspace access then stays on chip:
+// for the 48/150 mechanism (449 KB of workspace per wave) 512 waves run 24 %
+// faster than the 1,024 that fit the register file. At least one wave per CU.
+u32 resident_waves(int per_cu, u64 ws_bytes_per_wave) {
+    if (g.grid_override) return g.grid_override;
+    const u32 cus = g.cus ? g.cus : 1;
+    u64 r = (u64)(per_cu > 0 ? per_cu : 1) * cus;
+    if (g.cache_budget && ws_bytes_per_wave) {
+        u64 fit = g.cache_budget / ws_bytes_per_wave;
+        if (fit < cus) fit = cus;
+        if (fit < r) r = fit;
+    }
+    return (u32)r;
+}
+
 // Method and control in the lane type T (lane_splat's rounding for f32, base_lane.h:98).
 template <class T>
 SimkTConsts<T> t_consts() {
@@ -791,8 +809,7 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
         g.blocks_per_cu_d32 = blocks > 0 ? blocks : 1;
     }
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
-    u32 resident = (u32)g.blocks_per_cu_d32 * (g.cus ? g.cus : 1);
-    if (g.grid_override) resident = g.grid_override;
+    const u32 resident = resident_waves(g.blocks_per_cu_d32, (u64)slots * SIMK_WAVE * 4);
     const u32 waves = needed < resident ? needed : resident;
     hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(float));
     if (e != hipSuccess) return e;
@@ -864,8 +881,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
         return hipErrorInvalidValue;
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const int per_cu = use_gen ? jk->blocks_per_cu : g.blocks_per_cu_table;
-    u32 resident = (u32)(per_cu > 0 ? per_cu : 1) * (g.cus ? g.cus : 1);
-    if (g.grid_override) resident = g.grid_override;
+    const u32 resident = resident_waves(per_cu, (u64)slots * SIMK_WAVE * 8);
     const u32 waves = needed < resident ? needed : resident;
     hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(double));
     if (e != hipSuccess) return e;
@@ -994,8 +1010,7 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
     }
     P.slots = g.multi_jit.ws_slots;
     if ((u64)P.slots * SIMK_WAVE * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
-    u32 resident = (u32)(g.multi_jit.blocks_per_cu > 0 ? g.multi_jit.blocks_per_cu : 1) * (g.cus ? g.cus : 1);
-    if (g.grid_override) resident = g.grid_override;
+    const u32 resident = resident_waves(g.multi_jit.blocks_per_cu, (u64)P.slots * SIMK_WAVE * 8);
     P.waves = (u32)(total_waves < resident ? total_waves : resident);
     if (P.waves == 0) P.waves = 1;
     const u64 ws_bytes = (u64)P.waves * P.slots * SIMK_WAVE * sizeof(double);
@@ -1053,7 +1068,7 @@ SimkMultiShape simk_multi_shape() {
     if (!g.multi_ok) return S;
     S.count = (u32)g.multi_mechs.size();
     S.slots = g.multi_jit.ws_slots;
-    S.resident_waves = (u32)(g.multi_jit.blocks_per_cu > 0 ? g.multi_jit.blocks_per_cu : 1) * (g.cus ? g.cus : 1);
+    S.resident_waves = resident_waves(g.multi_jit.blocks_per_cu, (u64)S.slots * SIMK_WAVE * 8);
     for (u32 i = 0; i < S.count; ++i) {
         S.n[i] = g.multi_mechs[i].n;
         S.R[i] = g.multi_mechs[i].R;
@@ -1146,6 +1161,7 @@ GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info) {
     g.arena.used = 0;
     g.device = (int)device_index;
     g.cus = (u32)prop.multiProcessorCount;
+    if (const char *mb = getenv("SIMK_CACHE_BUDGET_MB")) g.cache_budget = (u64)strtoull(mb, nullptr, 10) << 20;
     g.ready = true;
     return 0;
 }
@@ -1306,6 +1322,187 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
         g.jit_mixed_ok = false;
     }
     g.sweeps = sw;
+    return 0;
+}
+
+// ---- autotuner ------------------------------------------------------------
+// A device copy of cells [0, cells) of a launch's block (compact, own row
+// stride), so candidates run on identical inputs without touching the caller's.
+struct TuneSample {
+    u8 *mem = nullptr;
+    GpuKinLaunch orig{}, work{};
+    u64 half = 0;   // bytes of one copy
+};
+
+hipError_t tune_sample(const GpuKinLaunch *L, u32 cells, TuneSample &S) {
+    const Mech &m = g.mech;
+    const u64 p = ((u64)cells + 63) & ~63ull;
+    const u32 rows[12] = {m.n, 1, 1, 1, 1, 1, m.R, m.n, m.n, 1, 1, 1};
+    const f64 *src[12] = {L->y, L->h, L->steps, L->rejects, L->pulses, L->failed,
+                          L->k, L->src, L->pulse, L->period, L->pulse_first, L->pulse_count};
+    u64 total = 0;
+    for (u32 r : rows) total += r;
+    S.half = total * p * 8;
+    hipError_t e = hipMalloc(&S.mem, 2 * S.half);
+    if (e != hipSuccess) return e;
+    f64 *dst[2][12];
+    for (int c = 0; c < 2; ++c) {
+        f64 *b = (f64 *)(S.mem + c * S.half);
+        for (int i = 0; i < 12; ++i) {
+            dst[c][i] = b;
+            b += rows[i] * p;
+        }
+    }
+    for (int i = 0; i < 12 && e == hipSuccess; ++i)
+        e = hipMemcpy2DAsync(dst[0][i], p * 8, src[i], (u64)L->padded * 8, (u64)cells * 8, rows[i],
+                             hipMemcpyDeviceToDevice, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    for (int c = 0; c < 2; ++c) {
+        GpuKinLaunch &T = c ? S.work : S.orig;
+        T = *L;
+        T.y = dst[c][0]; T.h = dst[c][1]; T.steps = dst[c][2]; T.rejects = dst[c][3]; T.pulses = dst[c][4];
+        T.failed = dst[c][5]; T.k = dst[c][6]; T.src = dst[c][7]; T.pulse = dst[c][8]; T.period = dst[c][9];
+        T.pulse_first = dst[c][10]; T.pulse_count = dst[c][11];
+        T.padded = (u32)p;
+        T.lane_count = cells;
+    }
+    return e;
+}
+
+// Kernel milliseconds of one advance of the sample under the current settings
+// (best of `reps`; inputs restored before each run).
+hipError_t tune_time(TuneSample &S, u32 reps, f32 &best) {
+    best = 1e30f;
+    for (u32 r = 0; r < reps; ++r) {
+        hipError_t e = hipMemcpyAsync(S.mem + S.half, S.mem, S.half, hipMemcpyDeviceToDevice, g.stream);
+        if (e != hipSuccess) return e;
+        hipEventRecord(g.kin_start, g.stream);
+        e = launch(&S.work, 1);
+        if (e != hipSuccess) return e;
+        hipEventRecord(g.kin_stop, g.stream);
+        e = hipEventSynchronize(g.kin_stop);
+        if (e != hipSuccess) return e;
+        f32 ms = 0.0f;
+        hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+        if (ms < best) best = ms;
+    }
+    return hipSuccess;
+}
+
+GPU_API s32 simk_kin_set_cache_budget(u64 bytes) {
+    g.cache_budget = bytes;
+    return 0;
+}
+
+GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
+    if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision != 0) return err_code(hipErrorInvalidValue);
+    memset(out, 0, sizeof(*out));
+    const u32 lanes = L->lane_count ? L->lane_count : L->padded;
+    const u32 cells = sample_cells && sample_cells < lanes ? sample_cells : lanes;
+    if (cells == 0) return err_code(hipErrorInvalidValue);
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return err_code(e);
+    TuneSample S;
+    e = tune_sample(L, cells, S);
+    if (duration > 0.0) S.orig.duration = S.work.duration = duration;
+    const u32 saved_mode = g.mode, saved_grid = g.grid_override, saved_kind = g.kernel_kind;
+    const GenOptions saved_gen = g.gen;
+    const u32 cus = g.cus ? g.cus : 1;
+    f32 best = 1e30f, t = 0.0f;
+    u32 best_mode = 0, best_grid = 0;
+    GenOptions best_gen = g.gen;
+    auto consider = [&](u32 mode, u32 grid, const GenOptions &opt, u32 sub = 0) -> hipError_t {
+        g.mode = mode;
+        g.grid_override = grid;
+        // sub: time only the first `sub` cells and scale to the sample (a
+        // persistent launch far beyond its resident cells scales linearly)
+        const u32 use = sub && sub < cells ? sub : cells;
+        S.work.lane_count = use;
+        if (mode == 0 && (opt.batch != g.gen.batch || opt.reg_window != g.gen.reg_window ||
+                          opt.unroll_stages != g.gen.unroll_stages || !g.jit_ok)) {
+            JitKernel k;
+            if (!load_generated(opt, k)) return hipSuccess;   // variant does not build: skip it
+            if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
+            g.jit = k;
+            g.jit_ok = true;
+            g.gen = opt;
+        }
+        hipError_t r = tune_time(S, 2, t);
+        S.work.lane_count = cells;
+        if (r != hipSuccess) return r;
+        t = t * ((f32)cells / (f32)use);
+        out->candidates++;
+        if (mode == 0 && grid == 0 && opt.batch == saved_gen.batch && opt.reg_window == saved_gen.reg_window &&
+            opt.unroll_stages == saved_gen.unroll_stages)
+            out->default_ms = t;
+        if (t < best) {
+            best = t;
+            best_mode = mode;
+            best_grid = grid;
+            best_gen = opt;
+        }
+        return hipSuccess;
+    };
+    g.kernel_kind = 0;
+    // 1. waves per CU (0: the cache-aware default), one cell per lane
+    const u32 grids[] = {0, cus, cus * 3 / 2, cus * 2, cus * 5 / 2, cus * 3, cus * 4};
+    for (u32 i = 0; i < sizeof(grids) / sizeof(grids[0]) && e == hipSuccess; ++i) e = consider(0, grids[i], g.gen);
+    // 2. generator knobs at the best grid, one at a time
+    const u32 batches[] = {8, 24}, windows[] = {4, 16}, unrolls[] = {1};
+    for (u32 b : batches)
+        if (e == hipSuccess) {
+            GenOptions o = best_gen;
+            o.batch = b;
+            e = consider(0, best_grid, o);
+        }
+    for (u32 w : windows)
+        if (e == hipSuccess) {
+            GenOptions o = best_gen;
+            o.reg_window = w;
+            e = consider(0, best_grid, o);
+        }
+    for (u32 u : unrolls)
+        if (e == hipSuccess) {
+            GenOptions o = best_gen;
+            o.unroll_stages = u;
+            e = consider(0, best_grid, o);
+        }
+    // 3. one wave per cell, timed on at most 16 resident rounds of its grid
+    if (e == hipSuccess) {
+        int blocks = 1;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_wave_kernel<f64, false>, SIMK_WAVE,
+                                                           (13ull * g.mech.n + 5ull * g.mech.R + g.mech.nnz + 3) * 8 +
+                                                               8ull * g.mech.n);
+        e = consider(1, 0, best_gen, 16u * (u32)(blocks > 0 ? blocks : 1) * cus);
+    }
+    if (S.mem) (void)hipFree(S.mem);
+    g.kernel_kind = saved_kind;
+    if (e != hipSuccess) {
+        g.mode = saved_mode;
+        g.grid_override = saved_grid;
+        return err_code(e);
+    }
+    // apply the winner
+    g.mode = best_mode;
+    g.grid_override = best_grid;
+    if (best_gen.batch != g.gen.batch || best_gen.reg_window != g.gen.reg_window ||
+        best_gen.unroll_stages != g.gen.unroll_stages) {
+        JitKernel k;
+        if (load_generated(best_gen, k)) {
+            if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
+            g.jit = k;
+            g.jit_ok = true;
+            g.gen = best_gen;
+        }
+    }
+    out->mode = best_mode;
+    out->lanes_per_cell = best_mode ? SIMK_WAVE : 1;
+    out->waves = best_grid;
+    out->batch = best_gen.batch;
+    out->reg_window = best_gen.reg_window;
+    out->unroll_stages = best_gen.unroll_stages;
+    out->best_ms = best;
+    out->sample_cells = cells;
     return 0;
 }
 

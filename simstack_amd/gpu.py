@@ -28,6 +28,15 @@ class SimkKinInfo(C.Structure):
                 ("signature", C.c_uint64), ("kernel_name", C.c_char * 64)]
 
 
+class SimkTuneResult(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("lanes_per_cell", C.c_uint32), ("waves", C.c_uint32), ("batch", C.c_uint32),
+                ("reg_window", C.c_uint32), ("unroll_stages", C.c_uint32), ("candidates", C.c_uint32),
+                ("sample_cells", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SimkServeConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_requests", C.c_uint32), ("steps", C.c_uint32),
                 ("substeps", C.c_uint32), ("slice_us", C.c_uint32), ("dt", C.c_double)]
@@ -56,7 +65,8 @@ EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_s
                      "simk_jit_precompile", "simk_copy_device", "simk_upwind_f64", "simk_upwind_f32",
                      "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
                      "simk_graph_destroy", "simk_jit_precompile_multi", "simk_serve_open", "simk_serve_submit",
-                     "simk_serve_step", "simk_serve_stats", "simk_serve_close")
+                     "simk_serve_step", "simk_serve_stats", "simk_serve_close", "simk_kin_set_cache_budget",
+                     "simk_kin_autotune")
 
 
 def _sig(lib, name, res, *args):
@@ -106,6 +116,8 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_graph_destroy", C.c_int32, C.c_uint32)
     _sig(lib, "simk_upwind_f64", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
     _sig(lib, "simk_upwind_f32", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
+    _sig(lib, "simk_kin_set_cache_budget", C.c_int32, C.c_uint64)
+    _sig(lib, "simk_kin_autotune", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, C.c_double, P(SimkTuneResult))
     _sig(lib, "simk_serve_open", C.c_int32, P(SimkServeConfig))
     _sig(lib, "simk_serve_submit", C.c_int32, P(SimkServeRequest), P(C.c_uint32))
     _sig(lib, "simk_serve_step", C.c_int32, P(C.c_uint32), C.c_uint32, P(C.c_uint32))
@@ -238,6 +250,9 @@ class GpuModule:
     def serve_close(self):
         self._check(self.lib.simk_serve_close(), "simk_serve_close")
 
+    def set_cache_budget(self, nbytes):
+        self._check(self.lib.simk_kin_set_cache_budget(nbytes), "simk_kin_set_cache_budget")
+
     def set_grid(self, waves):
         self._check(self.lib.simk_kin_set_grid(waves), "simk_kin_set_grid")
 
@@ -289,6 +304,15 @@ class DeviceZones:
         L.precision = precision
         L.duration = duration
         return L
+
+    def autotune(self, duration=0.0, sample_cells=0):
+        """simk_kin_autotune on this block (a device copy of it); the chosen
+        settings stay in effect for later FP64 advances."""
+        L = self.launch_desc(duration or 1.0)
+        res = SimkTuneResult()
+        self.gm._check(self.gm.lib.simk_kin_autotune(C.byref(L), sample_cells, duration, C.byref(res)),
+                       "simk_kin_autotune")
+        return res
 
     def advance(self, duration, lane_count=0, precision=0, repeats=1):
         """gpu_kin_advance (or, repeats > 1, simk_kin_advance_n) on the device block;

@@ -203,3 +203,33 @@ def test_mixed_precision_accuracy(gm, sweeps):
           f"steps mixed/fp64 {a.steps[lanes].sum() / b.steps[:len(lanes)].sum():.4f}")
     assert a.failed[lanes].sum() == 0
     assert worst["major >1e-6"] < 1e-4
+
+
+def test_autotune_picks_settings_and_keeps_bits(gm):
+    """simk_kin_autotune times candidate grids, generator variants and one wave
+    per cell on a device copy of the block, leaves the block untouched, and the
+    advance under the chosen settings is still bit-identical to the oracle."""
+    from simstack_amd import gpu
+    sp, mech, z = make_problem("c3c", 3000)
+    meth, ctl = problem.method(2), problem.control()
+    a = z.copy()
+    gm.arena_reset()
+    gm.set_kernel(0)
+    gm.set_mech(mech, meth, ctl)
+    dz = gpu.DeviceZones(gm, a)
+    dz.upload()
+    try:
+        res = dz.autotune(duration=0.02)
+        assert res.candidates >= 8 and res.best_ms > 0 and res.best_ms <= res.default_ms
+        assert res.lanes_per_cell in (1, 64)
+        dz.download(mutable_only=False)
+        assert a.block.tobytes() == z.block.tobytes()     # tuning ran on a copy
+        dz.advance(0.1)
+        dz.download(mutable_only=False)
+    finally:
+        gm.lib.gpu_kin_set_mode(0)
+        gm.set_grid(0)
+    lanes = np.arange(0, 3000, 11)
+    b = z.subset(lanes)
+    oracle_run(mech, meth, ctl, b, [0.1])
+    assert compare_bits(a.subset(lanes), b) == ZERO

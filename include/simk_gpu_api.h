@@ -102,7 +102,7 @@ typedef struct SimkGenOptions {
     u32 batch;           /* loads issued this many statements ahead (0: 16) */
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
-    u32 reserved;
+    u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 48; ~0u: never) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
@@ -118,14 +118,17 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * keeps the fastest for later FP64 advances of the current mechanism:
  *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
  *   2. at the best grid, the generator's load-pipeline depth (8, 24), register
- *      window (4, 16) and unrolled stages, one knob at a time;
+ *      window (4, 16), unrolled stages and rate-constant reuse window (none,
+ *      96), one knob at a time;
  *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
- * Every candidate computes the same bits; only time differs. */
+ * Every candidate computes the same bits; only time differs. The tuned
+ * settings last until the next gpu_kin_set_mech / simk_kin_set_mechs, which
+ * restores the caller's own. */
 typedef struct SimkTuneResult {
     u32 mode;            /* 0 one lane per cell, 1 one wave per cell */
     u32 lanes_per_cell;  /* 1 or 64 */
     u32 waves;           /* persistent grid (0: cache-aware default) */
-    u32 batch, reg_window, unroll_stages;
+    u32 batch, reg_window, unroll_stages, reuse;
     u32 candidates, sample_cells;
     f32 best_ms;         /* kernel time of the chosen settings on the sample */
     f32 default_ms;      /* the same under the defaults */

@@ -18,6 +18,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <vector>
 
 namespace simk {
@@ -49,13 +50,17 @@ struct Op {
     std::string load;      // "" when the statement needs no load
     std::string stmt;      // "%s" stands for the loaded value
     bool barrier = false;  // loads never move across it (fences, stores they depend on)
+    bool immutable = false;   // the load reads a slot no statement writes (a rate constant)
 };
 
 // Emits ops with their loads software-pipelined `batch` ops ahead: the loads
 // of ops [b + batch, b + 2 batch) are issued before the statements of
-// [b, b + batch) run. Every loaded value is its own SSA name.
-void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent, u32 &uid) {
+// [b, b + batch) run. Every loaded value is its own SSA name, except that an
+// immutable load issued at most `reuse` ops earlier (in any segment) is not
+// repeated: the op uses the earlier value.
+void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent, u32 &uid, u32 reuse = 0) {
     if (batch == 0) batch = 1;
+    std::map<std::string, std::pair<std::string, size_t>> recent;   // load -> (value, op index)
     size_t seg = 0;
     while (seg < ops.size()) {
         size_t end = seg;
@@ -65,11 +70,21 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
         size_t issued = seg;
         auto issue_until = [&](size_t limit) {
             for (; issued < limit && issued < end; ++issued) {
-                if (ops[issued].load.empty()) continue;
+                const Op &op = ops[issued];
+                if (op.load.empty()) continue;
+                if (reuse && op.immutable) {
+                    auto it = recent.find(op.load);
+                    if (it != recent.end() && issued - it->second.second <= reuse) {
+                        name[issued - seg] = it->second.first;
+                        it->second.second = issued;   // the value stays live until this use
+                        continue;
+                    }
+                }
                 char buf[32];
                 snprintf(buf, sizeof(buf), "q%u", uid++);
                 name[issued - seg] = buf;
-                w("%sconst double %s = %s;\n", indent, buf, ops[issued].load.c_str());
+                w("%sconst double %s = %s;\n", indent, buf, op.load.c_str());
+                if (reuse && op.immutable) recent[op.load] = {buf, issued};
             }
         };
         issue_until(seg + 2 * batch);
@@ -207,6 +222,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             Op op;
             snprintf(buf, sizeof(buf), "K(%u)", r);
             op.load = buf;
+            op.immutable = true;
             std::string st = "{ const double rate = %s";
             for (u32 p = 0; p < x.reactant_count; ++p) {
                 snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[p]);
@@ -224,7 +240,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         }
         w("    SIMK_DEV void fun(const Vec &y, Vec &f) const {\n");
         u32 uid = 0;
-        emit_ops(w, ops, o.batch, "        ", uid);
+        emit_ops(w, ops, o.batch, "        ", uid, o.reuse);
         w("    }\n");
     }
 
@@ -281,18 +297,21 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 snprintf(buf, sizeof(buf), "double a%u = 0.0;", idx);
                 stmt(buf);
             }
-            // -J contributions, per entry in (r, p, e) order.
+            // -J contributions, per entry in (r, p, e) order: one op per
+            // reaction touching row i, its reactants' terms in p order.
             for (u32 r = 0; r < m.R; ++r) {
                 const Reaction &x = m.rx[r];
+                bool any = false;
+                for (u32 e = 0; e < x.net_count; ++e)
+                    if (x.net_species[e] == i) any = true;
+                if (!any || x.reactant_count == 0) continue;
+                Op op;
+                snprintf(buf, sizeof(buf), "K(%u)", r);
+                op.load = buf;
+                op.immutable = true;
+                std::string st = "{ const double k = %s;";
                 for (u32 p = 0; p < x.reactant_count; ++p) {
-                    bool any = false;
-                    for (u32 e = 0; e < x.net_count; ++e)
-                        if (x.net_species[e] == i) any = true;
-                    if (!any) continue;
-                    Op op;
-                    snprintf(buf, sizeof(buf), "K(%u)", r);
-                    op.load = buf;
-                    std::string st = "{ const double d = %s";
+                    st += " { const double d = k";
                     for (u32 q = 0; q < x.reactant_count; ++q)
                         if (q != p) {
                             snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[q]);
@@ -306,9 +325,10 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                         st += buf;
                     }
                     st += " }";
-                    op.stmt = st;
-                    ops.push_back(op);
                 }
+                st += " }";
+                op.stmt = st;
+                ops.push_back(op);
             }
             // Eliminations against finished rows k < i: register values, or
             // workspace reloads (fenced mode / beyond the register window).
@@ -371,7 +391,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
         if (mixed) w("        const float gf = (float)g;\n");
         u32 uid = 0;
-        emit_ops(w, ops, o.batch, "        ", uid);
+        emit_ops(w, ops, o.batch, "        ", uid, o.reuse);
         w("    }\n");
     }
 

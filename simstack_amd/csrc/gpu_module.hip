@@ -229,6 +229,11 @@ struct State {
     u32 mode = 0;
     u32 kernel_kind = 0;   // 0 auto, 1 generated, 2 table
     GenOptions gen;
+    // Settings the autotuner replaced, restored when the mechanism changes
+    // (tuned settings belong to the mechanism they were measured on).
+    bool tuned = false;
+    GenOptions user_gen;
+    u32 user_mode = 0, user_grid = 0;
     // current mechanism
     bool have_mech = false;
     Mech mech;
@@ -645,6 +650,14 @@ bool load_generated(const GenOptions &opt, JitKernel &out) {
     return ok;
 }
 
+void untune() {
+    if (!g.tuned) return;
+    g.gen = g.user_gen;
+    g.mode = g.user_mode;
+    g.grid_override = g.user_grid;
+    g.tuned = false;
+}
+
 s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *control) {
     if (!g.ready) return err_code(hipErrorNotInitialized);
     if (!method || !control) return err_code(hipErrorInvalidValue);
@@ -653,6 +666,7 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     if (!v.empty()) return err_code(hipErrorInvalidValue);
     hipError_t e = hipStreamSynchronize(g.stream);
     if (e != hipSuccess) return err_code(e);
+    untune();
     free_mech_resources();
     g.mech = m;
     g.sig = signature(m);
@@ -964,6 +978,7 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.batch = options->batch == 0 ? 16u : options->batch;
     o.precision = options->precision;
     o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
+    o.reuse = options->reuse == 0 ? 48u : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
     return o;
 }
 
@@ -1273,6 +1288,7 @@ GPU_API s32 gpu_kin_wait(f32 *kernel_ms, f32 *total_ms) {
 GPU_API s32 gpu_kin_set_mode(u32 mode) {
     if (mode > 1) return err_code(hipErrorInvalidValue);
     g.mode = mode;
+    g.user_mode = mode;
     return 0;
 }
 
@@ -1293,6 +1309,7 @@ GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *L, u32 repeats, f32 *kernel_m
 
 GPU_API s32 simk_kin_set_grid(u32 waves) {
     g.grid_override = waves;
+    g.user_grid = waves;
     return 0;
 }
 
@@ -1314,6 +1331,8 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.waves_per_eu = n.waves_per_eu;
     g.gen.unroll_stages = n.unroll_stages;
     g.gen.reg_window = n.reg_window;
+    g.gen.reuse = n.reuse;
+    if (g.tuned) g.user_gen = g.gen;
     g.gen.batch = n.batch;
     const u32 sw = n.sweeps;
     if (sw != g.sweeps && g.jit_mixed.mod) {
@@ -1394,6 +1413,12 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes) {
     return 0;
 }
 
+bool same_gen(const GenOptions &a, const GenOptions &b) {
+    return a.fence_rows == b.fence_rows && a.waves_per_eu == b.waves_per_eu && a.unroll_stages == b.unroll_stages &&
+           a.batch == b.batch && a.precision == b.precision && a.sweeps == b.sweeps &&
+           a.reg_window == b.reg_window && a.reuse == b.reuse;
+}
+
 GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
     if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision != 0) return err_code(hipErrorInvalidValue);
     memset(out, 0, sizeof(*out));
@@ -1418,8 +1443,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
         // persistent launch far beyond its resident cells scales linearly)
         const u32 use = sub && sub < cells ? sub : cells;
         S.work.lane_count = use;
-        if (mode == 0 && (opt.batch != g.gen.batch || opt.reg_window != g.gen.reg_window ||
-                          opt.unroll_stages != g.gen.unroll_stages || !g.jit_ok)) {
+        if (mode == 0 && (!same_gen(opt, g.gen) || !g.jit_ok)) {
             JitKernel k;
             if (!load_generated(opt, k)) return hipSuccess;   // variant does not build: skip it
             if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
@@ -1432,9 +1456,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
         if (r != hipSuccess) return r;
         t = t * ((f32)cells / (f32)use);
         out->candidates++;
-        if (mode == 0 && grid == 0 && opt.batch == saved_gen.batch && opt.reg_window == saved_gen.reg_window &&
-            opt.unroll_stages == saved_gen.unroll_stages)
-            out->default_ms = t;
+        if (mode == 0 && grid == 0 && same_gen(opt, saved_gen)) out->default_ms = t;
         if (t < best) {
             best = t;
             best_mode = mode;
@@ -1467,6 +1489,13 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.unroll_stages = u;
             e = consider(0, best_grid, o);
         }
+    const u32 reuses[] = {0, 96};
+    for (u32 r : reuses)
+        if (e == hipSuccess) {
+            GenOptions o = best_gen;
+            o.reuse = r;
+            e = consider(0, best_grid, o);
+        }
     // 3. one wave per cell, timed on at most 16 resident rounds of its grid
     if (e == hipSuccess) {
         int blocks = 1;
@@ -1483,10 +1512,15 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
         return err_code(e);
     }
     // apply the winner
+    if (!g.tuned) {
+        g.tuned = true;
+        g.user_gen = saved_gen;
+        g.user_mode = saved_mode;
+        g.user_grid = saved_grid;
+    }
     g.mode = best_mode;
     g.grid_override = best_grid;
-    if (best_gen.batch != g.gen.batch || best_gen.reg_window != g.gen.reg_window ||
-        best_gen.unroll_stages != g.gen.unroll_stages) {
+    if (!same_gen(best_gen, g.gen)) {
         JitKernel k;
         if (load_generated(best_gen, k)) {
             if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
@@ -1501,6 +1535,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     out->batch = best_gen.batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
+    out->reuse = best_gen.reuse;
     out->best_ms = best;
     out->sample_cells = cells;
     return 0;
@@ -1595,6 +1630,7 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
         if (!mechs[i] || !from_bytes(mechs[i], caps[i], ms[i]).empty()) return err_code(hipErrorInvalidValue);
     hipError_t e = hipStreamSynchronize(g.stream);
     if (e != hipSuccess) return err_code(e);
+    untune();
     destroy_graphs();   // they launch the kernel of the module unloaded here
     if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
     g.multi_jit = JitKernel();

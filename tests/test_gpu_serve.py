@@ -49,7 +49,7 @@ def _serve(gm, pools, reqs, budget=8 << 30, submit_after=None, slice_us=0):
             for t in gm.serve_step():
                 ticket_of.pop(t)
             it += 1
-            assert it < 1000
+            assert it < 20000
         st = gm.serve_stats()
     finally:
         gm.serve_close()
@@ -107,12 +107,12 @@ def test_request_same_alone_in_mix_and_oracle(gm, pools):
 
 
 def test_time_slices_keep_bits(gm, pools):
-    """Slices of 0.3 ms stop cells mid-step many times (their state saved to
+    """Slices of 1 ms stop cells mid-step many times (their state saved to
     the block and resume rows, continued in later launches); the results equal
     the unsliced run's bit for bit."""
     reqs = [(0, 0, 1, 2000), (2, 100, 1, 300), (1, 0, 2, 400)]
     whole, st_whole = _serve(gm, pools, reqs, slice_us=0xFFFFFFFF)
-    sliced, st = _serve(gm, pools, reqs, slice_us=300)
+    sliced, st = _serve(gm, pools, reqs, slice_us=1000)
     assert st_whole.carried == 0
     assert st.carried > 0 and st.iterations > st_whole.iterations
     for i in range(len(reqs)):

@@ -240,8 +240,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64 (bit-exact), 2 mixed precision")
     ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1; ~0u: none)")
-    ap.add_argument("--tune", type=float, default=0.05,
-                    help="autotune on the workload for this many simulated seconds before timing (0: defaults)")
+    ap.add_argument("--tune", type=float, default=0.0,
+                    help="autotune on the workload for this many simulated seconds before timing (0: the defaults)")
     ap.add_argument("--no-ragged", action="store_true", help="skip the three-mechanism ragged batch")
     ap.add_argument("--ragged-steps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")

@@ -98,11 +98,11 @@ typedef struct SimkGenOptions {
     u32 fence_rows;      /* 1: factored rows re-read from the workspace; 0: kept in registers */
     u32 waves_per_eu;    /* amdgpu_waves_per_eu hint (0: none) */
     u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
-    u32 reg_window;      /* rows back a finished row stays in registers (0: 8, ~0u: all) */
+    u32 reg_window;      /* rows back a finished row stays in registers (0: 16 in FP64, 8 mixed; ~0u: all) */
     u32 batch;           /* loads issued this many statements ahead (0: 16) */
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
-    u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 48; ~0u: never) */
+    u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 96; ~0u: never) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
@@ -118,8 +118,8 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * keeps the fastest for later FP64 advances of the current mechanism:
  *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
  *   2. at the best grid, the generator's load-pipeline depth (8, 24), register
- *      window (4, 16), unrolled stages and rate-constant reuse window (none,
- *      96), one knob at a time;
+ *      window (8, 32), unrolled stages and rate-constant reuse window (none,
+ *      48), one knob at a time;
  *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
  * Every candidate computes the same bits; only time differs. The tuned
  * settings last until the next gpu_kin_set_mech / simk_kin_set_mechs, which
@@ -268,6 +268,14 @@ GPU_API s32 simk_serve_submit(const SimkServeRequest *request, u32 *ticket);
 GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done);
 GPU_API s32 simk_serve_stats(SimkServeStats *out);
 GPU_API s32 simk_serve_close(void);
+
+/* Plug flow along the zone chain (dbd_flow_exchange, src/sim/dbd.cpp:465-483):
+ * zone i of the block relaxes toward zone i - 1's old state and zone 0
+ * toward feed[species], y_i += a (up - y_i) with the product rounded on its
+ * own, for zones [0, zones) of every species row (row stride padded,
+ * zones <= 8192). a = dt * zones / residence must be in [0, 1]. Enqueued on
+ * the module stream. */
+GPU_API s32 simk_plugflow_f64(f64 *y, u64 padded, u32 species, u32 zones, f64 a, const f64 *feed);
 
 /* Measurement aid: device-to-device copy of `bytes` (16-byte multiple and
  * alignment) by a plain streaming kernel, timed with stream events. HBM

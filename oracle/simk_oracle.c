@@ -862,3 +862,16 @@ ORACLE_API void oracle_upwind_f64(f64 *x, const f64 *c, u32 nodes, u32 fields, u
             }
         }
 }
+
+/* dbd_flow_exchange (src/sim/dbd.cpp:465-483): zones processed back to front,
+ * zone i relaxes toward zone i-1's old state (zone 0 toward the feed) with
+ * y += a * (up - y), the product and the sum rounded separately. y: [species]
+ * rows of stride `stride`. */
+ORACLE_API void oracle_plugflow_f64(f64 *y, u32 stride, u32 species, u32 zones, f64 a, const f64 *feed) {
+    for (u32 i = zones; i-- > 0;)
+        for (u32 s = 0; s < species; ++s) {
+            f64 *v = y + (size_t)s * stride + i;
+            f64 up = i == 0 ? feed[s] : y[(size_t)s * stride + i - 1];
+            *v += a * (up - *v);
+        }
+}

@@ -39,6 +39,7 @@ ORACLE_API s32 oracle_advance_delta32(const void *mech, const SimkCaps *caps, co
                                       u32 threads);
 ORACLE_API f32 oracle_cbrt_clamped_f32(f32 x);
 ORACLE_API f32 oracle_step_factor_f32(f32 err, u32 order);
+ORACLE_API void oracle_plugflow_f64(f64 *y, u32 stride, u32 species, u32 zones, f64 a, const f64 *feed);
 ORACLE_API void oracle_upwind(f32 *x, const f32 *c, u32 nodes, u32 fields, u32 stride, u32 substeps, f32 *outlet_sum);
 ORACLE_API void oracle_upwind_f64(f64 *x, const f64 *c, u32 nodes, u32 fields, u32 stride, u32 substeps,
                                   const f64 *inflow);

@@ -34,9 +34,12 @@ __device__ __forceinline__ float fma_t<float>(float a, float b, float c) {
 
 // One (field, channel) row of `nodes` values at xr, advanced `substeps`
 // substeps by the whole workgroup (kThreads threads, CHUNK nodes each).
-template <class T, unsigned CHUNK>
+// FUSED: x + c (up - x) as one fused multiply-add (nft.cpp:1072); otherwise
+// the plug-flow form x + (c * (up - x)) with the product rounded on its own
+// (dbd.cpp:480). cr null: every node's coefficient is c_all.
+template <class T, unsigned CHUNK, bool FUSED = true>
 __device__ __forceinline__ void upwind_row(T *xr, const T *cr, unsigned nodes, unsigned substeps, T in,
-                                           T *outlet) {
+                                           T *outlet, T c_all = T(0)) {
     const unsigned t = threadIdx.x;
     __shared__ T stage[kThreads * CHUNK];
     __shared__ T last[kThreads];
@@ -48,10 +51,15 @@ __device__ __forceinline__ void upwind_row(T *xr, const T *cr, unsigned nodes, u
 #pragma unroll
     for (unsigned i = 0; i < CHUNK; ++i) v[i] = stage[t * CHUNK + i];
     __syncthreads();
-    for (unsigned j = t; j < kThreads * CHUNK; j += kThreads) stage[j] = j < nodes ? cr[j] : T(0);
-    __syncthreads();
+    if (cr) {
+        for (unsigned j = t; j < kThreads * CHUNK; j += kThreads) stage[j] = j < nodes ? cr[j] : T(0);
+        __syncthreads();
 #pragma unroll
-    for (unsigned i = 0; i < CHUNK; ++i) c[i] = stage[t * CHUNK + i];
+        for (unsigned i = 0; i < CHUNK; ++i) c[i] = stage[t * CHUNK + i];
+    } else {
+#pragma unroll
+        for (unsigned i = 0; i < CHUNK; ++i) c[i] = c_all;
+    }
 
     const unsigned first = t * CHUNK;
     const unsigned last_thread = (nodes - 1) / CHUNK;
@@ -65,7 +73,7 @@ __device__ __forceinline__ void upwind_row(T *xr, const T *cr, unsigned nodes, u
 #pragma unroll
         for (unsigned i = CHUNK; i-- > 0;) {
             const T up = i ? v[i - 1] : left;
-            if (first + i < nodes) v[i] = fma_t<T>(c[i], up - v[i], v[i]);
+            if (first + i < nodes) v[i] = FUSED ? fma_t<T>(c[i], up - v[i], v[i]) : v[i] + c[i] * (up - v[i]);
         }
         __syncthreads();
     }
@@ -85,6 +93,15 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned lo
     const unsigned field = row / channels, ch = row % channels;
     upwind_row<T, CHUNK>(x + field * padded + (unsigned long long)ch * nodes, courant + (unsigned long long)ch * nodes,
                          nodes, substeps, inflow ? inflow[field] : T(0), outlet_sum ? outlet_sum + row : nullptr);
+}
+
+// Plug flow along a chain of zones (dbd_flow_exchange, dbd.cpp:465-483): one
+// workgroup per species row; zone 0 relaxes toward the feed.
+template <unsigned CHUNK>
+__global__ void __launch_bounds__(kThreads) simk_plugflow_kernel(double *y, unsigned long long padded, unsigned zones,
+                                                                 double a, const double *feed) {
+    const unsigned s = blockIdx.x;
+    upwind_row<double, CHUNK, false>(y + s * padded, nullptr, zones, 1, feed[s], nullptr, a);
 }
 
 // Many requests' rows in one launch: block b belongs to the segment whose
@@ -153,6 +170,25 @@ GPU_API s32 simk_upwind_f64(f64 *y, u64 padded, u32 fields, u32 channels, u32 no
                             u32 substeps, const f64 *inflow) {
     return (s32)launch_upwind<f64>(simk_module_stream(), y, padded, fields, channels, nodes, courant, substeps,
                                    inflow, nullptr);
+}
+
+GPU_API s32 simk_plugflow_f64(f64 *y, u64 padded, u32 species, u32 zones, f64 a, const f64 *feed) {
+    if (!y || !feed || species == 0 || zones == 0 || zones > padded) return (s32)hipErrorInvalidValue;
+    if (!(a >= 0.0 && a <= 1.0)) return (s32)hipErrorInvalidValue;   // the reference asserts a <= 1 (dbd.cpp:473)
+    const u32 chunk_needed = (zones + kThreads - 1) / kThreads;
+    const dim3 grid(species), block(kThreads);
+    hipStream_t st = simk_module_stream();
+    if (chunk_needed <= 4)
+        hipLaunchKernelGGL((simk_plugflow_kernel<4>), grid, block, 0, st, y, (unsigned long long)padded, zones, a, feed);
+    else if (chunk_needed <= 16)
+        hipLaunchKernelGGL((simk_plugflow_kernel<16>), grid, block, 0, st, y, (unsigned long long)padded, zones, a,
+                           feed);
+    else if (chunk_needed <= kMaxChunk)
+        hipLaunchKernelGGL((simk_plugflow_kernel<kMaxChunk>), grid, block, 0, st, y, (unsigned long long)padded, zones,
+                           a, feed);
+    else
+        return (s32)hipErrorInvalidValue;
+    return (s32)hipGetLastError();
 }
 
 GPU_API s32 simk_upwind_f32(f32 *x, u64 padded, u32 fields, u32 channels, u32 nodes, const f32 *courant,

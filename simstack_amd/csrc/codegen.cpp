@@ -259,7 +259,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // In register mode a finished entry is read from its register when its
     // row is at most reg_window rows back (0: always), else reloaded from the
     // workspace after a fence, which bounds live ranges on dense mechanisms.
-    auto near = [&](u32 idx, u32 i) { return o.reg_window == 0 || i - row_of[idx] <= o.reg_window; };
+    const u32 window = o.reg_window == GenOptions::kDefaultWindow ? (mixed ? 8u : 16u) : o.reg_window;
+    auto near = [&](u32 idx, u32 i) { return window == 0 || i - row_of[idx] <= window; };
     auto fin = [&](u32 idx, u32 i) {
         char buf[24];
         snprintf(buf, sizeof(buf), (regs && near(idx, i)) ? "a%u" : "W(%u)", idx);

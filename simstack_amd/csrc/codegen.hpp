@@ -19,8 +19,10 @@ struct GenOptions {
     u32 batch = 16;          // loads issued this many statements ahead (software pipelining)
     u32 precision = 0;       // 0: FP64 throughout; 2: FP32 Jacobian/LU + FP64 residual refinement
     u32 sweeps = 1;          // precision 2: refinement sweeps per stage solve
-    u32 reg_window = 8;      // register mode: finished rows read from registers up to this many rows back (0: all)
-    u32 reuse = 48;          // a rate constant loaded within this many statements is reused, not reloaded (0: never)
+    u32 reg_window = kDefaultWindow;   // register mode: finished rows read from registers up to this many
+                                       // rows back (0: all; kDefaultWindow: 16 in FP64, 8 in mixed precision)
+    u32 reuse = 96;          // a rate constant loaded within this many statements is reused, not reloaded (0: never)
+    static constexpr u32 kDefaultWindow = 0xFFFF;
 };
 
 struct GenResult {

@@ -974,11 +974,12 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.fence_rows = options->fence_rows;
     o.waves_per_eu = options->waves_per_eu;
     o.unroll_stages = options->unroll_stages;
-    o.reg_window = options->reg_window == 0 ? 8u : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
+    o.reg_window = options->reg_window == 0 ? GenOptions::kDefaultWindow
+                                            : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
     o.batch = options->batch == 0 ? 16u : options->batch;
     o.precision = options->precision;
     o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
-    o.reuse = options->reuse == 0 ? 48u : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
+    o.reuse = options->reuse == 0 ? 96u : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
     return o;
 }
 
@@ -1470,7 +1471,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     const u32 grids[] = {0, cus, cus * 3 / 2, cus * 2, cus * 5 / 2, cus * 3, cus * 4};
     for (u32 i = 0; i < sizeof(grids) / sizeof(grids[0]) && e == hipSuccess; ++i) e = consider(0, grids[i], g.gen);
     // 2. generator knobs at the best grid, one at a time
-    const u32 batches[] = {8, 24}, windows[] = {4, 16}, unrolls[] = {1};
+    const u32 batches[] = {8, 24}, windows[] = {8, 32}, unrolls[] = {1};
     for (u32 b : batches)
         if (e == hipSuccess) {
             GenOptions o = best_gen;
@@ -1489,7 +1490,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.unroll_stages = u;
             e = consider(0, best_grid, o);
         }
-    const u32 reuses[] = {0, 96};
+    const u32 reuses[] = {0, 48};
     for (u32 r : reuses)
         if (e == hipSuccess) {
             GenOptions o = best_gen;

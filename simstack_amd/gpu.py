@@ -67,7 +67,7 @@ EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_s
                      "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
                      "simk_graph_destroy", "simk_jit_precompile_multi", "simk_serve_open", "simk_serve_submit",
                      "simk_serve_step", "simk_serve_stats", "simk_serve_close", "simk_kin_set_cache_budget",
-                     "simk_kin_autotune")
+                     "simk_kin_autotune", "simk_plugflow_f64")
 
 
 def _sig(lib, name, res, *args):
@@ -117,6 +117,7 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_graph_destroy", C.c_int32, C.c_uint32)
     _sig(lib, "simk_upwind_f64", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
     _sig(lib, "simk_upwind_f32", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_uint32, V)
+    _sig(lib, "simk_plugflow_f64", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, V)
     _sig(lib, "simk_kin_set_cache_budget", C.c_int32, C.c_uint64)
     _sig(lib, "simk_kin_autotune", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, C.c_double, P(SimkTuneResult))
     _sig(lib, "simk_serve_open", C.c_int32, P(SimkServeConfig))

@@ -62,6 +62,7 @@ def oracle():
              V, V, C.c_uint32, C.c_uint32)
         _sig(lib, "oracle_cbrt_clamped_f32", C.c_float, C.c_float)
         _sig(lib, "oracle_step_factor_f32", C.c_float, C.c_float, C.c_uint32)
+        _sig(lib, "oracle_plugflow_f64", None, V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, V)
         _sig(lib, "oracle_upwind", None, V, V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, V)
         _sig(lib, "oracle_upwind_f64", None, V, V, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, V)
         _oracle = lib

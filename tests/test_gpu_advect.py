@@ -81,3 +81,25 @@ def test_upwind_conserves_mass_with_closed_outlet_balance(gm):
         series += 0.5 * w[:, -1]
         oracle_ffi.oracle().oracle_upwind_f64(w.ctypes.data, c.ctypes.data, nodes, fields, nodes, 1, None)
     assert np.allclose(got.sum(axis=1), total0 - series, rtol=1e-12)
+
+
+@pytest.mark.parametrize("zones", [1, 300, 4096, 5000])
+def test_plugflow_matches_oracle(gm, zones):
+    """simk_plugflow_f64 (dbd_flow_exchange on the device): bit for bit, with
+    the product rounded separately from the sum as the reference computes it."""
+    rng = np.random.default_rng(zones)
+    species = 17
+    padded = (zones + 63) // 64 * 64
+    y = rng.random((species, padded)) * 10.0 ** rng.uniform(8, 19, (species, 1))
+    feed = rng.random(species) * 1e19
+    a = 0.3
+    gm.arena_reset()
+    dy, df = _dev(gm, y), _dev(gm, feed)
+    gm._check(gm.lib.simk_plugflow_f64(dy, padded, species, zones, a, df), "plugflow")
+    got = np.empty_like(y)
+    gm.to_host(got, dy)
+    want = y.copy()
+    oracle_ffi.oracle().oracle_plugflow_f64(want.ctypes.data, padded, species, zones, a, feed.ctypes.data)
+    assert got.tobytes() == want.tobytes()
+    with pytest.raises(Exception):
+        gm._check(gm.lib.simk_plugflow_f64(dy, padded, species, zones, 1.5, df), "plugflow")

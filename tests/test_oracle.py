@@ -412,3 +412,25 @@ def test_delta32_keeps_increments_below_f32_resolution():
     dA = z.y[0, 0] - 1.0
     assert dA != 0.0 and abs(dA / -4e-8 - 1.0) < 1e-3
     assert abs(z.y[1, 0] / 4e-8 - 1.0) < 1e-3
+
+
+def test_plugflow_oracle_balance_and_fixed_point():
+    """dbd_flow_exchange restated: the chain only moves mass (sum changes by
+    a (feed - y_last)), leaves a state equal to the feed unchanged, and a = 0
+    changes nothing."""
+    rng = np.random.default_rng(4)
+    species, zones, stride = 5, 37, 64
+    y = rng.random((species, stride)) * 1e15
+    feed = rng.random(species) * 1e15
+    a = 0.37
+    before = y[:, :zones].sum(axis=1)
+    last = y[:, zones - 1].copy()
+    oracle().oracle_plugflow_f64(y.ctypes.data, stride, species, zones, a, feed.ctypes.data)
+    assert np.allclose(y[:, :zones].sum(axis=1), before + a * (feed - last), rtol=1e-12)
+    flat = np.tile(feed[:, None], (1, stride)).copy()
+    oracle().oracle_plugflow_f64(flat.ctypes.data, stride, species, zones, a, feed.ctypes.data)
+    assert flat.tobytes() == np.tile(feed[:, None], (1, stride)).tobytes()
+    z = rng.random((species, stride))
+    z0 = z.copy()
+    oracle().oracle_plugflow_f64(z.ctypes.data, stride, species, zones, 0.0, feed.ctypes.data)
+    assert z.tobytes() == z0.tobytes()

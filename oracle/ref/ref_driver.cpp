@@ -8,6 +8,7 @@
 //   src/sim/kinetics/kin_solver.h    kin_fun / kin_jac / kin_factor / kin_solve / kin_integrate / kin_method
 //   src/sim/dbd_lanes.h              dbd_advance_group<V> over the [row][padded] SoA block
 //   src/base/base_simd.h             f64x4 (AVX2) lane type
+//   src/sim/kinetics/air_mech.cpp    the humid-air mechanism, its feed and discharge source
 // and re-states only the thread chunking of dbd_advance_threads
 // (src/sim/dbd.cpp:228-260) over pthreads instead of the platform work queue.
 //
@@ -29,6 +30,7 @@
 #include "base/base_simd.h"
 #include "sim/kinetics/kin_build.cpp"
 #include "sim/dbd_lanes.h"
+#include "sim/kinetics/air_mech.cpp"
 
 #define REF_API extern "C" __attribute__((visibility("default")))
 
@@ -206,4 +208,48 @@ REF_API s32 ref_advance(const void *mech, const void *method, const void *ctl, c
     free(tids);
     free(items);
     return 0;
+}
+
+// ---- the humid-air DBD mechanism (src/sim/kinetics/air_mech.cpp) ------------
+static KinModel g_air;
+static AirSpecies g_air_sp;
+static b32 g_air_built;
+
+static void ref_air_ensure() {
+    if (g_air_built) return;
+    air_mech_build(&g_air, &g_air_sp);
+    g_air_built = 1;
+}
+
+// KinMech bytes (sizeof(KinMech)) and the 17 species indices in AirSpecies
+// order; returns 1 when the builder reported an error.
+REF_API s32 ref_air_build(void *mech_out, u32 *species_out) {
+    ref_air_ensure();
+    memcpy(mech_out, &g_air.mech, sizeof(KinMech));
+    memcpy(species_out, &g_air_sp, sizeof(AirSpecies));
+    return g_air.error ? 1 : 0;
+}
+
+REF_API void ref_air_rate_constants(f64 T, f64 M, f64 *k) {
+    ref_air_ensure();
+    kin_eval_rate_constants(&g_air, T, M, k);
+}
+
+REF_API f64 ref_air_feed(f64 T, f64 p, f64 x_H2O, f64 *y) {
+    ref_air_ensure();
+    return air_feed(&g_air_sp, g_air.mech.species_count, T, p, x_H2O, y);
+}
+
+// air_source_delta with air_source_defaults(), optionally the table source.
+REF_API void ref_air_source_delta(const f64 *y, f64 energy_J_cm3, f64 T, u32 boltzmann, f64 field_Td, f64 *delta) {
+    ref_air_ensure();
+    AirSourceModel src = air_source_defaults();
+    src.boltzmann = boltzmann;
+    if (boltzmann) src.reduced_field_Td = field_Td;
+    air_source_delta(&g_air_sp, &src, y, g_air.mech.species_count, energy_J_cm3, delta, T);
+}
+
+REF_API void ref_air_count_elements(const f64 *y, f64 *atoms) {
+    ref_air_ensure();
+    kin_count_elements(&g_air, y, atoms);
 }

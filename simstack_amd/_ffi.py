@@ -81,6 +81,24 @@ class SimkRateLaw(C.Structure):
                 ("keq_A", C.c_double), ("keq_B", C.c_double)]
 
 
+class SimkAirSpecies(C.Structure):
+    NAMES = ("O", "O3", "N", "NO", "NO2", "NO3", "N2O5", "N2O", "HNO2", "HNO3", "OH", "HO2", "H2O2", "H", "N2", "O2",
+             "H2O")
+    _fields_ = [(n, C.c_uint32) for n in NAMES]
+
+
+class SimkAirSource(C.Structure):
+    _fields_ = [("g_O2_diss", C.c_double), ("f_O1D", C.c_double), ("g_N2A", C.c_double), ("g_N2_diss", C.c_double),
+                ("g_OH_per_100eV", C.c_double), ("boltzmann", C.c_int32), ("pad_", C.c_uint32),
+                ("reduced_field_Td", C.c_double), ("filament_J_cm3", C.c_double)]
+
+
+class SimkDbdZones(C.Structure):
+    _fields_ = [("n_species", C.c_uint32), ("n_reactions", C.c_uint32), ("count", C.c_uint32), ("padded", C.c_uint32),
+                ("block", C.c_void_p), ("T_K", C.c_void_p), ("p_Pa", C.c_void_p), ("x_H2O", C.c_void_p),
+                ("power_W_cm3", C.c_void_p), ("freq_Hz", C.c_void_p), ("pulse_index", C.c_void_p)]
+
+
 def _sig(lib, name, res, *args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -119,5 +137,17 @@ def host():
     _sig(lib, "simk_synth_mechanism", C.c_uint32, P(SimkSynthSpec), V, V, f64p)
     _sig(lib, "simk_synth_cells", None, P(SimkSynthSpec), f64p, C.c_uint64, C.c_uint32, C.c_uint32,
          V, V, V, V)
+    _sig(lib, "simk_air_model", V, P(SimkAirSpecies))
+    _sig(lib, "simk_air_source_defaults", None, P(SimkAirSource))
+    _sig(lib, "simk_air_feed", C.c_double, P(SimkAirSpecies), C.c_uint32, C.c_double, C.c_double, C.c_double, f64p)
+    _sig(lib, "simk_air_source_delta", C.c_int32, P(SimkAirSpecies), P(SimkAirSource), f64p, C.c_uint32, C.c_double,
+         C.c_double, f64p)
+    _sig(lib, "simk_dbd_fill_padding", None, P(SimkDbdZones))
+    _sig(lib, "simk_dbd_reset", None, P(SimkDbdZones), P(SimkAirSpecies))
+    _sig(lib, "simk_dbd_prepare_rates", None, P(SimkDbdZones), V)
+    _sig(lib, "simk_dbd_prepare_source", C.c_int32, P(SimkDbdZones), P(SimkAirSpecies), P(SimkAirSource),
+         C.c_uint32, C.c_double, C.c_double)
+    _sig(lib, "simk_dbd_commit_pulses", None, P(SimkDbdZones))
+    _sig(lib, "simk_dbd_default_control", None, P(KinControl))
     _host = lib
     return lib

@@ -1,0 +1,98 @@
+"""The DBD reactor on the device, as the reference's own GPU test runs it
+(src/tests/test_kinetics.cpp:1086-1130): the humid-air mechanism through the
+reference ABI (gpu_kin_set_mech at the default capacities), 12 zones with
+varied operating points, 5 master steps of 1 ms with the host preparing each
+step's pulses or averaged source, pulsed and averaged discharges. Every step's
+result is bit-identical to the oracle's: FP64 lanes (generated, table walk and
+one wave per cell) and the reference's f32 delta lanes."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_ffi
+from simstack_amd import _ffi, dbd, gpu, problem
+from helpers import compare_bits
+
+pytestmark = pytest.mark.gpu
+
+ZERO = {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+
+
+@pytest.fixture(scope="module")
+def air():
+    return dbd.AirMechanism(caps=_ffi.DEFAULT_CAPS)
+
+
+def _gpu_run(gm, air, z, mode, steps, precision=0, kernel=0, wave=False):
+    meth, ctl = problem.method(2), dbd.default_control()
+    gm.arena_reset()
+    gm.set_kernel(kernel)
+    gm._check(gm.lib.gpu_kin_set_mech(air.bytes.ctypes.data, C.byref(meth), C.byref(ctl)), "gpu_kin_set_mech")
+    gm._check(gm.lib.gpu_kin_set_mode(1 if wave else 0), "gpu_kin_set_mode")
+    src = dbd.source_defaults()
+    dz = gpu.DeviceZones(gm, z)
+    try:
+        for step in range(steps):
+            t0, t1 = step * 1e-3, (step + 1) * 1e-3
+            z.prepare_source(src, mode, t0, t1)
+            dz.upload()
+            dz.advance(t1 - t0, precision=precision)
+            dz.download(mutable_only=True)
+            z.commit_pulses()
+    finally:
+        gm.lib.gpu_kin_set_mode(0)
+
+
+def _oracle_run(air, z, mode, steps, precision=0):
+    meth, ctl = problem.method(2), dbd.default_control()
+    src = dbd.source_defaults()
+    adv = oracle_ffi.oracle_advance_delta32 if precision == 1 else oracle_ffi.oracle_advance
+    for step in range(steps):
+        t0, t1 = step * 1e-3, (step + 1) * 1e-3
+        z.prepare_source(src, mode, t0, t1)
+        adv(air, meth, ctl, z, t1 - t0, threads=8)
+        z.commit_pulses()
+
+
+@pytest.mark.parametrize("mode", [dbd.PULSED, dbd.AVERAGED], ids=["pulsed", "averaged"])
+@pytest.mark.parametrize("variant", ["generated", "table", "wave"])
+def test_dbd_fp64_bit_exact(gm, air, mode, variant):
+    a = dbd.reference_zones(air, 12)
+    b = a.copy()
+    _gpu_run(gm, air, a, mode, 5, kernel={"generated": 1, "table": 2, "wave": 0}[variant], wave=variant == "wave")
+    _oracle_run(air, b, mode, 5)
+    assert compare_bits(a, b) == ZERO
+    assert b.steps[:12].min() > 0
+    if mode == dbd.PULSED:
+        assert b.pulses[:12].min() >= 50
+
+
+@pytest.mark.parametrize("mode", [dbd.PULSED, dbd.AVERAGED], ids=["pulsed", "averaged"])
+def test_dbd_f32_delta_bit_exact(gm, air, mode):
+    """The reference scene's default precision (f32 delta lanes)."""
+    a = dbd.reference_zones(air, 12)
+    b = a.copy()
+    _gpu_run(gm, air, a, mode, 5, precision=1)
+    _oracle_run(air, b, mode, 5, precision=1)
+    assert compare_bits(a, b) == ZERO
+
+
+def test_dbd_many_zones(gm, air):
+    """2,048 zones x 20 ms pulsed on the device: 64 sampled zones equal the
+    oracle bit for bit; N and O atoms (totals ~1e19 cm^-3) are kept to 1e-10
+    and H atoms (totals ~1e17-1e18, so an atol-sized drift of ~1e8 is 1e-10
+    of them) to 1e-9; no zone fails."""
+    z = dbd.reference_zones(air, 2048)
+    lanes = np.random.default_rng(2).choice(2048, 64, replace=False)
+    sample = z.subset(lanes)
+    before = np.array([air.count_elements(z.y[:, i]) for i in range(2048)])
+    _gpu_run(gm, air, z, dbd.PULSED, 20)
+    _oracle_run(air, sample, dbd.PULSED, 20)
+    got = z.subset(lanes)
+    assert compare_bits(got, sample) == ZERO
+    after = np.array([air.count_elements(z.y[:, i]) for i in range(2048)])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rel = np.where(before > 0, np.abs(after - before) / np.maximum(np.abs(after), np.abs(before)), 0.0)
+    assert rel[:, :2].max() < 1e-10 and rel[:, 2].max() < 1e-9
+    assert z.failed[:2048].sum() == 0

@@ -55,7 +55,7 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration):
 
 
 def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
-                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0):
+                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0, lds_slots=0):
     """Advances `cells` cells of the workload `steps` x 1 s from the seeded initial
     state (after `warmup` untimed seconds on a copy). Returns timings and the block."""
     z0 = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
@@ -68,7 +68,8 @@ def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precis
         gm = run_workload.gm = gpu.GpuModule(arena_bytes=need)
     gm.set_kernel(kernel)
     gm.lib.gpu_kin_set_mode(0)
-    gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps)
+    gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps,
+                       lds_slots=lds_slots)
     gm.set_grid(grid)
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z0.copy())
@@ -118,7 +119,7 @@ def rel_err_groups(got, ref, atol, rtol):
     return out
 
 
-def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0):
+def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0, lds_slots=0):
     """Three mechanisms (60/200, 40/130, 24/70) with 1k / 20k / 200k cells in one
     persistent launch per operator-split step, the steps captured in one HIP
     graph; beside it the same steps as eager launches and as one mechanism at a
@@ -137,7 +138,7 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0)
         gm = run_workload.gm = gpu.GpuModule(arena_bytes=need)
     gm.arena_reset()
     gm.set_grid(0)
-    gm.set_gen_options()
+    gm.set_gen_options(lds_slots=lds_slots)
     dzs = [gpu.DeviceZones(gm, z.copy()) for z in z0s]
 
     def reset():
@@ -238,6 +239,8 @@ def main():
     ap.add_argument("--reg-window", type=int, default=0, help="generator register window (0: default)")
     ap.add_argument("--waves-per-eu", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
+    ap.add_argument("--lds-slots", type=int, default=0,
+                    help="LU entries per lane held in LDS (0: default, -1: none)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64 (bit-exact), 2 mixed precision")
     ap.add_argument("--sweeps", type=int, default=0, help="mixed precision refinement sweeps (0: 1; ~0u: none)")
     ap.add_argument("--tune", type=float, default=0.0,
@@ -259,7 +262,7 @@ def main():
     meth, ctl = problem.method(2), problem.control()
     flops_attempt = mech.flops_per_attempt()
     knobs = dict(kernel=args.kernel, grid=args.grid, reg_window=args.reg_window, waves_per_eu=args.waves_per_eu,
-                 batch=args.batch)
+                 batch=args.batch, lds_slots=args.lds_slots & 0xFFFFFFFF)
     r = run_workload(gpu, problem, mech, sp, meth, ctl, args.cells, args.steps, args.warmup, args.precision,
                      args.sweeps, args.separate, tune=args.tune if not args.grid else 0.0, **knobs)
     kernel_ms = r["kernel_ms"]

@@ -103,11 +103,13 @@ typedef struct SimkGenOptions {
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 96; ~0u: never) */
+    u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default, 80 KiB per wave =
+                            160 f64 / 320 f32 entries, none for refined mixed kernels; ~0u: none; at most 160 KiB) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
 /* Persistent launches keep the workspace of all resident waves within this
- * many bytes (default 224 MiB, below the MI355X's 256 MiB Infinity Cache; the
+ * many bytes (default 208 MiB, below the MI355X's 256 MiB Infinity Cache; the
  * environment variable SIMK_CACHE_BUDGET_MB sets it at gpu_init; 0 = launch
  * as many waves as fit the register file). simk_kin_set_grid overrides it. */
 GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
@@ -118,8 +120,8 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * keeps the fastest for later FP64 advances of the current mechanism:
  *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
  *   2. at the best grid, the generator's load-pipeline depth (8, 24), register
- *      window (8, 32), unrolled stages and rate-constant reuse window (none,
- *      48), one knob at a time;
+ *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48)
+ *      and LU entries in LDS (none, 80, 128, 160), one knob at a time;
  *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
  * Every candidate computes the same bits; only time differs. The tuned
  * settings last until the next gpu_kin_set_mech / simk_kin_set_mechs, which
@@ -129,6 +131,7 @@ typedef struct SimkTuneResult {
     u32 lanes_per_cell;  /* 1 or 64 */
     u32 waves;           /* persistent grid (0: cache-aware default) */
     u32 batch, reg_window, unroll_stages, reuse;
+    u32 lds_slots;       /* LU entries in LDS of the chosen kernel */
     u32 candidates, sample_cells;
     f32 best_ms;         /* kernel time of the chosen settings on the sample */
     f32 default_ms;      /* the same under the defaults */

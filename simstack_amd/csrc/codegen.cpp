@@ -13,7 +13,9 @@
 //               (descending col) then the reciprocal pivot (kin_solver.h:122-139)
 // y, f and the stage vector being solved live in registers; the LU and earlier
 // stage vectors live in the lane's workspace slots ([wave][slot][64] f64, so
-// each access is one coalesced 512-byte wave transaction).
+// each access is one coalesced 512-byte wave transaction). Optionally the
+// first LU entries live in LDS instead ([entry][64] f64 per wave; a lane only
+// ever touches its own column, so no barrier is needed).
 #include "codegen.hpp"
 
 #include <cstdarg>
@@ -103,6 +105,47 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
 
 }  // namespace
 
+u32 lds_entries(const Mech &m, const GenOptions &o) {
+    // 160 KiB of LDS per workgroup (gfx950): 320 f64 or 640 f32 entries of 64 lanes.
+    const u32 most = o.precision == 2 ? 640u : 320u;
+    const u32 cap = m.nnz < most ? m.nnz : most;
+    return o.lds_slots < cap ? o.lds_slots : cap;
+}
+
+// Per-lane workspace layout (f64 slots): FP64 [LU entries lds.. | K stages |
+// f | k copy | src copy]; mixed [K stages | f | refinement rhs | FP32 LU
+// entries lds.. (two per slot) | k copy | src copy]. The last stage's vector
+// stays in registers.
+struct Layout {
+    u32 lds, koff, foff, rhsoff, lu32off, kcp, scp, slots;
+};
+
+static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
+    Layout L;
+    const bool mixed = o.precision == 2;
+    const u32 sweeps = mixed ? o.sweeps : 0;
+    const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;
+    L.lds = lds_entries(m, o);
+    L.koff = mixed ? 0 : m.nnz - L.lds;
+    L.foff = L.koff + stored_stages * m.n;
+    L.rhsoff = L.foff + m.n;
+    L.lu32off = L.rhsoff + (sweeps ? m.n : 0);
+    const u32 end = mixed ? L.lu32off + (m.nnz - L.lds + 1) / 2 : L.foff + m.n;
+    L.kcp = end;
+    L.scp = L.kcp + m.R;
+    L.slots = L.scp + m.n;
+    return L;
+}
+
+// Measured on the MI355X (48/150, 24/70 and the three-mechanism kernel):
+// two waves per CU with 80 KiB of LU each beat every grid with less LDS per
+// wave by 20-70 %; more than 80 KiB leaves one wave per CU and loses. The
+// refined mixed kernel is the exception (slower with LDS entries).
+u32 auto_lds(const GenOptions &o) {
+    if (o.precision == 2) return o.sweeps ? 0u : 320u;
+    return 160u;
+}
+
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
     u32 holder = 0;
@@ -133,19 +176,13 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const u32 n = m.n;
     const bool mixed = o.precision == 2;
     const u32 sweeps = mixed ? o.sweeps : 0;
-    const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;   // the last stage stays in registers
-    // Workspace slots per lane (f64 units): FP64 mode [LU | K stages | f];
-    // mixed mode [K stages | f | refinement rhs | FP32 LU (two per slot)].
-    const u32 koff = mixed ? 0 : m.nnz;
-    const u32 foff = koff + stored_stages * n;
-    const u32 rhsoff = foff + n;
-    const u32 lu32off = rhsoff + (sweeps ? n : 0);
-    const u32 end = mixed ? lu32off + (m.nnz + 1) / 2 : foff + n;
-    // The cell's rate constants and sources are copied into the lane's slots
-    // when the lane takes the cell, so every later read is a coalesced wave
-    // transaction whatever cells the lanes of a wave hold.
-    const u32 kcp = end, scp = kcp + m.R;
-    const u32 ws_slots = scp + n;
+    // LU entries below lds are in LDS. The cell's rate constants and sources
+    // are copied into the lane's slots (kcp, scp) when the lane takes the
+    // cell, so every later read is a coalesced wave transaction whatever cells
+    // the lanes of a wave hold.
+    const Layout Lo = layout(m, me, o);
+    const u32 lds = Lo.lds, koff = Lo.koff, foff = Lo.foff, rhsoff = Lo.rhsoff, lu32off = Lo.lu32off;
+    const u32 kcp = Lo.kcp, scp = Lo.scp, ws_slots = Lo.slots;
     w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
     w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u;\n", koff, foff, rhsoff,
@@ -153,7 +190,14 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
     if (mixed) w("    SimkRows32 w32;\n");
+    if (lds) {
+        const char *T = mixed ? "float" : "double", *A = mixed ? "simk_lds32" : "simk_lds";
+        w("    simk_u32 ln;\n");
+        w("    SIMK_DEV %s L(simk_u32 i) const { return %s[i * SIMK_WAVE + ln]; }\n", T, A);
+        w("    SIMK_DEV void LS(simk_u32 i, %s v) const { %s[i * SIMK_WAVE + ln] = v; }\n", T, A);
+    }
     w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
+    if (lds) w("        ln = lane;\n");
     w("        const simk_u32 stride = A.padded * 8u;\n");
     w("        kr.init(A.k, 0, stride, R * stride);\n");
     w("        sr.init(A.src, 0, stride, N * stride);\n");
@@ -261,9 +305,17 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // workspace after a fence, which bounds live ranges on dense mechanisms.
     const u32 window = o.reg_window == GenOptions::kDefaultWindow ? (mixed ? 8u : 16u) : o.reg_window;
     auto near = [&](u32 idx, u32 i) { return window == 0 || i - row_of[idx] <= window; };
-    auto fin = [&](u32 idx, u32 i) {
+    // LU entry idx: LDS entry idx, or workspace entry idx - lds (W: f64
+    // slots; W32: the FP32 LU region of mixed kernels).
+    auto lu_ld = [&](u32 idx) {
         char buf[24];
-        snprintf(buf, sizeof(buf), (regs && near(idx, i)) ? "a%u" : "W(%u)", idx);
+        snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : (mixed ? "W32(%u)" : "W(%u)"), idx < lds ? idx : idx - lds);
+        return std::string(buf);
+    };
+    auto lu_st = [&](u32 idx, const char *v) {
+        char buf[40];
+        snprintf(buf, sizeof(buf), idx < lds ? "LS(%u, %s%u);" : (mixed ? "W32S(%u, %s%u);" : "WS(%u, %s%u);"),
+                 idx < lds ? idx : idx - lds, v, idx);
         return std::string(buf);
     };
     {
@@ -337,7 +389,6 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             // FP32 (b = f32 entries, FP32 fma).
             const char *V = mixed ? "b" : "a";
             const char *T = mixed ? "float" : "double";
-            const char *LD = mixed ? "W32" : "W";
             if (mixed)
                 for (u32 idx : row) {
                     snprintf(buf, sizeof(buf), "float b%u = (float)a%u;", idx, idx);
@@ -348,8 +399,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 const bool kk_reg = regs && near(dv.kk, i);
                 Op op;
                 if (!kk_reg) {
-                    snprintf(buf, sizeof(buf), "%s(%u)", LD, dv.kk);
-                    op.load = buf;
+                    op.load = lu_ld(dv.kk);
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %%s; %s%u = l%u;", T, d, V, dv.ik, V, dv.ik, d);
                 } else {
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %s%u; %s%u = l%u;", T, d, V, dv.ik, V, dv.kk, V,
@@ -364,8 +414,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                     if (regs && near(sb.kj, i)) {
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %s%u, %s%u);", V, sb.ij, fn, d, V, sb.kj, V, sb.ij);
                     } else {
-                        snprintf(buf, sizeof(buf), "%s(%u)", LD, sb.kj);
-                        o2.load = buf;
+                        o2.load = lu_ld(sb.kj);
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %%s, %s%u);", V, sb.ij, fn, d, V, sb.ij);
                     }
                     o2.stmt = buf;
@@ -375,17 +424,11 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             if (mixed) {
                 snprintf(buf, sizeof(buf), "b%u = 1.0f / (b%u + gf);", m.diag[i], m.diag[i]);
                 stmt(buf);
-                for (u32 idx : row) {
-                    snprintf(buf, sizeof(buf), "W32S(%u, b%u);", idx, idx);
-                    stmt(buf);
-                }
+                for (u32 idx : row) stmt(lu_st(idx, "b"));
             } else {
                 snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
                 stmt(buf);
-                for (u32 idx : row) {
-                    snprintf(buf, sizeof(buf), "WS(%u, a%u);", idx, idx);
-                    stmt(buf);
-                }
+                for (u32 idx : row) stmt(lu_st(idx, "a"));
             }
         }
         barrier();
@@ -400,13 +443,11 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     auto solve_ops = [&](const char *vec, bool f32) {
         std::vector<Op> ops;
         char buf[160];
-        const char *LD = f32 ? "W32" : "W";
         const char *FN = f32 ? "__builtin_fmaf(-%s" : "simk_fnma(%s";
         for (u32 i = 0; i < n; ++i)
             for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
                 Op op;
-                snprintf(buf, sizeof(buf), "%s(%u)", LD, m.lower[e].index);
-                op.load = buf;
+                op.load = lu_ld(m.lower[e].index);
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, i);
                 std::string st = buf + f + ", ";
@@ -417,8 +458,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         for (u32 ii = n; ii-- > 0;) {
             for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
                 Op op;
-                snprintf(buf, sizeof(buf), "%s(%u)", LD, m.upper[e].index);
-                op.load = buf;
+                op.load = lu_ld(m.upper[e].index);
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, ii);
                 std::string st = buf + f + ", ";
@@ -427,8 +467,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 ops.push_back(op);
             }
             Op op;
-            snprintf(buf, sizeof(buf), "%s(%u)", LD, m.diag[ii]);
-            op.load = buf;
+            op.load = lu_ld(m.diag[ii]);
             snprintf(buf, sizeof(buf), "%s[%u] = %s[%u] * %%s;", vec, ii, vec, ii);
             op.stmt = buf;
             ops.push_back(op);
@@ -537,6 +576,8 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w("// generated by simstack_amd codegen: %u species, %u reactions, %u non-zeros\n", m.n, m.R, m.nnz);
     w.s += device_header;
     w.s += "\n";
+    if (const u32 lds = lds_entries(m, o))
+        w("__shared__ %s[%u * SIMK_WAVE];\n\n", mixed ? "float simk_lds32" : "double simk_lds", lds);
     g.ws_slots = emit_ops_struct(w, m, me, o, "GenOps");
     emit_method(w, me, o);
     if (o.waves_per_eu)
@@ -564,6 +605,11 @@ GenResult generate_multi(const std::vector<const Mech *> &ms, const KinMethod &m
     w("// generated by simstack_amd codegen: %u mechanisms in one persistent launch\n", (u32)ms.size());
     w.s += device_header;
     w.s += "\n";
+    // One LDS array serves every mechanism: a wave runs one mechanism at a time.
+    u32 lds = 0;
+    for (const Mech *m : ms)
+        if (lds_entries(*m, o) > lds) lds = lds_entries(*m, o);
+    if (lds) w("__shared__ double simk_lds[%u * SIMK_WAVE];\n\n", lds);
     for (size_t i = 0; i < ms.size(); ++i) {
         char sname[32];
         snprintf(sname, sizeof(sname), "GenOps%u", (u32)i);

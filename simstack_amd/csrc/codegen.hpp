@@ -22,7 +22,10 @@ struct GenOptions {
     u32 reg_window = kDefaultWindow;   // register mode: finished rows read from registers up to this many
                                        // rows back (0: all; kDefaultWindow: 16 in FP64, 8 in mixed precision)
     u32 reuse = 96;          // a rate constant loaded within this many statements is reused, not reloaded (0: never)
+    u32 lds_slots = kAutoLds;   // the first this many LU entries of each lane live in LDS, not the workspace
+                                // (kAutoLds: resolved by auto_lds before generation)
     static constexpr u32 kDefaultWindow = 0xFFFF;
+    static constexpr u32 kAutoLds = 0xFFFF;
 };
 
 struct GenResult {
@@ -31,6 +34,15 @@ struct GenResult {
     u32 ws_slots = 0;     // workspace slots per lane the kernel uses (the maximum over mechanisms)
     std::vector<u32> mech_slots;   // multi-mechanism kernels: slots of each mechanism
 };
+
+// LU entries per lane the kernel keeps in LDS (o.lds_slots clamped to the LU
+// and to 160 KiB per wave).
+u32 lds_entries(const Mech &m, const GenOptions &o);
+
+// The default LU entries in LDS: 80 KiB per wave (160 f64 or 320 f32
+// entries of 64 lanes), so two waves share a CU's 160 KiB; none for
+// mixed-precision kernels with refinement sweeps.
+u32 auto_lds(const GenOptions &o);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

@@ -381,9 +381,9 @@ bool jit_code(const GenResult &gr, std::vector<char> &code, bool &cached, std::s
 }
 
 void free_mech_resources() {
-    if (g.d_tables) hipFree(g.d_tables);
+    if (g.d_tables) (void)hipFree(g.d_tables);
     g.d_tables = nullptr;
-    if (g.d_wave) hipFree(g.d_wave);
+    if (g.d_wave) (void)hipFree(g.d_wave);
     g.d_wave = nullptr;
     if (g.jit.mod) (void)hipModuleUnload(g.jit.mod);
     g.jit = JitKernel();
@@ -480,7 +480,7 @@ hipError_t upload_tables(const Mech &m) {
     memcpy(blob.data(), &t, sizeof(t));
     e = hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(d);
+        (void)hipFree(d);
         return e;
     }
     g.d_tables = d;
@@ -577,7 +577,7 @@ hipError_t upload_wave_tables(const Mech &m) {
     memcpy(blob.data(), &t, sizeof(t));
     e = hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(d);
+        (void)hipFree(d);
         return e;
     }
     g.d_wave = d;
@@ -606,7 +606,7 @@ hipError_t ensure_ws(u64 bytes) {
     if (g.ws) {
         hipError_t e = hipStreamSynchronize(g.stream);
         if (e != hipSuccess) return e;
-        hipFree(g.ws);
+        (void)hipFree(g.ws);
         g.ws = nullptr;
         g.ws_bytes = 0;
     }
@@ -621,9 +621,15 @@ hipError_t ensure_ws(u64 bytes) {
 
 // Generates, compiles (or loads from the cache) and loads one kernel variant
 // of the current mechanism.
+// Options with the automatic LDS share resolved.
+GenOptions resolved(GenOptions o) {
+    if (o.lds_slots == GenOptions::kAutoLds) o.lds_slots = auto_lds(o);
+    return o;
+}
+
 bool load_generated(const GenOptions &opt, JitKernel &out) {
     auto t0 = std::chrono::steady_clock::now();
-    GenResult gr = generate(g.mech, g.method, opt, kDeviceKinSource);
+    GenResult gr = generate(g.mech, g.method, resolved(opt), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false, ok = false;
     if (jit_code(gr, code, cached, g.jit_log)) {
@@ -980,6 +986,8 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.precision = options->precision;
     o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
     o.reuse = options->reuse == 0 ? 96u : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
+    o.lds_slots = options->lds_slots == 0 ? GenOptions().lds_slots
+                                          : (options->lds_slots == 0xFFFFFFFFu ? 0u : options->lds_slots);
     return o;
 }
 
@@ -1145,7 +1153,7 @@ GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info) {
     e = hipGetDeviceProperties(&prop, (int)device_index);
     if (e != hipSuccess) return err_code(e);
     size_t free_b = 0, total_b = 0;
-    hipMemGetInfo(&free_b, &total_b);
+    (void)hipMemGetInfo(&free_b, &total_b);
     if (info) {
         snprintf(info->name, sizeof(info->name), "%s (%s)", prop.name, prop.gcnArchName);
         info->sm_major = (u32)prop.major;
@@ -1156,16 +1164,16 @@ GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info) {
         info->arena_bytes = arena_bytes;
     }
     if (g.ready) {   // re-init: keep the stream and events, replace the arena
-        hipStreamSynchronize(g.stream);
-        if (g.arena.base) hipFree(g.arena.base);
+        (void)hipStreamSynchronize(g.stream);
+        if (g.arena.base) (void)hipFree(g.arena.base);
         g.arena = Arena();
     } else {
         e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
         if (e != hipSuccess) return err_code(e);
-        hipEventCreate(&g.kin_start);
-        hipEventCreate(&g.kin_stop);
-        hipEventCreate(&g.async_begin);
-        hipEventCreate(&g.async_end);
+        (void)hipEventCreate(&g.kin_start);
+        (void)hipEventCreate(&g.kin_stop);
+        (void)hipEventCreate(&g.async_begin);
+        (void)hipEventCreate(&g.async_end);
     }
     void *base = nullptr;
     if (arena_bytes) {
@@ -1195,7 +1203,7 @@ GPU_API void *gpu_alloc(u64 size) {
 GPU_API u64 gpu_arena_used(void) { return g.arena.used; }
 
 GPU_API void gpu_arena_reset(void) {
-    if (g.ready) hipDeviceSynchronize();
+    if (g.ready) (void)hipDeviceSynchronize();
     g.arena.used = 0;
 }
 
@@ -1236,14 +1244,14 @@ GPU_API s32 gpu_kin_set_mech(const KinMech *mech, const KinMethod *method, const
 
 GPU_API s32 gpu_kin_advance(const GpuKinLaunch *L, f32 *kernel_ms) {
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
-    hipEventRecord(g.kin_start, g.stream);
+    (void)hipEventRecord(g.kin_start, g.stream);
     hipError_t e = launch(L, 1);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_stop, g.stream);
+    (void)hipEventRecord(g.kin_stop, g.stream);
     e = hipEventSynchronize(g.kin_stop);
     if (e != hipSuccess) return err_code(e);
     f32 ms = 0.0f;
-    hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    (void)hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
     g.last_ms = ms;
     if (kernel_ms) *kernel_ms = ms;
     return err_code(hipGetLastError());
@@ -1252,7 +1260,7 @@ GPU_API s32 gpu_kin_advance(const GpuKinLaunch *L, f32 *kernel_ms) {
 GPU_API s32 gpu_host_register(void *host, u64 size) { return (host && size) ? 0 : err_code(hipErrorInvalidValue); }
 
 GPU_API s32 gpu_host_unregister(void *host) {
-    hipStreamSynchronize(g.stream);
+    (void)hipStreamSynchronize(g.stream);
     return host ? 0 : err_code(hipErrorInvalidValue);
 }
 
@@ -1261,16 +1269,16 @@ GPU_API s32 gpu_kin_advance_async(const GpuKinLaunch *L, void *host_block, void 
     if (!g.have_mech || !L || !host_block || !device_block || mutable_rows > rows) return err_code(hipErrorInvalidValue);
     const u64 pitch = (u64)L->padded * sizeof(f64);
     const u64 width = (u64)(L->lane_count ? L->lane_count : L->padded) * sizeof(f64);
-    hipEventRecord(g.async_begin, g.stream);
+    (void)hipEventRecord(g.async_begin, g.stream);
     hipError_t e = hipMemcpy2DAsync(device_block, pitch, host_block, pitch, width, rows, hipMemcpyHostToDevice, g.stream);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_start, g.stream);
+    (void)hipEventRecord(g.kin_start, g.stream);
     e = launch(L, 1);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_stop, g.stream);
+    (void)hipEventRecord(g.kin_stop, g.stream);
     e = hipMemcpy2DAsync(host_block, pitch, device_block, pitch, width, mutable_rows, hipMemcpyDeviceToHost, g.stream);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.async_end, g.stream);
+    (void)hipEventRecord(g.async_end, g.stream);
     return err_code(hipGetLastError());
 }
 
@@ -1278,8 +1286,8 @@ GPU_API s32 gpu_kin_wait(f32 *kernel_ms, f32 *total_ms) {
     hipError_t e = hipEventSynchronize(g.async_end);
     if (e != hipSuccess) return err_code(e);
     f32 k = 0.0f, t = 0.0f;
-    hipEventElapsedTime(&k, g.kin_start, g.kin_stop);
-    hipEventElapsedTime(&t, g.async_begin, g.async_end);
+    (void)hipEventElapsedTime(&k, g.kin_start, g.kin_stop);
+    (void)hipEventElapsedTime(&t, g.async_begin, g.async_end);
     g.last_ms = k;
     if (kernel_ms) *kernel_ms = k;
     if (total_ms) *total_ms = t;
@@ -1295,14 +1303,14 @@ GPU_API s32 gpu_kin_set_mode(u32 mode) {
 
 GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *L, u32 repeats, f32 *kernel_ms) {
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
-    hipEventRecord(g.kin_start, g.stream);
+    (void)hipEventRecord(g.kin_start, g.stream);
     hipError_t e = launch(L, repeats);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_stop, g.stream);
+    (void)hipEventRecord(g.kin_stop, g.stream);
     e = hipEventSynchronize(g.kin_stop);
     if (e != hipSuccess) return err_code(e);
     f32 ms = 0.0f;
-    hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    (void)hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
     g.last_ms = ms;
     if (kernel_ms) *kernel_ms = ms;
     return err_code(hipGetLastError());
@@ -1333,6 +1341,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.unroll_stages = n.unroll_stages;
     g.gen.reg_window = n.reg_window;
     g.gen.reuse = n.reuse;
+    g.gen.lds_slots = n.lds_slots;
     if (g.tuned) g.user_gen = g.gen;
     g.gen.batch = n.batch;
     const u32 sw = n.sweeps;
@@ -1396,14 +1405,14 @@ hipError_t tune_time(TuneSample &S, u32 reps, f32 &best) {
     for (u32 r = 0; r < reps; ++r) {
         hipError_t e = hipMemcpyAsync(S.mem + S.half, S.mem, S.half, hipMemcpyDeviceToDevice, g.stream);
         if (e != hipSuccess) return e;
-        hipEventRecord(g.kin_start, g.stream);
+        (void)hipEventRecord(g.kin_start, g.stream);
         e = launch(&S.work, 1);
         if (e != hipSuccess) return e;
-        hipEventRecord(g.kin_stop, g.stream);
+        (void)hipEventRecord(g.kin_stop, g.stream);
         e = hipEventSynchronize(g.kin_stop);
         if (e != hipSuccess) return e;
         f32 ms = 0.0f;
-        hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+        (void)hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
         if (ms < best) best = ms;
     }
     return hipSuccess;
@@ -1417,7 +1426,7 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes) {
 bool same_gen(const GenOptions &a, const GenOptions &b) {
     return a.fence_rows == b.fence_rows && a.waves_per_eu == b.waves_per_eu && a.unroll_stages == b.unroll_stages &&
            a.batch == b.batch && a.precision == b.precision && a.sweeps == b.sweeps &&
-           a.reg_window == b.reg_window && a.reuse == b.reuse;
+           a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots;
 }
 
 GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
@@ -1497,6 +1506,15 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.reuse = r;
             e = consider(0, best_grid, o);
         }
+    // LU entries in LDS: none, and shares for 4, 3 and 2 waves per CU
+    const u32 lds_now = lds_entries(g.mech, resolved(best_gen));
+    const u32 ldss[] = {0, 80, 128, 160};
+    for (u32 l : ldss)
+        if (e == hipSuccess && lds_entries(g.mech, [&] { GenOptions t = best_gen; t.lds_slots = l; return t; }()) != lds_now) {
+            GenOptions o = best_gen;
+            o.lds_slots = l;
+            e = consider(0, best_grid, o);
+        }
     // 3. one wave per cell, timed on at most 16 resident rounds of its grid
     if (e == hipSuccess) {
         int blocks = 1;
@@ -1537,6 +1555,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
     out->reuse = best_gen.reuse;
+    out->lds_slots = lds_entries(g.mech, resolved(best_gen));
     out->best_ms = best;
     out->sample_cells = cells;
     return 0;
@@ -1570,7 +1589,7 @@ GPU_API s32 simk_jit_precompile(const void *mech, const SimkCaps *caps, const Ki
     GenOptions o = gen_options(options);
     KinMethod me = *method;
     me.name = nullptr;
-    GenResult gr = generate(m, me, o, kDeviceKinSource);
+    GenResult gr = generate(m, me, resolved(o), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
     std::string text;
@@ -1593,7 +1612,7 @@ GPU_API s32 simk_jit_precompile_multi(u32 count, const void *const *mechs, const
     o.precision = 0;
     KinMethod me = *method;
     me.name = nullptr;
-    GenResult gr = generate_multi(ptrs, me, o, kDeviceKinSource);
+    GenResult gr = generate_multi(ptrs, me, resolved(o), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
     std::string text;
@@ -1609,14 +1628,14 @@ GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms) {
     u64 grid = (n + block - 1) / block;
     if (grid > 65536) grid = 65536;
     if (grid == 0) return 0;
-    hipEventRecord(g.kin_start, g.stream);
+    (void)hipEventRecord(g.kin_start, g.stream);
     hipLaunchKernelGGL(simk_copy_kernel, dim3((u32)grid), dim3(block), 0, g.stream, (double2 *)dst,
                        (const double2 *)src, n);
-    hipEventRecord(g.kin_stop, g.stream);
+    (void)hipEventRecord(g.kin_stop, g.stream);
     hipError_t e = hipEventSynchronize(g.kin_stop);
     if (e != hipSuccess) return err_code(e);
     f32 t = 0.0f;
-    hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
+    (void)hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
     if (ms) *ms = t;
     return err_code(hipGetLastError());
 }
@@ -1658,7 +1677,7 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
     for (const Mech &m : g.multi_mechs) ptrs.push_back(&m);
     GenOptions o = g.gen;
     o.precision = 0;
-    GenResult gr = generate_multi(ptrs, g.method, o, kDeviceKinSource);
+    GenResult gr = generate_multi(ptrs, g.method, resolved(o), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
     if (!jit_code(gr, code, cached, g.jit_log)) return err_code(hipErrorInvalidImage);
@@ -1688,14 +1707,14 @@ GPU_API s32 simk_kin_advance_multi(u32 count, const GpuKinLaunch *launches, u32 
     MultiPlan P;
     hipError_t e = prepare_multi(count, launches, repeats, P);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_start, g.stream);
+    (void)hipEventRecord(g.kin_start, g.stream);
     e = enqueue_multi(P);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_stop, g.stream);
+    (void)hipEventRecord(g.kin_stop, g.stream);
     e = hipEventSynchronize(g.kin_stop);
     if (e != hipSuccess) return err_code(e);
     f32 ms = 0.0f;
-    hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    (void)hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
     g.last_ms = ms;
     if (kernel_ms) *kernel_ms = ms;
     return err_code(hipGetLastError());
@@ -1733,14 +1752,14 @@ GPU_API s32 simk_graph_capture_multi(u32 count, const GpuKinLaunch *launches, u3
 
 GPU_API s32 simk_graph_replay(u32 graph_id, f32 *ms) {
     if (graph_id >= g.graphs.size() || !g.graphs[graph_id].exec) return err_code(hipErrorInvalidValue);
-    hipEventRecord(g.kin_start, g.stream);
+    (void)hipEventRecord(g.kin_start, g.stream);
     hipError_t e = hipGraphLaunch(g.graphs[graph_id].exec, g.stream);
     if (e != hipSuccess) return err_code(e);
-    hipEventRecord(g.kin_stop, g.stream);
+    (void)hipEventRecord(g.kin_stop, g.stream);
     e = hipEventSynchronize(g.kin_stop);
     if (e != hipSuccess) return err_code(e);
     f32 t = 0.0f;
-    hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
+    (void)hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
     if (ms) *ms = t;
     return err_code(hipGetLastError());
 }

@@ -51,6 +51,28 @@ def test_three_mechanisms_one_launch(gm):
         assert want.steps[:want.count].max() >= 5 * want.steps[:want.count].min()
 
 
+def test_multi_with_lds_lu_entries(gm):
+    """The multi-mechanism kernel with LU entries in LDS (one array shared by
+    the mechanisms a wave runs in turn): bit-identical to the oracle."""
+    try:
+        gm.set_gen_options(lds_slots=96)
+        blocks = _blocks((150, 260, 190), first=500)
+        meth, ctl = _set(gm, blocks)
+        work = [z.copy() for _, z in blocks]
+        dzs = [gpu.DeviceZones(gm, z) for z in work]
+        for d in dzs:
+            d.upload()
+        gm.advance_multi(dzs, 0.25, repeats=2)
+        for d in dzs:
+            d.download(mutable_only=False)
+        for (mech, z), got in zip(blocks, work):
+            want = z.copy()
+            oracle_run(mech, meth, ctl, want, [0.25, 0.25])
+            assert compare_bits(got, want) == ZERO
+    finally:
+        gm.set_gen_options()
+
+
 def test_multi_with_refill_repeats_and_small_grid(gm):
     """Three resident waves for 700 cells (every wave crosses mechanisms and
     refills many times) with two fused operator-split repeats."""

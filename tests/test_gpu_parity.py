@@ -58,6 +58,37 @@ def test_c2_sampled_bit_exact(gm):
     assert compare_bits(a_sub, b) == ZERO
 
 
+@pytest.mark.parametrize("lds", [64, 0xFFFFFFFE], ids=["lds64", "lds_max"])
+def test_lds_lu_entries_keep_bits(gm, lds):
+    """LU entries held in LDS instead of the workspace (the generator's
+    lds_slots, at most 320 entries = 160 KiB per wave) change where values
+    live, not one operation: the advance stays bit-identical to the oracle,
+    with pulses, for the 24/70 mechanism (lds_max: all 216 entries in LDS) and
+    a 48/150 sample."""
+    try:
+        gm.set_gen_options(lds_slots=lds)
+        sp, mech, z = make_problem("c3c", 256)
+        add_pulses(z, seed=7)
+        meth, ctl = problem.method(2), problem.control(rtol=1e-5, atol=1e-10)
+        a, b = z.copy(), z.copy()
+        gpu_run(gm, mech, meth, ctl, a, [0.1, 0.1], kernel=1)
+        oracle_run(mech, meth, ctl, b, [0.1, 0.1])
+        assert gm.kin_info().kernel == 1
+        assert compare_bits(a, b) == ZERO
+        sp, mech, z = make_problem("c2", 1024)
+        meth, ctl = problem.method(2), problem.control()
+        a = z.copy()
+        gpu_run(gm, mech, meth, ctl, a, [0.5], kernel=1)
+        info = gm.kin_info()
+        assert info.ws_slots == 3 * 48 + 48 + 150 + 48 + 439 - min(lds, 320)
+        lanes = np.random.default_rng(5).choice(1024, 48, replace=False)
+        b = z.subset(lanes)
+        oracle_run(mech, meth, ctl, b, [0.5])
+        assert compare_bits(a.subset(lanes), b) == ZERO
+    finally:
+        gm.set_gen_options()
+
+
 def test_generated_matches_table(gm):
     """Generated straight-line kernel == table walk (test_kinetics.cpp:254-274)."""
     sp, mech, z = make_problem("c3c", 512)

@@ -294,8 +294,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f32 LU / f64 state",
         "data": "synthetic (seeded mechanism and cells)",
-        "config": {"workload": "48 species / 150 reactions, 65,536 cells, 10 x 1 s operator-split, Rodas3, "
-                               "rtol 1e-6 atol 1e-12", "cells": args.cells, "species": mech.n,
+        "config": {"workload": f"48 species / 150 reactions, {args.cells:,} cells, {args.steps} x 1 s operator-split, "
+                               "Rodas3, rtol 1e-6 atol 1e-12", "cells": args.cells, "species": mech.n,
                    "reactions": mech.R, "nnz": mech.nnz, "lu_div": mech.lu_div, "lu_sub": mech.lu_sub,
                    "kernel": r["kernel"], "waves": r["waves"], "launches": len(kernel_ms), "autotune": r["tuned"],
                    "parallelism": "1 GPU, one cell per lane, persistent lane-refilling kernel"},

@@ -41,10 +41,10 @@ def main():
     ap.add_argument("--name", required=True)
     ap.add_argument("--cmd", required=True)
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-fetch", required=True)
-    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--l2")
     ap.add_argument("--sq")
@@ -56,6 +56,16 @@ def main():
 
     dur = kernel_durations(a.trace, a.kernel)
     avg_ns = sum(dur) / len(dur)
+    if not a.fetch:   # kernel trace only: every kernel of the run
+        rows = _q(a.trace, "select name, count(*), avg(duration), sum(duration) from kernels group by name "
+                           "order by sum(duration) desc")
+        lines = [f"# {a.name}", "", "Command:", "", "```",
+                 f"rocprofv3 --kernel-trace --stats -d <dir> -o kt -- {a.cmd}", "```", "",
+                 "| kernel | dispatches | average ms | total ms |", "|---|---|---|---|"]
+        lines += [f"| `{n[:60]}` | {c} | {av / 1e6:.3f} | {tot / 1e6:.1f} |" for n, c, av, tot in rows[:12]]
+        open(a.out, "w").write("\n".join(lines) + "\n")
+        print("\n".join(lines))
+        return
     cf = counter(a.copy_fetch, "simk_copy_kernel", "FETCH_SIZE")
     cw = counter(a.copy_write, "simk_copy_kernel", "WRITE_SIZE")
     f_scale = a.copy_bytes / (min(cf) * 1024.0)
@@ -77,20 +87,25 @@ def main():
              f"| HBM bytes read per dispatch | {rd:.4e} |",
              f"| HBM bytes written per dispatch | {wr:.4e} |",
              f"| HBM traffic per dispatch | {rd + wr:.4e} |",
-             f"| HBM bandwidth (traffic / avg duration) | {(rd + wr) / avg_ns:.1f} GB/s |"]
+             f"| HBM bandwidth (traffic / avg duration) | {(rd + wr) / avg_ns:.1f} GB/s |", "",
+             "FETCH_SIZE and WRITE_SIZE count the L2's traffic to the data fabric, so the "
+             "\"HBM\" rows include what the 256 MB Infinity Cache serves.", ""]
+    extra = []
     if a.algo_flops_per_launch:
-        lines.append(f"| algorithmic FP64 flops per dispatch | {a.algo_flops_per_launch:.4e} |")
-        lines.append(f"| achieved FP64 | {a.algo_flops_per_launch / avg_ns / 1e3:.3f} TFLOP/s |")
+        extra.append(f"| algorithmic FP64 flops per dispatch | {a.algo_flops_per_launch:.4e} |")
+        extra.append(f"| achieved FP64 | {a.algo_flops_per_launch / avg_ns / 1e3:.3f} TFLOP/s |")
     if a.l2:
         hit = sum(counter(a.l2, a.kernel, "TCC_HIT_sum"))
         miss = sum(counter(a.l2, a.kernel, "TCC_MISS_sum"))
-        lines.append(f"| L2 hit rate | {hit / max(hit + miss, 1):.3f} |")
+        extra.append(f"| L2 hit rate | {hit / max(hit + miss, 1):.3f} |")
     if a.sq:
         for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_WAVES",
                   "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY"):
             v = counter(a.sq, a.kernel, c)
             if v:
-                lines.append(f"| {c} (sum per dispatch) | {sum(v) / len(v):.4e} |")
+                extra.append(f"| {c} (sum per dispatch) | {sum(v) / len(v):.4e} |")
+    if extra:
+        lines += ["| quantity | value |", "|---|---|"] + extra
     open(a.out, "w").write("\n".join(lines) + "\n")
     if a.traffic_json:
         json.dump({"kernel": a.kernel, "bytes_per_launch": rd + wr, "bytes_read": rd, "bytes_written": wr,

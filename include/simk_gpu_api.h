@@ -89,8 +89,8 @@ GPU_API s32 simk_kin_set_grid(u32 waves);
  * do not depend on it. The array must stay valid while advances run. */
 GPU_API s32 simk_kin_set_order(const u32 *order_device);
 
-/* Kernel used by the advances: 0 auto (generated when it compiled, else
- * table), 1 generated straight-line, 2 table walk. */
+/* Kernel used by the FP64 and f32 delta advances: 0 auto (generated when it
+ * compiled, else table), 1 generated straight-line, 2 table walk. */
 GPU_API s32 simk_kin_set_kernel(u32 kind);
 
 /* Generator knobs (speed only; results unchanged): see SimkGenOptions. */
@@ -100,7 +100,7 @@ typedef struct SimkGenOptions {
     u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
     u32 reg_window;      /* rows back a finished row stays in registers (0: 16 in FP64, 8 mixed; ~0u: all) */
     u32 batch;           /* loads issued this many statements ahead (0: 16) */
-    u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 2 mixed-precision kernel */
+    u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 96; ~0u: never) */
     u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default, 80 KiB per wave =
@@ -140,7 +140,8 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
 
 /* Statistics of the current mechanism and the last advance. */
 typedef struct SimkKinInfo {
-    u32 kernel;          /* last launch: 1 generated FP64, 2 table FP64, 3 mixed precision, 4 multi-mechanism, 5 f32 delta */
+    u32 kernel;          /* last launch: 1 generated FP64, 2 table FP64, 3 mixed precision, 4 multi-mechanism,
+                            5 f32 delta (table walk), 6 / 7 one wave per cell (FP64 / f32 delta), 8 generated f32 delta */
     u32 ws_slots;        /* workspace f64 slots per lane */
     u32 waves;           /* persistent waves of the last launch */
     u32 pad_;

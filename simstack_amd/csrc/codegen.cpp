@@ -60,7 +60,8 @@ struct Op {
 // [b, b + batch) run. Every loaded value is its own SSA name, except that an
 // immutable load issued at most `reuse` ops earlier (in any segment) is not
 // repeated: the op uses the earlier value.
-void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent, u32 &uid, u32 reuse = 0) {
+void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent, u32 &uid, u32 reuse = 0,
+              const char *type = "double") {
     if (batch == 0) batch = 1;
     std::map<std::string, std::pair<std::string, size_t>> recent;   // load -> (value, op index)
     size_t seg = 0;
@@ -85,7 +86,7 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
                 char buf[32];
                 snprintf(buf, sizeof(buf), "q%u", uid++);
                 name[issued - seg] = buf;
-                w("%sconst double %s = %s;\n", indent, buf, op.load.c_str());
+                w("%sconst %s %s = %s;\n", indent, type, buf, op.load.c_str());
                 if (reuse && op.immutable) recent[op.load] = {buf, issued};
             }
         };
@@ -107,7 +108,7 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
 
 u32 lds_entries(const Mech &m, const GenOptions &o) {
     // 160 KiB of LDS per workgroup (gfx950): 320 f64 or 640 f32 entries of 64 lanes.
-    const u32 most = o.precision == 2 ? 640u : 320u;
+    const u32 most = o.precision != 0 ? 640u : 320u;
     const u32 cap = m.nnz < most ? m.nnz : most;
     return o.lds_slots < cap ? o.lds_slots : cap;
 }
@@ -142,6 +143,7 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
 // wave by 20-70 %; more than 80 KiB leaves one wave per CU and loses. The
 // refined mixed kernel is the exception (slower with LDS entries).
 u32 auto_lds(const GenOptions &o) {
+    if (o.precision == 1) return 320u;
     if (o.precision == 2) return o.sweeps ? 0u : 320u;
     return 160u;
 }
@@ -537,6 +539,253 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     return ws_slots;
 }
 
+// Emits the f32 delta-lane policy `sname` (simk_d32g_body): the arithmetic of
+// simk_d32_fun / simk_d32_jac_factor / simk_d32_solve as straight-line f32
+// code. Workspace (f32 slots per lane): [LU entries lds.. | K stages (all
+// but the last) | k | src | pulse], the first lds LU entries in LDS. Returns
+// the workspace size in f64 slots.
+static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const GenOptions &o, const char *sname) {
+    const u32 n = m.n;
+    const u32 lds = lds_entries(m, o);
+    const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;
+    const u32 koff = m.nnz - lds, kcp = koff + stored_stages * n, scp = kcp + m.R, pcp = scp + n;
+    const u32 slots32 = pcp + n;
+    w("struct %s {\n", sname);
+    w("    static constexpr simk_u32 N = %u, R = %u, STAGES = %u, KOFF = %u;\n", n, m.R, me.stages, koff);
+    w("    SimkRows kr, sr, pr;\n    SimkRows32 ws;\n");
+    if (lds) {
+        w("    simk_u32 ln;\n");
+        w("    SIMK_DEV float L(simk_u32 i) const { return simk_lds32[i * SIMK_WAVE + ln]; }\n");
+        w("    SIMK_DEV void LS(simk_u32 i, float v) const { simk_lds32[i * SIMK_WAVE + ln] = v; }\n");
+    }
+    w("    SIMK_DEV float W32(simk_u32 i) const { return ws.ld(i); }\n");
+    w("    SIMK_DEV void W32S(simk_u32 i, float v) const { ws.st(i, v); }\n");
+    w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
+    w("        const simk_u32 stride = A.padded * 8u;\n");
+    w("        kr.init(A.k, 0, stride, R * stride);\n");
+    w("        sr.init(A.src, 0, stride, N * stride);\n");
+    w("        pr.init(A.pulse, 0, stride, N * stride);\n");
+    w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    if (lds) w("        ln = lane;\n");
+    w("    }\n");
+    // lane_load_f64 (base_lane.h:101): inputs rounded to f32 once, when the lane takes the cell
+    w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
+    w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
+    w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) W32S(%u + r, (float)kr.ld(r));\n", kcp);
+    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) W32S(%u + s, (float)sr.ld(s));\n", scp);
+    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) W32S(%u + s, (float)pr.ld(s));\n", pcp);
+    w("    }\n");
+    w("    SIMK_DEV float K(simk_u32 r) const { return W32(%u + r); }\n", kcp);
+    w("    SIMK_DEV float S(simk_u32 s) const { return W32(%u + s); }\n", scp);
+    w("    SIMK_DEV float pulse(simk_u32 s) const { return W32(%u + s); }\n", pcp);
+    w("    SIMK_DEV void kstore(simk_u32 st, const float (&x)[N]) const {\n");
+    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) W32S(KOFF + st * N + s, x[s]);\n    }\n");
+    {
+        std::vector<Op> ops;
+        char buf[96];
+        for (u32 s2 = 0; s2 < n; ++s2) {
+            Op op;
+            snprintf(buf, sizeof(buf), "W32(KOFF + j * N + %u)", s2);
+            op.load = buf;
+            snprintf(buf, sizeof(buf), "v[%u] = __builtin_fmaf(c, %%s, v[%u]);", s2, s2);
+            op.stmt = buf;
+            ops.push_back(op);
+        }
+        w("    SIMK_DEV void kaxpy(simk_u32 j, float c, float (&v)[N]) const {\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid, 0, "float");
+        w("    }\n");
+    }
+    // fun: f = src, then per reaction rate = k * pt_a [* pt_b], f_s = fma(coef, rate, f_s)
+    {
+        std::vector<Op> ops;
+        char buf[96];
+        for (u32 s2 = 0; s2 < n; ++s2) {
+            Op op;
+            snprintf(buf, sizeof(buf), "S(%u)", s2);
+            op.load = buf;
+            snprintf(buf, sizeof(buf), "f[%u] = %%s;", s2);
+            op.stmt = buf;
+            ops.push_back(op);
+        }
+        for (u32 r = 0; r < m.R; ++r) {
+            const Reaction &x = m.rx[r];
+            Op op;
+            snprintf(buf, sizeof(buf), "K(%u)", r);
+            op.load = buf;
+            op.immutable = true;
+            std::string st = "{ const float rate = %s";
+            for (u32 p = 0; p < x.reactant_count; ++p) {
+                snprintf(buf, sizeof(buf), " * pt[%u]", x.reactant[p]);
+                st += buf;
+            }
+            st += ";";
+            for (u32 e = 0; e < x.net_count; ++e) {
+                snprintf(buf, sizeof(buf), " f[%u] = __builtin_fmaf(%d.0f, rate, f[%u]);", x.net_species[e],
+                         (int)x.net_coef[e], x.net_species[e]);
+                st += buf;
+            }
+            st += " }";
+            op.stmt = st;
+            ops.push_back(op);
+        }
+        w("    SIMK_DEV void fun(const float (&pt)[N], float (&f)[N]) const {\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid, o.reuse, "float");
+        w("    }\n");
+    }
+    // jac_factor: per row, -J accumulated in (reaction, reactant, entry) order
+    // from +0, the eliminations against finished rows (registers within the
+    // window, else reloads), the reciprocal pivot, then the row is stored.
+    std::vector<u32> row_of(m.nnz, 0);
+    for (u32 i = 0; i < n; ++i) {
+        row_of[m.diag[i]] = i;
+        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) row_of[m.lower[e].index] = i;
+        for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e) row_of[m.upper[e].index] = i;
+    }
+    const u32 window = o.reg_window == GenOptions::kDefaultWindow ? 8u : o.reg_window;
+    auto near = [&](u32 idx, u32 i) { return window == 0 || i - row_of[idx] <= window; };
+    auto lu_ld = [&](u32 idx) {
+        char buf[24];
+        snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : "W32(%u)", idx < lds ? idx : idx - lds);
+        return std::string(buf);
+    };
+    {
+        std::vector<Op> ops;
+        auto stmt = [&](const std::string &t) {
+            Op op;
+            op.stmt = t;
+            ops.push_back(op);
+        };
+        auto barrier = [&]() {
+            Op op;
+            op.stmt = "SIMK_MEMORY_FENCE();";
+            op.barrier = true;
+            ops.push_back(op);
+        };
+        char buf[160];
+        for (u32 i = 0; i < n; ++i) {
+            std::vector<u32> row;
+            for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) row.push_back(m.lower[e].index);
+            row.push_back(m.diag[i]);
+            for (u32 e = m.upper_begin[i]; e < m.upper_begin[i + 1]; ++e) row.push_back(m.upper[e].index);
+            bool far = false;
+            for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
+                if (!near(m.lu_div[d].kk, i)) far = true;
+                for (u32 s2 = m.lu_div[d].sub_begin; s2 < m.lu_div[d].sub_end; ++s2)
+                    if (!near(m.lu_sub[s2].kj, i)) far = true;
+            }
+            if (far) barrier();
+            snprintf(buf, sizeof(buf), "// row %u", i);
+            stmt(buf);
+            for (u32 idx : row) {
+                snprintf(buf, sizeof(buf), "float a%u = 0.0f;", idx);
+                stmt(buf);
+            }
+            for (u32 r = 0; r < m.R; ++r) {
+                const Reaction &x = m.rx[r];
+                bool any = false;
+                for (u32 e = 0; e < x.net_count; ++e)
+                    if (x.net_species[e] == i) any = true;
+                if (!any || x.reactant_count == 0) continue;
+                Op op;
+                snprintf(buf, sizeof(buf), "K(%u)", r);
+                op.load = buf;
+                op.immutable = true;
+                std::string st = "{ const float k = %s;";
+                for (u32 p = 0; p < x.reactant_count; ++p) {
+                    st += " { const float d = k";
+                    for (u32 q = 0; q < x.reactant_count; ++q)
+                        if (q != p) {
+                            snprintf(buf, sizeof(buf), " * pt[%u]", x.reactant[q]);
+                            st += buf;
+                        }
+                    st += ";";
+                    for (u32 e = 0; e < x.net_count; ++e) {
+                        if (x.net_species[e] != i) continue;
+                        const u32 idx = x.jac_index[p][e];
+                        snprintf(buf, sizeof(buf), " a%u = __builtin_fmaf(%d.0f, d, a%u);", idx, -(int)x.net_coef[e],
+                                 idx);
+                        st += buf;
+                    }
+                    st += " }";
+                }
+                st += " }";
+                op.stmt = st;
+                ops.push_back(op);
+            }
+            for (u32 d = m.div_begin[i]; d < m.div_begin[i + 1]; ++d) {
+                const KinLuDiv &dv = m.lu_div[d];
+                Op op;
+                if (!near(dv.kk, i)) {
+                    op.load = lu_ld(dv.kk);
+                    snprintf(buf, sizeof(buf), "const float l%u = a%u * %%s; a%u = l%u;", d, dv.ik, dv.ik, d);
+                } else {
+                    snprintf(buf, sizeof(buf), "const float l%u = a%u * a%u; a%u = l%u;", d, dv.ik, dv.kk, dv.ik, d);
+                }
+                op.stmt = buf;
+                ops.push_back(op);
+                for (u32 su = dv.sub_begin; su < dv.sub_end; ++su) {
+                    const KinLuSub &sb = m.lu_sub[su];
+                    Op o2;
+                    if (near(sb.kj, i)) {
+                        snprintf(buf, sizeof(buf), "a%u = __builtin_fmaf(-l%u, a%u, a%u);", sb.ij, d, sb.kj, sb.ij);
+                    } else {
+                        o2.load = lu_ld(sb.kj);
+                        snprintf(buf, sizeof(buf), "a%u = __builtin_fmaf(-l%u, %%s, a%u);", sb.ij, d, sb.ij);
+                    }
+                    o2.stmt = buf;
+                    ops.push_back(o2);
+                }
+            }
+            snprintf(buf, sizeof(buf), "a%u = 1.0f / (a%u + ghinv);", m.diag[i], m.diag[i]);
+            stmt(buf);
+            for (u32 idx : row) {
+                if (idx < lds) snprintf(buf, sizeof(buf), "LS(%u, a%u);", idx, idx);
+                else snprintf(buf, sizeof(buf), "W32S(%u, a%u);", idx - lds, idx);
+                stmt(buf);
+            }
+        }
+        barrier();
+        w("    SIMK_DEV void jac_factor(const float (&pt)[N], const float ghinv) const {\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid, o.reuse, "float");
+        w("    }\n");
+    }
+    // solve: forward over lower, backward over upper and the reciprocal pivot
+    {
+        std::vector<Op> ops;
+        char buf[160];
+        for (u32 i = 0; i < n; ++i)
+            for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
+                Op op;
+                op.load = lu_ld(m.lower[e].index);
+                snprintf(buf, sizeof(buf), "x[%u] = __builtin_fmaf(-%%s, x[%u], x[%u]);", i, m.lower[e].col, i);
+                op.stmt = buf;
+                ops.push_back(op);
+            }
+        for (u32 ii = n; ii-- > 0;) {
+            for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
+                Op op;
+                op.load = lu_ld(m.upper[e].index);
+                snprintf(buf, sizeof(buf), "x[%u] = __builtin_fmaf(-%%s, x[%u], x[%u]);", ii, m.upper[e].col, ii);
+                op.stmt = buf;
+                ops.push_back(op);
+            }
+            Op op;
+            op.load = lu_ld(m.diag[ii]);
+            snprintf(buf, sizeof(buf), "x[%u] = x[%u] * %%s;", ii, ii);
+            op.stmt = buf;
+            ops.push_back(op);
+        }
+        w("    SIMK_DEV void solve(float (&x)[N]) const {\n        SIMK_MEMORY_FENCE();\n");
+        u32 uid = 0;
+        emit_ops(w, ops, o.batch, "        ", uid, 0, "float");
+        w("    }\n};\n\n");
+    }
+    return (slots32 + 1) / 2;
+}
+
 // Emits the method policy with compile-time coefficients.
 static void emit_method(Out &w, const KinMethod &me, const GenOptions &o) {
     u32 eval[4], save_f[4];
@@ -570,6 +819,7 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     char name[64];
     const bool mixed = o.precision == 2;
     if (mixed) snprintf(name, sizeof(name), "simk_mix%u_%016llx", o.sweeps, (unsigned long long)sig);
+    else if (o.precision == 1) snprintf(name, sizeof(name), "simk_d32g_%016llx", (unsigned long long)sig);
     else snprintf(name, sizeof(name), "simk_gen_%016llx", (unsigned long long)sig);
     g.kernel = name;
     Out w;
@@ -577,7 +827,15 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w.s += device_header;
     w.s += "\n";
     if (const u32 lds = lds_entries(m, o))
-        w("__shared__ %s[%u * SIMK_WAVE];\n\n", mixed ? "float simk_lds32" : "double simk_lds", lds);
+        w("__shared__ %s[%u * SIMK_WAVE];\n\n", o.precision != 0 ? "float simk_lds32" : "double simk_lds", lds);
+    if (o.precision == 1) {
+        // f32 delta lanes: the method and control come in the argument block
+        g.ws_slots = emit_d32_struct(w, m, me, o, "GenOps32");
+        w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) %s(SimkD32Args DA) {\n", name);
+        w("    simk_d32g_body<GenOps32>(DA.A, DA.C);\n}\n");
+        g.source = std::move(w.s);
+        return g;
+    }
     g.ws_slots = emit_ops_struct(w, m, me, o, "GenOps");
     emit_method(w, me, o);
     if (o.waves_per_eu)

@@ -17,7 +17,7 @@ struct GenOptions {
     u32 waves_per_eu = 0;    // amdgpu_waves_per_eu lower bound (0: compiler default)
     u32 unroll_stages = 0;   // unroll the stage loop (bigger code, constant stage offsets)
     u32 batch = 16;          // loads issued this many statements ahead (software pipelining)
-    u32 precision = 0;       // 0: FP64 throughout; 2: FP32 Jacobian/LU + FP64 residual refinement
+    u32 precision = 0;       // 0: FP64 throughout; 1: f32 delta lanes; 2: FP32 Jacobian/LU + FP64 residual refinement
     u32 sweeps = 1;          // precision 2: refinement sweeps per stage solve
     u32 reg_window = kDefaultWindow;   // register mode: finished rows read from registers up to this many
                                        // rows back (0: all; kDefaultWindow: 16 in FP64, 8 in mixed precision)

@@ -378,3 +378,259 @@ SIMK_DEV void simk_d32_body(const SimkKinArgs &A, const SimkD32Consts &C) {
         }
     }
 }
+
+// ---- generated f32 delta kernels -------------------------------------------
+//
+// The same lane formulation with a generated policy P (straight-line f32 fun,
+// -J/LU and solves for one mechanism): base and the increment y live in
+// registers, the stage vectors K in f32 workspace slots, the LU in LDS and f32
+// slots. Every operation is the one simk_d32_attempt performs, in the same
+// order, so both kernels (and the oracle) give the same bits. P provides
+// N, fun(pt, f), jac_factor(pt, ghinv), solve(x), kstore(st, x),
+// kaxpy(j, c, v) (v = fma(c, K_j, v) per element), pulse(s), init and set_cell.
+
+// The argument block of a generated f32 delta kernel.
+struct SimkD32Args {
+    SimkKinArgs A;
+    SimkD32Consts C;
+};
+
+// One step attempt (kin_solver.h:216-311 in f32 delta lanes).
+template <class P>
+SIMK_DEV void simk_d32g_attempt(const P &p, const SimkD32Consts &C, const float (&base)[P::N], float (&y)[P::N],
+                                float &t, const float t_end, float &h, float &steps, float &rejects,
+                                bool &reject_last, bool &reject_more) {
+    constexpr simk_u32 N = P::N;
+    const float one = 1.0f;
+    const float remaining = t_end - t;
+    float h_cap = remaining;
+    if (C.h_max_pos) h_cap = simk_fminf_(h_cap, C.h_max);
+    const float h_wanted = simk_fmaxf_(h, C.h_min);
+    const bool last = h_wanted >= remaining;
+    const float hs = last ? remaining : simk_fminf_(h_wanted, h_cap);
+
+    float f[N], x[N];
+    {
+        float ev[N];
+#pragma unroll
+        for (simk_u32 s = 0; s < N; ++s) ev[s] = base[s] + y[s];
+        p.fun(ev, f);
+        const float ghinv = one / (hs * C.gamma0);
+        p.jac_factor(ev, ghinv);
+    }
+    const float hinv = one / hs;
+
+#pragma unroll 1
+    for (simk_u32 st = 0; st < C.stages; ++st) {
+        const simk_u32 packed = st * (st - 1) / 2;
+        if (st > 0 && ((C.new_f >> st) & 1u)) {
+            float ev[N];
+#pragma unroll
+            for (simk_u32 s = 0; s < N; ++s) ev[s] = y[s];
+            for (simk_u32 j = 0; j < st; ++j)
+                if ((C.a_nz >> (packed + j)) & 1u) p.kaxpy(j, C.a[packed + j], ev);
+#pragma unroll
+            for (simk_u32 s = 0; s < N; ++s) ev[s] = base[s] + ev[s];
+            p.fun(ev, f);
+        }
+#pragma unroll
+        for (simk_u32 s = 0; s < N; ++s) x[s] = f[s];
+        for (simk_u32 j = 0; j < st; ++j)
+            if ((C.c_nz >> (packed + j)) & 1u) p.kaxpy(j, C.c[packed + j] * hinv, x);
+        p.solve(x);
+        if (st + 1 < C.stages) p.kstore(st, x);
+    }
+
+    // y_new and the error estimate: stages in ascending order per element; the
+    // last stage's K is still in x.
+    const simk_u32 S = C.stages;
+    float ynew[N], yerr[N];
+#pragma unroll
+    for (simk_u32 s = 0; s < N; ++s) {
+        ynew[s] = y[s];
+        yerr[s] = 0.0f;
+    }
+    for (simk_u32 st = 0; st + 1 < S; ++st) {
+        if ((C.m_nz >> st) & 1u) p.kaxpy(st, C.m[st], ynew);
+        if ((C.e_nz >> st) & 1u) p.kaxpy(st, C.e[st], yerr);
+    }
+    const bool m_last = (C.m_nz >> (S - 1)) & 1u, e_last = (C.e_nz >> (S - 1)) & 1u;
+    float err = 0.0f;
+#pragma unroll
+    for (simk_u32 s = 0; s < N; ++s) {
+        float yn = ynew[s], ye = yerr[s];
+        if (m_last) yn = __builtin_fmaf(C.m[S - 1], x[s], yn);
+        if (e_last) ye = __builtin_fmaf(C.e[S - 1], x[s], ye);
+        const float b = base[s];
+        const float scale = __builtin_fmaf(C.rtol, simk_fmaxf_(simk_fabsf_(b + y[s]), simk_fabsf_(b + yn)), C.atol);
+        const float q = ye / scale;
+        err = __builtin_fmaf(q, q, err);
+        ynew[s] = yn;
+    }
+    err = __builtin_sqrtf(err * C.inv_n);
+    if (!simk_finitef_(err)) err = (float)1e10;
+
+    float fac = simk_step_factor_f32(err, C.order);
+    bool ok = (err <= one) || (hs <= C.h_min);
+    if (C.fixed_step) {
+        ok = true;
+        fac = one;
+    }
+    if (ok) {
+#pragma unroll
+        for (simk_u32 s = 0; s < N; ++s) {
+            float v = ynew[s];
+            if (C.clamp_negative) v = simk_fmaxf_(v, -base[s]);
+            y[s] = v;
+        }
+        t = last ? t_end : t + hs;
+    }
+    float h_acc = hs * fac;
+    if (reject_last) h_acc = simk_fminf_(h_acc, hs);
+    if (last && fac >= one) h_acc = simk_fmaxf_(h_acc, h_wanted);
+    float h_rej = reject_more ? hs * (float)0.1 : simk_fminf_(hs * fac, hs);
+    if (C.fixed_step) {
+        h_acc = h_wanted;
+        h_rej = h_wanted;
+    }
+    h = ok ? h_acc : h_rej;
+    reject_more = !ok && reject_last;
+    reject_last = !ok;
+    steps = steps + (ok ? one : 0.0f);
+    rejects = rejects + (ok ? 0.0f : one);
+}
+
+// Persistent lane-refilling advance with a generated policy: simk_d32_body's
+// phase machine, load and store (dbd_lanes.h:31-77).
+template <class P>
+SIMK_DEV void simk_d32g_body(const SimkKinArgs &A, const SimkD32Consts &C) {
+    constexpr simk_u32 N = P::N;
+    const simk_u32 wave = blockIdx.x, lane = threadIdx.x;
+    const simk_u32 row_bytes = A.padded * 8u;
+    P p;
+    p.init(A, wave, lane);
+    SimkRows ys;
+    ys.init(A.y, 0, row_bytes, N * row_bytes);
+
+    simk_u32 pos = 0, cell = 0, phase = SIMK_PH_CELL, it = 0, rep = 0;
+    float h = 0.0f, t = 0.0f, applied = 0.0f, seg_end = 0.0f, period = 0.0f, first = 0.0f, count = 0.0f;
+    float steps = 0.0f, rejects = 0.0f, pulses = 0.0f, failed = 0.0f;
+    float base[N], y[N];
+    const float duration = (float)A.duration;
+    bool pending = false, reject_last = false, reject_more = false, fresh = !A.queue_first;
+    pos = wave * SIMK_WAVE + lane;
+
+    auto load = [&](bool first_call) {
+        const simk_u32 voff = cell * 8u;
+        ys.voff = voff;
+#pragma unroll
+        for (simk_u32 s = 0; s < N; ++s) {
+            base[s] = (float)ys.ld(s);
+            y[s] = 0.0f;
+        }
+        if (first_call) p.set_cell(A, cell);
+        SimkRows one;
+        one.init(A.h, voff, 0, row_bytes);
+        h = (float)one.ld(0);
+        one.init(A.failed, voff, 0, row_bytes);
+        failed = (float)one.ld(0);
+        one.init(A.period, voff, 0, row_bytes);
+        period = (float)one.ld(0);
+        one.init(A.pulse_first, voff, 0, row_bytes);
+        first = (float)one.ld(0);
+        one.init(A.pulse_count, voff, 0, row_bytes);
+        count = (float)one.ld(0);
+        steps = rejects = pulses = 0.0f;
+        t = 0.0f;
+        applied = 0.0f;
+    };
+    auto store = [&]() {
+        const simk_u32 voff = cell * 8u;
+        ys.voff = voff;
+#pragma unroll
+        for (simk_u32 s = 0; s < N; ++s) {
+            double v = ys.ld(s) + (double)y[s];
+            if (C.clamp_negative) v = v > 0.0 ? v : 0.0;
+            ys.st(s, v);
+        }
+        SimkRows one;
+        one.init(A.h, voff, 0, row_bytes);
+        one.st(0, (double)h);
+        one.init(A.failed, voff, 0, row_bytes);
+        one.st(0, (double)failed);
+        one.init(A.steps, voff, 0, row_bytes);
+        one.st(0, one.ld(0) + (double)steps);
+        one.init(A.rejects, voff, 0, row_bytes);
+        one.st(0, one.ld(0) + (double)rejects);
+        one.init(A.pulses, voff, 0, row_bytes);
+        one.st(0, one.ld(0) + (double)pulses);
+    };
+
+    for (;;) {
+        while (phase != SIMK_PH_ATTEMPT && phase != SIMK_PH_EXIT) {
+            if (phase == SIMK_PH_CELL) {
+                if (!fresh) {
+                    const simk_u64 ask = __builtin_amdgcn_read_exec();
+                    const simk_u32 rank = __builtin_amdgcn_mbcnt_hi((simk_u32)(ask >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((simk_u32)ask, 0u));
+                    simk_u32 qbase = 0;
+                    if (rank == 0) qbase = atomicAdd(A.queue, (simk_u32)__builtin_popcountll(ask));
+                    const simk_u32 leader = (simk_u32)__builtin_ctzll(ask);
+                    qbase = (simk_u32)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)qbase);
+                    pos = qbase + rank;
+                }
+                fresh = false;
+                if (pos >= A.lanes) {
+                    phase = SIMK_PH_EXIT;
+                    break;
+                }
+                cell = A.order ? A.order[pos] : pos;
+                load(true);
+                rep = 0;
+                phase = SIMK_PH_SEGMENT;
+            } else if (phase == SIMK_PH_SEGMENT) {
+                pending = applied < count;
+                const float next_pulse = __builtin_fmaf(applied, period, first);
+                seg_end = pending ? simk_fminf_(next_pulse, duration) : duration;
+                h = h > 0.0f ? h : C.h_start;
+                reject_last = reject_more = false;
+                it = 0;
+                phase = ((t < seg_end) && !(failed > 0.0f)) ? SIMK_PH_CHECK : SIMK_PH_END;
+            } else if (phase == SIMK_PH_CHECK) {
+                if (++it > C.max_steps) {
+                    failed = 1.0f;
+                    phase = SIMK_PH_END;
+                } else {
+                    phase = SIMK_PH_ATTEMPT;
+                }
+            } else {   // SIMK_PH_END
+                const bool alive = !(failed > 0.0f);
+                const bool hit = pending && alive && (t >= seg_end);
+                if (hit) {
+#pragma unroll
+                    for (simk_u32 s = 0; s < N; ++s) y[s] = y[s] + p.pulse(s);
+                    applied = applied + 1.0f;
+                    pulses = pulses + 1.0f;
+                    h = C.h_start;
+                }
+                const bool done = ((t >= duration) && (applied >= count || !pending)) || !alive;
+                if (!done) {
+                    phase = SIMK_PH_SEGMENT;
+                } else {
+                    store();
+                    if (++rep < A.repeats) {
+                        load(false);
+                        phase = SIMK_PH_SEGMENT;
+                    } else {
+                        phase = SIMK_PH_CELL;
+                    }
+                }
+            }
+        }
+        if (__builtin_amdgcn_ballot_w64(phase == SIMK_PH_ATTEMPT) == 0) break;
+        if (phase == SIMK_PH_ATTEMPT) {
+            simk_d32g_attempt(p, C, base, y, t, seg_end, h, steps, rejects, reject_last, reject_more);
+            phase = (t < seg_end) ? SIMK_PH_CHECK : SIMK_PH_END;
+        }
+    }
+}

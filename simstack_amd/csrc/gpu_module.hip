@@ -22,7 +22,7 @@
 
 #include "device_delta32.h"
 #include "device_wave.h"
-#include "device_kin_src.inc"   // kDeviceKinSource: device_kin.h as text, for hipRTC
+#include "device_kin_src.inc"   // kDeviceKinSource: device_kin.h + device_delta32.h as text, for hipRTC
 
 using namespace simk;
 
@@ -248,6 +248,8 @@ struct State {
     bool jit_ok = false;
     JitKernel jit_mixed;       // compiled on the first mixed-precision launch
     bool jit_mixed_ok = false;
+    JitKernel jit_d32;         // generated f32 delta kernel, compiled on the first precision-1 launch
+    bool jit_d32_ok = false, jit_d32_tried = false;
     u32 sweeps = 1;
     std::string jit_log;
     // ragged batches: several mechanisms in one persistent launch
@@ -391,6 +393,9 @@ void free_mech_resources() {
     if (g.jit_mixed.mod) (void)hipModuleUnload(g.jit_mixed.mod);
     g.jit_mixed = JitKernel();
     g.jit_mixed_ok = false;
+    if (g.jit_d32.mod) (void)hipModuleUnload(g.jit_d32.mod);
+    g.jit_d32 = JitKernel();
+    g.jit_d32_ok = g.jit_d32_tried = false;
 }
 
 template <class T>
@@ -816,11 +821,23 @@ hipError_t launch_wave(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     return hipErrorNotSupported;   // mixed precision has no cooperative form
 }
 
+// f32 delta lanes: the generated kernel (kernel kind 0 when it builds, or 1),
+// else the table walk (kind 2, or 0 when generation fails).
 hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     const Mech &m = g.mech;
-    const u32 slots = m.nnz + 11 * m.n + m.R;   // f32 slots (simk_d32_slots)
+    if (g.kernel_kind != 2 && !g.jit_d32_tried) {
+        g.jit_d32_tried = true;
+        GenOptions o = g.gen;
+        o.precision = 1;
+        g.jit_d32_ok = load_generated(o, g.jit_d32);
+    }
+    if (g.kernel_kind == 1 && !g.jit_d32_ok) return hipErrorInvalidImage;
+    const bool gen = g.kernel_kind != 2 && g.jit_d32_ok;
+    // f32 slots per lane (simk_d32_slots), or the generated kernel's f64 slots
+    const u64 slot_bytes = gen ? 8 : 4;
+    const u32 slots = gen ? g.jit_d32.ws_slots : m.nnz + 11 * m.n + m.R;
     const u64 rows = m.R > m.n ? m.R : m.n;
-    if (rows * L->padded * 8ull >= (1ull << 31) || (u64)slots * SIMK_WAVE * 4ull >= (1ull << 31))
+    if (rows * L->padded * 8ull >= (1ull << 31) || (u64)slots * SIMK_WAVE * slot_bytes >= (1ull << 31))
         return hipErrorInvalidValue;
     if (!g.blocks_per_cu_d32) {
         int blocks = 1;
@@ -829,9 +846,10 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
         g.blocks_per_cu_d32 = blocks > 0 ? blocks : 1;
     }
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
-    const u32 resident = resident_waves(g.blocks_per_cu_d32, (u64)slots * SIMK_WAVE * 4);
+    const u32 resident = resident_waves(gen ? g.jit_d32.blocks_per_cu : g.blocks_per_cu_d32,
+                                        (u64)slots * SIMK_WAVE * slot_bytes);
     const u32 waves = needed < resident ? needed : resident;
-    hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(float));
+    hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * slot_bytes);
     if (e != hipSuccess) return e;
     if (!g.queue) {
         e = hipMalloc(&g.queue, 256);
@@ -857,10 +875,18 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     A.ctl = g.ctl;
     A.method = g.dev_method;
     const SimkD32Consts C = t_consts<f32>();
-    g.last_kernel = 5;
-    g.last_name = "simk_delta32_kernel";
+    g.last_kernel = gen ? 8 : 5;
+    g.last_name = gen ? g.jit_d32.name : std::string("simk_delta32_kernel");
     g.last_slots = slots;
     g.last_waves = waves;
+    if (gen) {
+        SimkD32Args DA;
+        DA.A = A;
+        DA.C = C;
+        size_t size = sizeof(DA);
+        void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &DA, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+        return hipModuleLaunchKernel(g.jit_d32.fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+    }
     hipLaunchKernelGGL(simk_delta32_kernel, dim3(waves), dim3(SIMK_WAVE), 0, g.stream, A, C);
     return hipGetLastError();
 }

@@ -42,6 +42,7 @@ def _gpu_run(gm, air, z, mode, steps, precision=0, kernel=0, wave=False):
             z.commit_pulses()
     finally:
         gm.lib.gpu_kin_set_mode(0)
+        gm.set_kernel(0)
 
 
 def _oracle_run(air, z, mode, steps, precision=0):
@@ -69,13 +70,28 @@ def test_dbd_fp64_bit_exact(gm, air, mode, variant):
 
 
 @pytest.mark.parametrize("mode", [dbd.PULSED, dbd.AVERAGED], ids=["pulsed", "averaged"])
-def test_dbd_f32_delta_bit_exact(gm, air, mode):
+@pytest.mark.parametrize("variant", ["generated", "table"])
+def test_dbd_f32_delta_bit_exact(gm, air, mode, variant):
     """The reference scene's default precision (f32 delta lanes)."""
     a = dbd.reference_zones(air, 12)
     b = a.copy()
-    _gpu_run(gm, air, a, mode, 5, precision=1)
+    _gpu_run(gm, air, a, mode, 5, precision=1, kernel={"generated": 1, "table": 2}[variant])
     _oracle_run(air, b, mode, 5, precision=1)
+    assert gm.kin_info().kernel == {"generated": 8, "table": 5}[variant]
     assert compare_bits(a, b) == ZERO
+
+
+def test_dbd_f32_delta_many_zones_generated(gm, air):
+    """2,048 zones x 20 ms pulsed in f32 delta lanes (generated kernel): 64
+    sampled zones equal the oracle's f32 delta lanes bit for bit."""
+    z = dbd.reference_zones(air, 2048)
+    lanes = np.random.default_rng(6).choice(2048, 64, replace=False)
+    sample = z.subset(lanes)
+    _gpu_run(gm, air, z, dbd.PULSED, 20, precision=1, kernel=1)
+    _oracle_run(air, sample, dbd.PULSED, 20, precision=1)
+    assert gm.kin_info().kernel == 8
+    assert compare_bits(z.subset(lanes), sample) == ZERO
+    assert z.failed[:2048].sum() == 0
 
 
 def test_dbd_many_zones(gm, air):

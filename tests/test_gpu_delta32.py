@@ -3,7 +3,8 @@ restatement of the reference's dbd_advance_group<f32, .., Delta = true>
 (src/sim/dbd_lanes.h:28-78, src/sim/kinetics/kin_solver.h:189-347): bit for
 bit on y (f64 storage after the increment is added and clamped), h, the
 counters and the failure flag. This is the reference's default chemistry
-precision (--precision f32)."""
+precision (--precision f32). Every test runs the generated straight-line f32
+kernel and the table walk."""
 import ctypes as C
 
 import numpy as np
@@ -16,6 +17,14 @@ from helpers import add_pulses, compare_bits, make_problem
 pytestmark = pytest.mark.gpu
 
 ZERO = {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
+KERNEL_CODE = {1: 8, 2: 5}   # kernel kind -> SimkKinInfo.kernel
+
+
+@pytest.fixture(params=[1, 2], ids=["generated", "table"])
+def kind(request, gm):
+    gm.set_kernel(request.param)
+    yield request.param
+    gm.set_kernel(0)
 
 
 def _gpu(gm, mech, meth, ctl, z, durations, repeats=1, default_abi=False):
@@ -38,7 +47,7 @@ def _oracle(mech, meth, ctl, z, durations, lanes=None):
 
 
 @pytest.mark.parametrize("mid", [0, 1, 2], ids=["ros2", "ros3", "rodas3"])
-def test_delta32_bit_exact_with_pulses(gm, mid):
+def test_delta32_bit_exact_with_pulses(gm, kind, mid):
     """Every method, pulses on half the cells, two calls carrying h. max_steps
     bounds the cells f32 cannot converge (the reference's f32 lanes fail them
     too); the second-order method runs at rtol 1e-2."""
@@ -49,12 +58,12 @@ def test_delta32_bit_exact_with_pulses(gm, mid):
     a, b = z.copy(), z.copy()
     info = _gpu(gm, mech, meth, ctl, a, [0.01, 0.01], default_abi=True)
     _oracle(mech, meth, ctl, b, [0.01, 0.01])
-    assert info.kernel == 5
+    assert info.kernel == KERNEL_CODE[kind]
     assert compare_bits(a, b) == ZERO
     assert b.pulses[:256].sum() > 0 and b.steps[:256].min() > 2
 
 
-def test_delta32_c2_sampled(gm):
+def test_delta32_c2_sampled(gm, kind):
     """48/150 with pH spread: 1,024 cells, 128 checked by the oracle."""
     sp, mech, z = make_problem("c2", 1024)
     meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
@@ -66,7 +75,7 @@ def test_delta32_c2_sampled(gm):
     assert compare_bits(a.subset(lanes), b) == ZERO
 
 
-def test_delta32_fused_repeats_equal_separate_calls(gm):
+def test_delta32_fused_repeats_equal_separate_calls(gm, kind):
     """simk_kin_advance_n(3) in delta lanes == three calls: each call restarts
     from the f64 storage (base re-rounded, counters added)."""
     sp, mech, z = make_problem("c3c", 500)
@@ -79,7 +88,7 @@ def test_delta32_fused_repeats_equal_separate_calls(gm):
     assert compare_bits(a, c) == ZERO
 
 
-def test_delta32_refill_with_few_waves(gm):
+def test_delta32_refill_with_few_waves(gm, kind):
     """Three resident waves for 400 cells of the 40/130 mechanism (lanes refill
     many times). max_steps bounds the cells f32 cannot converge: about 37,000
     attempts in all, at most 1,000 for one cell."""

@@ -197,6 +197,40 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
     return out
 
 
+def dbd_block(gpu, problem, zones=65536, steps=5, dt=1e-3):
+    """The DBD reactor workload: SimStack's humid-air mechanism (17 species, 44
+    reactions) through the reference ABI, the reference's zone operating points,
+    pulsed discharge, `steps` master steps of dt with the host preparing each
+    step's pulses (not timed). Kernel time per step for FP64 lanes and for the
+    reference's default f32 delta lanes (generated kernel, and the table walk
+    for comparison)."""
+    from simstack_amd import _ffi, dbd
+    air = dbd.AirMechanism(caps=_ffi.DEFAULT_CAPS)
+    gm = run_workload.gm
+    meth, ctl = problem.method(2), dbd.default_control()
+    src = dbd.source_defaults()
+    out = {"workload": f"DBD humid air 17/44, {zones} zones (T 300-340 K, 1 atm, 0-3 % H2O, 5-12.5 kHz), pulsed, "
+                       f"{steps} x {dt * 1e3:g} ms, Rodas3, rtol {ctl.rtol:g} atol {ctl.atol:g}"}
+    for name, precision, kind in (("fp64", 0, 1), ("f32_delta", 1, 1), ("f32_delta_table_walk", 1, 2)):
+        z = dbd.reference_zones(air, zones)
+        gm.arena_reset()
+        gm.set_kernel(kind)
+        gm._check(gm.lib.gpu_kin_set_mech(air.bytes.ctypes.data, C.byref(meth), C.byref(ctl)), "gpu_kin_set_mech")
+        dz = gpu.DeviceZones(gm, z)
+        kms = []
+        for step in range(steps):
+            z.prepare_source(src, dbd.PULSED, step * dt, (step + 1) * dt)
+            dz.upload()
+            kms.append(dz.advance(dt, precision=precision))
+            dz.download(mutable_only=True)
+            z.commit_pulses()
+        gm.set_kernel(0)
+        acc = float(z.steps[:zones].sum())
+        out[name] = {"kernel_ms_per_step": float(np.mean(kms)), "cell_steps_per_s": acc / (sum(kms) / 1e3),
+                     "failed_zones": int(z.failed[:zones].sum()), "kernel": gm.kin_info().kernel_name.decode()}
+    return out
+
+
 def serving_block(gpu, problem, rate=200.0, window=1.0, steps=20, substeps=4, dt=1e-3):
     """Poisson arrivals over the three ragged mechanisms through simk_serve_*."""
     from simstack_amd import serving
@@ -248,6 +282,7 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the three-mechanism ragged batch")
     ap.add_argument("--ragged-steps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
+    ap.add_argument("--no-dbd", action="store_true", help="skip the DBD reactor workload")
     ap.add_argument("--serve-window", type=float, default=0.25, help="seconds of Poisson arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
@@ -324,6 +359,8 @@ def main():
         out["ragged"] = ragged_block(gpu, problem, args.ragged_steps)
     if not args.no_serve:
         out["serving"] = serving_block(gpu, problem, rate=args.serve_rate, window=args.serve_window)
+    if not args.no_dbd:
+        out["dbd"] = dbd_block(gpu, problem)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
     print(json.dumps(out), flush=True)

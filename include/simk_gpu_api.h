@@ -103,8 +103,9 @@ typedef struct SimkGenOptions {
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 96; ~0u: never) */
-    u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default, 80 KiB per wave =
-                            160 f64 / 320 f32 entries, none for refined mixed kernels; ~0u: none; at most 160 KiB) */
+    u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default: 160 f64 entries
+                            = 80 KiB per wave, 320 f32 in mixed precision, 160 f32 in f32 delta lanes, none for
+                            refined mixed kernels; ~0u: none; at most 160 KiB) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 

@@ -138,12 +138,15 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
     return L;
 }
 
-// Measured on the MI355X (48/150, 24/70 and the three-mechanism kernel):
-// two waves per CU with 80 KiB of LU each beat every grid with less LDS per
-// wave by 20-70 %; more than 80 KiB leaves one wave per CU and loses. The
-// refined mixed kernel is the exception (slower with LDS entries).
+// Measured on the MI355X. FP64 (48/150, 24/70, the DBD air mechanism and
+// the three-mechanism kernel): two waves per CU with 80 KiB of LU each beat
+// every grid with less LDS per wave by 20-100 %; more than 80 KiB leaves one
+// wave per CU and loses. f32 delta lanes (fewer VGPRs): 40 KiB per wave and
+// four waves per CU run the DBD air workload 1.9x faster than its whole LU
+// (54 KiB, three waves per CU). The refined mixed kernel is slower with LDS
+// entries.
 u32 auto_lds(const GenOptions &o) {
-    if (o.precision == 1) return 320u;
+    if (o.precision == 1) return 160u;
     if (o.precision == 2) return o.sweeps ? 0u : 320u;
     return 160u;
 }

@@ -39,9 +39,9 @@ struct GenResult {
 // and to 160 KiB per wave).
 u32 lds_entries(const Mech &m, const GenOptions &o);
 
-// The default LU entries in LDS: 80 KiB per wave (160 f64 or 320 f32
-// entries of 64 lanes), so two waves share a CU's 160 KiB; none for
-// mixed-precision kernels with refinement sweeps.
+// The default LU entries in LDS: 80 KiB per wave (160 f64, or 320 f32 in
+// mixed precision), so two waves share a CU's 160 KiB; 40 KiB (160 f32) for
+// f32 delta lanes; none for mixed-precision kernels with refinement sweeps.
 u32 auto_lds(const GenOptions &o);
 
 // Stage plan shared with the host launch code: which stages evaluate f.

@@ -197,6 +197,51 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
     return out
 
 
+def advect_block(species=48, channels=512, nodes=4096, substeps=4, reps=50):
+    """The NFT upwind row update (simk_upwind_f64) on a [species][channels *
+    nodes] f64 field of 805 MB, beyond the 256 MB Infinity Cache. One launch
+    reads and writes each row once whatever the substep count (substeps run
+    on registers and LDS), so its algorithmic bytes are 2 x field + courant
+    + inflow. Wall time over `reps` back-to-back launches, synchronized."""
+    gm = run_workload.gm
+    gm.arena_reset()
+    cells = channels * nodes
+    field = species * cells * 8
+    d_y, d_c, d_in = gm.alloc(field), gm.alloc(cells * 8), gm.alloc(species * 8)
+    rng = np.random.default_rng(11)
+    gm.to_device(d_y, rng.random(species * cells))
+    gm.to_device(d_c, 0.2 + 0.6 * rng.random(cells))
+    gm.to_device(d_in, rng.random(species))
+    lib = gm.lib
+
+    def launch():
+        gm._check(lib.simk_upwind_f64(d_y, cells, species, channels, nodes, d_c, substeps, d_in), "simk_upwind_f64")
+
+    for _ in range(3):
+        launch()
+    lib.gpu_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        launch()
+    lib.gpu_sync()
+    per = (time.perf_counter() - t0) / reps
+    algo = 2 * field + cells * 8 + species * 8
+    # a plain device copy of half the field onto the other half moves the same
+    # bytes (read + write): the bandwidth a streaming kernel reaches here
+    half = field // 2
+    cms = []
+    for _ in range(10):
+        ms = C.c_float()
+        gm._check(lib.simk_copy_device(d_y + half, d_y, half, C.byref(ms)), "simk_copy_device")
+        cms.append(ms.value)
+    copy_gbs = 2 * half / (float(np.median(cms)) * 1e-3) / 1e9
+    return {"workload": f"{species} species x {channels} channels x {nodes} nodes f64, {substeps} substeps per launch",
+            "ms_per_launch": per * 1e3, "node_substeps_per_s": species * cells * substeps / per,
+            "roofline": {"bound": "hbm", "achieved": algo / per / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": algo / per / 1e9 / PEAK_HBM_GBS, "algorithmic_bytes": algo},
+            "copy_kernel_GBps": copy_gbs, "frac_of_copy": algo / per / 1e9 / copy_gbs}
+
+
 def dbd_block(gpu, problem, zones=65536, steps=5, dt=1e-3):
     """The DBD reactor workload: SimStack's humid-air mechanism (17 species, 44
     reactions) through the reference ABI, the reference's zone operating points,
@@ -283,6 +328,7 @@ def main():
     ap.add_argument("--ragged-steps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--no-dbd", action="store_true", help="skip the DBD reactor workload")
+    ap.add_argument("--no-advect", action="store_true", help="skip the upwind advection bandwidth block")
     ap.add_argument("--serve-window", type=float, default=0.25, help="seconds of Poisson arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
@@ -361,6 +407,8 @@ def main():
         out["serving"] = serving_block(gpu, problem, rate=args.serve_rate, window=args.serve_window)
     if not args.no_dbd:
         out["dbd"] = dbd_block(gpu, problem)
+    if not args.no_advect:
+        out["advect"] = advect_block()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
     print(json.dumps(out), flush=True)

@@ -286,6 +286,9 @@ GPU_API s32 simk_plugflow_f64(f64 *y, u64 padded, u32 species, u32 zones, f64 a,
  * alignment) by a plain streaming kernel, timed with stream events. HBM
  * byte counters of this kernel calibrate the solver kernels' counters. */
 GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms);
+/* The same copy with one 8-byte access per lane, the access width of the
+ * solver and advection kernels: counter calibration for them. */
+GPU_API s32 simk_copy_device_b64(void *dst, const void *src, u64 bytes, f32 *ms);
 
 #ifdef __cplusplus
 }

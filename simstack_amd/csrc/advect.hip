@@ -4,13 +4,21 @@
 //
 // Per substep every node of a channel takes x_j + c_j (x_{j-1} - x_j) from the
 // OLD value of its upstream neighbour (node 0 reads the inflow): a Jacobi
-// update, so all nodes of a substep are independent. One workgroup owns one
-// (field, channel) row of `nodes` values: it reads the row once, keeps it in
-// registers (chunk of consecutive nodes per thread) for every substep,
-// exchanges only chunk boundaries through LDS, and writes the row once — HBM
-// traffic per launch is one read and one write of the field, whatever the
-// substep count. Rows are [field][padded] of the zone block with cell =
-// channel * nodes + node.
+// update, so all nodes of a substep are independent. A row is read once, kept
+// in registers for every substep and written once, so HBM traffic per launch
+// is one read and one write of the field, whatever the substep count. Rows are
+// [field][padded] of the zone block with cell = channel * nodes + node.
+//
+// Upwind rows (up to 8,192 nodes): 1, 2 or 4 waves per (field, channel) row,
+// each holding up to 2,048 nodes with node slot * 64 + lane in register slot
+// `slot` of lane `lane`, so every global load and store is one coalesced
+// 512-byte wave access straight to registers. The upstream neighbour is the
+// same slot one lane down (DPP wave_shr:1); lane 0 takes lane 63 of the slot
+// before (DPP wave_ror:1 of that slot), slots updated from the last down so
+// every read sees the old value. Only a part's first node crosses waves
+// (through LDS, one barrier per substep). The plug-flow chain: one 256-thread
+// workgroup per species row, a chunk of consecutive zones per thread, chunk
+// boundaries through LDS.
 #include <hip/hip_runtime.h>
 
 #include "simk_gpu_api.h"
@@ -19,7 +27,8 @@
 namespace {
 
 constexpr unsigned kThreads = 256;
-constexpr unsigned kMaxChunk = 32;   // nodes per thread: up to 8192 nodes per channel
+constexpr unsigned kMaxChunk = 32;   // plug flow: zones per thread, up to 8192 zones
+constexpr unsigned kRowsPerBlock = kThreads / 64u;   // waves per workgroup
 
 template <class T>
 __device__ __forceinline__ T fma_t(T a, T b, T c);
@@ -85,14 +94,114 @@ __device__ __forceinline__ void upwind_row(T *xr, const T *cr, unsigned nodes, u
     for (unsigned j = t; j < nodes; j += kThreads) xr[j] = stage[j];
 }
 
-template <class T, unsigned CHUNK>
-__global__ void __launch_bounds__(kThreads) simk_upwind_kernel(T *x, unsigned long long padded, unsigned channels,
-                                                               unsigned nodes, const T *courant, unsigned substeps,
-                                                               const T *inflow, T *outlet_sum) {
-    const unsigned row = blockIdx.x;   // field * channels + channel
-    const unsigned field = row / channels, ch = row % channels;
-    upwind_row<T, CHUNK>(x + field * padded + (unsigned long long)ch * nodes, courant + (unsigned long long)ch * nodes,
-                         nodes, substeps, inflow ? inflow[field] : T(0), outlet_sum ? outlet_sum + row : nullptr);
+// DPP across the whole wave: wave_shr:1 (lane l takes lane l - 1; lane 0
+// keeps `old`) and wave_ror:1 (lane l takes lane l - 1, lane 0 takes lane 63).
+constexpr int kWaveShr1 = 0x138, kWaveRor1 = 0x13C;
+template <int CTRL>
+__device__ __forceinline__ float dpp(float old, float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double old, double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF,
+                                               false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// The upstream values of slot k: lane l gets cur's lane l - 1, lane 0 gets
+// prev's lane 63 (prev: the slot before, or `first` broadcast for slot 0).
+template <class T>
+__device__ __forceinline__ T shift_up(T cur, T prev_rotated) {
+    return dpp<kWaveShr1>(prev_rotated, cur);
+}
+__device__ __forceinline__ float lane_value(float v, unsigned lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), (int)lane));
+}
+__device__ __forceinline__ double lane_value(double v, unsigned lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, (int)lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), (int)lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// One part of a row: nodes [part * SLOTS * 64, (part + 1) * SLOTS * 64) of a
+// row of `nodes` values, advanced by one wave; a row of WPR parts is advanced
+// by WPR consecutive waves of the workgroup. Each substep a part's first node
+// takes its upstream value from the previous part's last node (old value)
+// through LDS, one workgroup barrier per substep (double-buffered by
+// substep parity); WPR = 1 needs neither. `active` false: a wave past the
+// last row, which only joins the barriers. Nodes past the end hold 0 with
+// coefficient 0: fma(0, up - 0, 0) leaves them unchanged for finite data,
+// they only feed further nodes past the end, and they are never stored.
+template <class T, unsigned SLOTS, unsigned WPR>
+__device__ __forceinline__ void upwind_row_wave(T *xr, const T *cr, unsigned nodes, unsigned substeps, T in,
+                                                T *outlet, bool active) {
+    constexpr unsigned kPart = SLOTS * 64u;
+    __shared__ T edge[2][kRowsPerBlock];
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x / 64u, part = wave % WPR;
+    const unsigned base = part * kPart;
+    T v[SLOTS], c[SLOTS];
+#pragma unroll
+    for (unsigned k = 0; k < SLOTS; ++k) {
+        const unsigned j = base + k * 64u + lane;
+        const bool in_row = active && j < nodes;
+        v[k] = in_row ? xr[j] : T(0);
+        c[k] = in_row ? cr[j] : T(0);
+    }
+    // the outlet node (nodes - 1) lives in one part
+    const bool owns_outlet = active && outlet && (nodes - 1) / kPart == part;
+    const unsigned oslot = ((nodes - 1) % kPart) / 64u, olane = (nodes - 1) % 64u;
+    T out = owns_outlet ? *outlet : T(0);
+    for (unsigned s = 0; s < substeps; ++s) {
+        if (owns_outlet) {
+            T ov = T(0);
+#pragma unroll
+            for (unsigned k = 0; k < SLOTS; ++k)
+                if (k == oslot) ov = v[k];
+            out = out + lane_value(ov, olane);
+        }
+        T first = in;
+        if constexpr (WPR > 1) {
+            if (lane == 63) edge[s & 1u][wave] = v[SLOTS - 1];
+            __syncthreads();
+            if (part) first = edge[s & 1u][wave - 1];
+        }
+#pragma unroll
+        for (unsigned k = SLOTS; k-- > 0;) {
+            const T prev = k ? dpp<kWaveRor1>(T(0), v[k - 1]) : first;
+            const T up = shift_up(v[k], prev);
+            v[k] = fma_t<T>(c[k], up - v[k], v[k]);
+        }
+    }
+    if (owns_outlet && lane == 0) *outlet = out;
+    if (!active) return;
+#pragma unroll
+    for (unsigned k = 0; k < SLOTS; ++k) {
+        const unsigned j = base + k * 64u + lane;
+        if (j < nodes) xr[j] = v[k];
+    }
+}
+
+// Rows of the field in row order (field * channels + channel), WPR waves per
+// row, kRowsPerBlock / WPR rows per workgroup. (Measured: an XCD-aware order
+// that keeps each XCD's coefficient rows in its L2 was no faster; the 256 MB
+// Infinity Cache already serves their re-reads.)
+template <class T, unsigned SLOTS, unsigned WPR>
+__global__ void __launch_bounds__(kThreads) simk_upwind_wave_kernel(T *x, unsigned long long padded, unsigned rows,
+                                                                    unsigned channels, unsigned nodes,
+                                                                    const T *courant, unsigned substeps,
+                                                                    const T *inflow, T *outlet_sum) {
+    const unsigned row = blockIdx.x * (kRowsPerBlock / WPR) + threadIdx.x / 64u / WPR;
+    const bool active = row < rows;
+    const unsigned r = active ? row : 0;
+    const unsigned field = r / channels, ch = r % channels;
+    if (WPR == 1 && !active) return;   // whole waves, no barriers
+    upwind_row_wave<T, SLOTS, WPR>(x + field * padded + (unsigned long long)ch * nodes,
+                                   courant + (unsigned long long)ch * nodes, nodes, substeps,
+                                   inflow ? inflow[field] : T(0), outlet_sum ? outlet_sum + r : nullptr, active);
 }
 
 // Plug flow along a chain of zones (dbd_flow_exchange, dbd.cpp:465-483): one
@@ -104,25 +213,42 @@ __global__ void __launch_bounds__(kThreads) simk_plugflow_kernel(double *y, unsi
     upwind_row<double, CHUNK, false>(y + s * padded, nullptr, zones, 1, feed[s], nullptr, a);
 }
 
-// Many requests' rows in one launch: block b belongs to the segment whose
+// Many requests' rows in one launch: wave row b belongs to the segment whose
 // [row_begin, row_begin + fields * channels) holds b (segments sorted by
-// row_begin, the first at 0).
-template <unsigned CHUNK>
-__global__ void __launch_bounds__(kThreads) simk_upwind_batch_kernel(const SimkUpwindSeg *segs, unsigned count,
-                                                                     unsigned substeps) {
-    const unsigned b = blockIdx.x;
-    unsigned lo = 0, hi = count;   // last segment with row_begin <= b
+// row_begin, the first at 0). SLOTS and WPR fit the longest row; shorter rows
+// leave their last parts empty.
+template <unsigned SLOTS, unsigned WPR>
+__global__ void __launch_bounds__(kThreads) simk_upwind_batch_wave_kernel(const SimkUpwindSeg *segs, unsigned count,
+                                                                          unsigned total_rows, unsigned substeps) {
+    const unsigned b = blockIdx.x * (kRowsPerBlock / WPR) + threadIdx.x / 64u / WPR;
+    const bool active = b < total_rows;
+    if (WPR == 1 && !active) return;   // whole waves, no barriers
+    const unsigned bb = active ? b : 0;
+    unsigned lo = 0, hi = count;
     while (hi - lo > 1) {
         const unsigned mid = (lo + hi) / 2;
-        if (segs[mid].row_begin <= b) lo = mid;
+        if (segs[mid].row_begin <= bb) lo = mid;
         else hi = mid;
     }
     const SimkUpwindSeg &S = segs[lo];
-    const unsigned row = b - S.row_begin;
+    const unsigned row = bb - S.row_begin;
     const unsigned field = row / S.channels, ch = row % S.channels;
-    upwind_row<double, CHUNK>(S.x + field * S.padded + (unsigned long long)ch * S.nodes,
-                              S.courant + (unsigned long long)ch * S.nodes, S.nodes, substeps,
-                              S.inflow ? S.inflow[field] : 0.0, nullptr);
+    upwind_row_wave<double, SLOTS, WPR>(S.x + field * S.padded + (unsigned long long)ch * S.nodes,
+                                        S.courant + (unsigned long long)ch * S.nodes, S.nodes, substeps,
+                                        S.inflow ? S.inflow[field] : 0.0, nullptr, active);
+}
+
+// Wave shape for rows of up to `nodes` values: 16 or 32 slots per wave, and
+// 1, 2 or 4 waves per row (0: too long).
+struct WaveShape {
+    unsigned slots, wpr;
+};
+WaveShape wave_shape(u32 nodes) {
+    if (nodes <= 1024) return {16, 1};
+    if (nodes <= 2048) return {32, 1};
+    if (nodes <= 4096) return {32, 2};
+    if (nodes <= 8192) return {32, 4};
+    return {0, 0};
 }
 
 template <class T>
@@ -130,19 +256,25 @@ hipError_t launch_upwind(hipStream_t stream, T *x, u64 padded, u32 fields, u32 c
                          const T *courant, u32 substeps, const T *inflow, T *outlet_sum) {
     if (!x || !courant || fields == 0 || channels == 0 || nodes == 0) return hipErrorInvalidValue;
     if ((u64)channels * nodes > padded) return hipErrorInvalidValue;
-    const u32 chunk_needed = (nodes + kThreads - 1) / kThreads;
-    const dim3 grid(fields * channels), block(kThreads);
-    if (chunk_needed <= 4)
-        hipLaunchKernelGGL((simk_upwind_kernel<T, 4>), grid, block, 0, stream, x, (unsigned long long)padded, channels,
+    if ((u64)fields * channels >= (1ull << 32)) return hipErrorInvalidValue;
+    const u32 rows = fields * channels;
+    const WaveShape w = wave_shape(nodes);
+    if (!w.wpr) return hipErrorInvalidValue;
+    const u32 per_block = kRowsPerBlock / w.wpr;
+    const dim3 grid((rows + per_block - 1) / per_block), block(kThreads);
+    const unsigned long long pd = padded;
+    if (w.slots == 16)
+        hipLaunchKernelGGL((simk_upwind_wave_kernel<T, 16, 1>), grid, block, 0, stream, x, pd, rows, channels,
                            nodes, courant, substeps, inflow, outlet_sum);
-    else if (chunk_needed <= 16)
-        hipLaunchKernelGGL((simk_upwind_kernel<T, 16>), grid, block, 0, stream, x, (unsigned long long)padded,
-                           channels, nodes, courant, substeps, inflow, outlet_sum);
-    else if (chunk_needed <= kMaxChunk)
-        hipLaunchKernelGGL((simk_upwind_kernel<T, kMaxChunk>), grid, block, 0, stream, x, (unsigned long long)padded,
-                           channels, nodes, courant, substeps, inflow, outlet_sum);
+    else if (w.wpr == 1)
+        hipLaunchKernelGGL((simk_upwind_wave_kernel<T, 32, 1>), grid, block, 0, stream, x, pd, rows, channels,
+                           nodes, courant, substeps, inflow, outlet_sum);
+    else if (w.wpr == 2)
+        hipLaunchKernelGGL((simk_upwind_wave_kernel<T, 32, 2>), grid, block, 0, stream, x, pd, rows, channels,
+                           nodes, courant, substeps, inflow, outlet_sum);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((simk_upwind_wave_kernel<T, 32, 4>), grid, block, 0, stream, x, pd, rows, channels,
+                           nodes, courant, substeps, inflow, outlet_sum);
     return hipGetLastError();
 }
 
@@ -151,16 +283,22 @@ hipError_t launch_upwind(hipStream_t stream, T *x, u64 padded, u32 fields, u32 c
 hipError_t simk_enqueue_upwind_batch(hipStream_t stream, const SimkUpwindSeg *segs, u32 count, u32 total_rows,
                                      u32 max_nodes, u32 substeps) {
     if (!segs || count == 0 || total_rows == 0 || max_nodes == 0) return hipErrorInvalidValue;
-    const u32 chunk_needed = (max_nodes + kThreads - 1) / kThreads;
-    const dim3 grid(total_rows), block(kThreads);
-    if (chunk_needed <= 4)
-        hipLaunchKernelGGL((simk_upwind_batch_kernel<4>), grid, block, 0, stream, segs, count, substeps);
-    else if (chunk_needed <= 16)
-        hipLaunchKernelGGL((simk_upwind_batch_kernel<16>), grid, block, 0, stream, segs, count, substeps);
-    else if (chunk_needed <= kMaxChunk)
-        hipLaunchKernelGGL((simk_upwind_batch_kernel<kMaxChunk>), grid, block, 0, stream, segs, count, substeps);
+    const WaveShape w = wave_shape(max_nodes);
+    if (!w.wpr) return hipErrorInvalidValue;
+    const u32 per_block = kRowsPerBlock / w.wpr;
+    const dim3 grid((total_rows + per_block - 1) / per_block), block(kThreads);
+    if (w.slots == 16)
+        hipLaunchKernelGGL((simk_upwind_batch_wave_kernel<16, 1>), grid, block, 0, stream, segs, count, total_rows,
+                           substeps);
+    else if (w.wpr == 1)
+        hipLaunchKernelGGL((simk_upwind_batch_wave_kernel<32, 1>), grid, block, 0, stream, segs, count, total_rows,
+                           substeps);
+    else if (w.wpr == 2)
+        hipLaunchKernelGGL((simk_upwind_batch_wave_kernel<32, 2>), grid, block, 0, stream, segs, count, total_rows,
+                           substeps);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((simk_upwind_batch_wave_kernel<32, 4>), grid, block, 0, stream, segs, count, total_rows,
+                           substeps);
     return hipGetLastError();
 }
 

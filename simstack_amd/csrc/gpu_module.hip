@@ -191,6 +191,13 @@ __global__ void __launch_bounds__(256) simk_copy_kernel(double2 *dst, const doub
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// The same with one 8-byte access per lane (512 bytes per wave access, the
+// width of the solver and advection kernels' loads and stores).
+__global__ void __launch_bounds__(256) simk_copy8_kernel(double *dst, const double *src, u64 n) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // Sets the persistent kernels' queue head (a kernel, not a memset, so the
 // sequence stays valid inside a captured graph).
 __global__ void simk_queue_reset_kernel(u32 *queue, u32 value) {
@@ -1647,16 +1654,20 @@ GPU_API s32 simk_jit_precompile_multi(u32 count, const void *const *mechs, const
     return ok ? 0 : err_code(hipErrorInvalidImage);
 }
 
-GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms) {
+static s32 copy_device(void *dst, const void *src, u64 bytes, f32 *ms, u32 width) {
     if (!dst || !src || (bytes & 15) || ((u64)dst & 15) || ((u64)src & 15)) return err_code(hipErrorInvalidValue);
-    const u64 n = bytes / 16;
+    const u64 n = bytes / width;
     const u32 block = 256;
     u64 grid = (n + block - 1) / block;
     if (grid > 65536) grid = 65536;
     if (grid == 0) return 0;
     (void)hipEventRecord(g.kin_start, g.stream);
-    hipLaunchKernelGGL(simk_copy_kernel, dim3((u32)grid), dim3(block), 0, g.stream, (double2 *)dst,
-                       (const double2 *)src, n);
+    if (width == 16)
+        hipLaunchKernelGGL(simk_copy_kernel, dim3((u32)grid), dim3(block), 0, g.stream, (double2 *)dst,
+                           (const double2 *)src, n);
+    else
+        hipLaunchKernelGGL(simk_copy8_kernel, dim3((u32)grid), dim3(block), 0, g.stream, (double *)dst,
+                           (const double *)src, n);
     (void)hipEventRecord(g.kin_stop, g.stream);
     hipError_t e = hipEventSynchronize(g.kin_stop);
     if (e != hipSuccess) return err_code(e);
@@ -1664,6 +1675,12 @@ GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms) {
     (void)hipEventElapsedTime(&t, g.kin_start, g.kin_stop);
     if (ms) *ms = t;
     return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_copy_device(void *dst, const void *src, u64 bytes, f32 *ms) { return copy_device(dst, src, bytes, ms, 16); }
+
+GPU_API s32 simk_copy_device_b64(void *dst, const void *src, u64 bytes, f32 *ms) {
+    return copy_device(dst, src, bytes, ms, 8);
 }
 
 GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCaps *caps, const KinMethod *method,

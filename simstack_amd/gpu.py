@@ -63,7 +63,8 @@ REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_use
                      "gpu_kin_wait", "gpu_kin_set_mode")
 EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_set_grid", "simk_kin_set_order",
                      "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
-                     "simk_jit_precompile", "simk_copy_device", "simk_upwind_f64", "simk_upwind_f32",
+                     "simk_jit_precompile", "simk_copy_device", "simk_copy_device_b64", "simk_upwind_f64",
+                     "simk_upwind_f32",
                      "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
                      "simk_graph_destroy", "simk_jit_precompile_multi", "simk_serve_open", "simk_serve_submit",
                      "simk_serve_step", "simk_serve_stats", "simk_serve_close", "simk_kin_set_cache_budget",
@@ -105,6 +106,7 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_jit_precompile", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(SimkGenOptions),
          C.c_char_p, C.c_uint32)
     _sig(lib, "simk_copy_device", C.c_int32, V, V, C.c_uint64, P(C.c_float))
+    _sig(lib, "simk_copy_device_b64", C.c_int32, V, V, C.c_uint64, P(C.c_float))
     _sig(lib, "simk_kin_advance_n", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, P(C.c_float))
     _sig(lib, "simk_kin_set_grid", C.c_int32, C.c_uint32)
     _sig(lib, "simk_kin_set_order", C.c_int32, V)

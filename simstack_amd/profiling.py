@@ -46,10 +46,13 @@ def main():
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--copy-kernel", default="simk_copy_kernel",
+                    help="simk_copy_kernel (16-byte lanes) or simk_copy8_kernel (8-byte lanes)")
     ap.add_argument("--l2")
     ap.add_argument("--sq")
     ap.add_argument("--kernel", default="simk_gen")
     ap.add_argument("--algo-flops-per-launch", type=float, default=0.0)
+    ap.add_argument("--algo-bytes-per-launch", type=float, default=0.0)
     ap.add_argument("--out", required=True)
     ap.add_argument("--traffic-json")
     a = ap.parse_args()
@@ -66,8 +69,8 @@ def main():
         open(a.out, "w").write("\n".join(lines) + "\n")
         print("\n".join(lines))
         return
-    cf = counter(a.copy_fetch, "simk_copy_kernel", "FETCH_SIZE")
-    cw = counter(a.copy_write, "simk_copy_kernel", "WRITE_SIZE")
+    cf = counter(a.copy_fetch, a.copy_kernel, "FETCH_SIZE")
+    cw = counter(a.copy_write, a.copy_kernel, "WRITE_SIZE")
     f_scale = a.copy_bytes / (min(cf) * 1024.0)
     w_scale = a.copy_bytes / (min(cw) * 1024.0)
     fetch = counter(a.fetch, a.kernel, "FETCH_SIZE")
@@ -78,7 +81,8 @@ def main():
              "`rocprofv3 --pmc FETCH_SIZE`, `--pmc WRITE_SIZE`, ...):", "", "```",
              f"rocprofv3 --kernel-trace --stats -d <dir> -o kt -- {a.cmd}", "```", "",
              "Calibration: `rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -- python3 -m simstack_amd.calibrate "
-             f"--mib {a.copy_bytes >> 20} --reps 2` (a copy of {a.copy_bytes} bytes reads and writes that many).", "",
+             f"--mib {a.copy_bytes >> 20} --reps 2{' --width 8' if a.copy_kernel == 'simk_copy8_kernel' else ''}` "
+             f"(`{a.copy_kernel}`: a copy of {a.copy_bytes} bytes reads and writes that many).", "",
              "| quantity | value |", "|---|---|",
              f"| kernel `{a.kernel}*` dispatches | {len(dur)} |",
              f"| average duration per dispatch | {avg_ns / 1e6:.3f} ms |",
@@ -94,6 +98,10 @@ def main():
     if a.algo_flops_per_launch:
         extra.append(f"| algorithmic FP64 flops per dispatch | {a.algo_flops_per_launch:.4e} |")
         extra.append(f"| achieved FP64 | {a.algo_flops_per_launch / avg_ns / 1e3:.3f} TFLOP/s |")
+    if a.algo_bytes_per_launch:
+        extra.append(f"| algorithmic bytes per dispatch | {a.algo_bytes_per_launch:.4e} |")
+        extra.append(f"| achieved (algorithmic bytes / avg duration) | {a.algo_bytes_per_launch / avg_ns:.1f} GB/s |")
+        extra.append(f"| counted traffic / algorithmic bytes | {(rd + wr) / a.algo_bytes_per_launch:.3f} |")
     if a.l2:
         hit = sum(counter(a.l2, a.kernel, "TCC_HIT_sum"))
         miss = sum(counter(a.l2, a.kernel, "TCC_MISS_sum"))

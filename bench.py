@@ -119,7 +119,7 @@ def rel_err_groups(got, ref, atol, rtol):
     return out
 
 
-def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0, lds_slots=0):
+def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0, lds_slots=0, tune=1.0):
     """Three mechanisms (60/200, 40/130, 24/70) with 1k / 20k / 200k cells in one
     persistent launch per operator-split step, the steps captured in one HIP
     graph; beside it the same steps as eager launches and as one mechanism at a
@@ -177,18 +177,33 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
     eager_wall = time.perf_counter() - t0
     eager_acc, eager_att, eager_steps = totals()
     # the same steps captured in one graph
-    reset()
-    gid = gm.capture_multi(dzs, duration, 1, steps)
-    gm.lib.gpu_sync()
-    t0 = time.perf_counter()
-    gm.replay(gid)
-    gm.lib.gpu_sync()
-    graph_wall = time.perf_counter() - t0
-    gm.destroy_graph(gid)
-    graph_acc, graph_att, graph_steps = totals()
+    def graph_run():
+        reset()
+        gid = gm.capture_multi(dzs, duration, 1, steps)
+        gm.lib.gpu_sync()
+        t0 = time.perf_counter()
+        gm.replay(gid)
+        gm.lib.gpu_sync()
+        wall = time.perf_counter() - t0
+        gm.destroy_graph(gid)
+        return (wall,) + totals()
+
+    untuned_wall, untuned_acc, _, untuned_steps = graph_run()
+    tuned = None
+    if tune > 0.0:
+        # simk_kin_autotune_multi on a copy of up to 65,536 cells of each
+        # mechanism (more than the resident lanes, so every candidate fills
+        # its grid), one advance of `tune` seconds per candidate
+        reset()
+        t0 = time.perf_counter()
+        tuned = gm.autotune_multi(dzs, duration=tune, sample_cells=65536).as_dict()
+        tuned["tuning_wall_s"] = time.perf_counter() - t0
+    graph_wall, graph_acc, graph_att, graph_steps = graph_run()
     per_cell = np.concatenate(graph_steps)
-    same = all(np.array_equal(a, b) and np.array_equal(a, c) for a, b, c in zip(graph_steps, eager_steps, sep_steps))
+    same = all(np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d)
+               for a, b, c, d in zip(graph_steps, eager_steps, sep_steps, untuned_steps))
     out.update({"value": graph_acc / graph_wall, "unit": "cell-steps/s", "graph_wall_s": graph_wall,
+                "autotune": tuned, "untuned_graph_value": untuned_acc / untuned_wall,
                 "eager_value": eager_acc / eager_wall, "separate_value": sep_acc / sep_wall,
                 "graph_vs_separate": (graph_acc / graph_wall) / (sep_acc / sep_wall),
                 "attempts_per_s": graph_att / graph_wall, "kernel_ms_per_step": float(np.mean(kms)),

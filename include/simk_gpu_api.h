@@ -138,6 +138,14 @@ typedef struct SimkTuneResult {
     f32 default_ms;      /* the same under the defaults */
 } SimkTuneResult;
 GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out);
+/* The same for the multi-mechanism kernel of simk_kin_set_mechs (ragged
+ * batches): a sample of sample_cells cells of every launches[i] (all when 0),
+ * over the persistent grid and, at the best grid, the load-pipeline depth,
+ * register window, unrolled stages and LU entries in LDS. One lane per cell
+ * is this kernel's only lane layout. Destroys every captured graph. The tuned
+ * settings last until the next simk_kin_set_mechs / gpu_kin_set_mech. */
+GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32 sample_cells, f64 duration,
+                                    SimkTuneResult *out);
 
 /* Statistics of the current mechanism and the last advance. */
 typedef struct SimkKinInfo {

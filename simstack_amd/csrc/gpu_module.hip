@@ -1390,14 +1390,15 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
 // ---- autotuner ------------------------------------------------------------
 // A device copy of cells [0, cells) of a launch's block (compact, own row
 // stride), so candidates run on identical inputs without touching the caller's.
+constexpr f32 kTuneMargin = 0.98f;   // a candidate replaces the best only when 2 % faster
+
 struct TuneSample {
     u8 *mem = nullptr;
     GpuKinLaunch orig{}, work{};
     u64 half = 0;   // bytes of one copy
 };
 
-hipError_t tune_sample(const GpuKinLaunch *L, u32 cells, TuneSample &S) {
-    const Mech &m = g.mech;
+hipError_t tune_sample(const Mech &m, const GpuKinLaunch *L, u32 cells, TuneSample &S) {
     const u64 p = ((u64)cells + 63) & ~63ull;
     const u32 rows[12] = {m.n, 1, 1, 1, 1, 1, m.R, m.n, m.n, 1, 1, 1};
     const f64 *src[12] = {L->y, L->h, L->steps, L->rejects, L->pulses, L->failed,
@@ -1471,7 +1472,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     hipError_t e = hipStreamSynchronize(g.stream);
     if (e != hipSuccess) return err_code(e);
     TuneSample S;
-    e = tune_sample(L, cells, S);
+    e = tune_sample(g.mech, L, cells, S);
     if (duration > 0.0) S.orig.duration = S.work.duration = duration;
     const u32 saved_mode = g.mode, saved_grid = g.grid_override, saved_kind = g.kernel_kind;
     const GenOptions saved_gen = g.gen;
@@ -1500,7 +1501,9 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
         t = t * ((f32)cells / (f32)use);
         out->candidates++;
         if (mode == 0 && grid == 0 && same_gen(opt, saved_gen)) out->default_ms = t;
-        if (t < best) {
+        // a candidate must beat the best so far by 2 %: timing noise never
+        // replaces the default with an equal setting
+        if (t < best * kTuneMargin) {
             best = t;
             best_mode = mode;
             best_grid = grid;
@@ -1683,6 +1686,38 @@ GPU_API s32 simk_copy_device_b64(void *dst, const void *src, u64 bytes, f32 *ms)
     return copy_device(dst, src, bytes, ms, 8);
 }
 
+// Generates, compiles (or loads from the cache) and loads the multi-mechanism
+// kernel of g.multi_mechs under options o.
+hipError_t load_multi(GenOptions o, JitKernel &out) {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<const Mech *> ptrs;
+    for (const Mech &m : g.multi_mechs) ptrs.push_back(&m);
+    o.precision = 0;
+    GenResult gr = generate_multi(ptrs, g.method, resolved(o), kDeviceKinSource);
+    std::vector<char> code;
+    bool cached = false;
+    if (!jit_code(gr, code, cached, g.jit_log)) return hipErrorInvalidImage;
+    hipModule_t mod;
+    hipError_t e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) return e;
+    hipFunction_t fn;
+    e = hipModuleGetFunction(&fn, mod, gr.kernel.c_str());
+    if (e != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return e;
+    }
+    int blocks = 1;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, SIMK_WAVE, 0) != hipSuccess) blocks = 1;
+    out.mod = mod;
+    out.fn = fn;
+    out.ws_slots = gr.ws_slots;
+    out.name = gr.kernel;
+    out.cached = cached;
+    out.blocks_per_cu = blocks;
+    out.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return hipSuccess;
+}
+
 GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCaps *caps, const KinMethod *method,
                                const KinControl *control) {
     if (!g.ready) return err_code(hipErrorNotInitialized);
@@ -1715,33 +1750,8 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
         c.fixed_step = control->fixed_step ? 1 : 0;
         c.clamp_negative = control->clamp_negative ? 1 : 0;
     }
-    auto t0 = std::chrono::steady_clock::now();
-    std::vector<const Mech *> ptrs;
-    for (const Mech &m : g.multi_mechs) ptrs.push_back(&m);
-    GenOptions o = g.gen;
-    o.precision = 0;
-    GenResult gr = generate_multi(ptrs, g.method, resolved(o), kDeviceKinSource);
-    std::vector<char> code;
-    bool cached = false;
-    if (!jit_code(gr, code, cached, g.jit_log)) return err_code(hipErrorInvalidImage);
-    hipModule_t mod;
-    e = hipModuleLoadData(&mod, code.data());
+    e = load_multi(g.gen, g.multi_jit);
     if (e != hipSuccess) return err_code(e);
-    hipFunction_t fn;
-    e = hipModuleGetFunction(&fn, mod, gr.kernel.c_str());
-    if (e != hipSuccess) {
-        (void)hipModuleUnload(mod);
-        return err_code(e);
-    }
-    int blocks = 1;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, SIMK_WAVE, 0) != hipSuccess) blocks = 1;
-    g.multi_jit.mod = mod;
-    g.multi_jit.fn = fn;
-    g.multi_jit.ws_slots = gr.ws_slots;
-    g.multi_jit.name = gr.kernel;
-    g.multi_jit.cached = cached;
-    g.multi_jit.blocks_per_cu = blocks;
-    g.multi_jit.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
     g.multi_ok = true;
     return 0;
 }
@@ -1761,6 +1771,145 @@ GPU_API s32 simk_kin_advance_multi(u32 count, const GpuKinLaunch *launches, u32 
     g.last_ms = ms;
     if (kernel_ms) *kernel_ms = ms;
     return err_code(hipGetLastError());
+}
+
+// Autotuner of the multi-mechanism kernel (ragged batches): the same
+// protocol as simk_kin_autotune on a sample of every mechanism's cells, over
+// the persistent grid and the generator's knobs, the kernel rebuilt (or taken
+// from the cache) for each candidate. One lane per cell is the only lane
+// layout of this kernel.
+GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32 sample_cells, f64 duration,
+                                    SimkTuneResult *out) {
+    if (!g.multi_ok || !launches || !out || count != g.multi_mechs.size()) return err_code(hipErrorInvalidValue);
+    for (u32 i = 0; i < count; ++i)
+        if (!launch_ok(launches + i) || launches[i].precision != 0) return err_code(hipErrorInvalidValue);
+    memset(out, 0, sizeof(*out));
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return err_code(e);
+    destroy_graphs();   // candidates replace the kernel the graphs launch
+    std::vector<TuneSample> S(count);
+    std::vector<GpuKinLaunch> work(count);
+    u32 sampled = 0;
+    for (u32 i = 0; i < count && e == hipSuccess; ++i) {
+        const u32 lanes = launches[i].lane_count ? launches[i].lane_count : launches[i].padded;
+        const u32 cells = sample_cells && sample_cells < lanes ? sample_cells : lanes;
+        e = tune_sample(g.multi_mechs[i], launches + i, cells, S[i]);
+        if (duration > 0.0) S[i].orig.duration = S[i].work.duration = duration;
+        work[i] = S[i].work;
+        sampled += cells;
+    }
+    auto time_it = [&](f32 &best) -> hipError_t {
+        best = 1e30f;
+        for (u32 r = 0; r < 2; ++r) {
+            for (u32 i = 0; i < count; ++i) {
+                hipError_t c = hipMemcpyAsync(S[i].mem + S[i].half, S[i].mem, S[i].half, hipMemcpyDeviceToDevice,
+                                              g.stream);
+                if (c != hipSuccess) return c;
+            }
+            MultiPlan P;
+            hipError_t c = prepare_multi(count, work.data(), 1, P);
+            if (c != hipSuccess) return c;
+            (void)hipEventRecord(g.kin_start, g.stream);
+            c = enqueue_multi(P);
+            if (c != hipSuccess) return c;
+            (void)hipEventRecord(g.kin_stop, g.stream);
+            c = hipEventSynchronize(g.kin_stop);
+            if (c != hipSuccess) return c;
+            f32 ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+            if (ms < best) best = ms;
+        }
+        return hipSuccess;
+    };
+    const u32 saved_grid = g.grid_override;
+    const GenOptions saved_gen = g.gen;
+    const u32 cus = g.cus ? g.cus : 1;
+    f32 best = 1e30f, t = 0.0f;
+    u32 best_grid = 0;
+    GenOptions best_gen = g.gen;
+    auto consider = [&](u32 grid, const GenOptions &opt) -> hipError_t {
+        g.grid_override = grid;
+        if (!same_gen(opt, g.gen)) {
+            JitKernel k;
+            if (load_multi(opt, k) != hipSuccess) return hipSuccess;   // variant does not build: skip it
+            if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
+            g.multi_jit = k;
+            g.gen = opt;
+        }
+        hipError_t r = time_it(t);
+        if (r != hipSuccess) return r;
+        out->candidates++;
+        if (grid == 0 && same_gen(opt, saved_gen)) out->default_ms = t;
+        if (t < best * kTuneMargin) {
+            best = t;
+            best_grid = grid;
+            best_gen = opt;
+        }
+        return hipSuccess;
+    };
+    const u32 grids[] = {0, cus, cus * 3 / 2, cus * 2, cus * 3, cus * 4};
+    for (u32 i = 0; i < sizeof(grids) / sizeof(grids[0]) && e == hipSuccess; ++i) e = consider(grids[i], g.gen);
+    auto knob = [&](auto set) {
+        if (e != hipSuccess) return;
+        GenOptions o = best_gen;
+        set(o);
+        if (!same_gen(o, best_gen)) e = consider(best_grid, o);
+    };
+    knob([](GenOptions &o) { o.batch = 8; });
+    knob([](GenOptions &o) { o.batch = 24; });
+    knob([](GenOptions &o) { o.reg_window = 8; });
+    knob([](GenOptions &o) { o.reg_window = 32; });
+    knob([](GenOptions &o) { o.unroll_stages = 1; });
+    knob([](GenOptions &o) { o.lds_slots = 0; });
+    knob([](GenOptions &o) { o.lds_slots = 80; });
+    knob([](GenOptions &o) { o.lds_slots = 128; });
+    for (TuneSample &x : S)
+        if (x.mem) (void)hipFree(x.mem);
+    if (e == hipSuccess && !same_gen(best_gen, g.gen)) {
+        JitKernel k;
+        e = load_multi(best_gen, k);
+        if (e == hipSuccess) {
+            if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
+            g.multi_jit = k;
+            g.gen = best_gen;
+        }
+    }
+    if (e != hipSuccess) {
+        // leave a consistent kernel: the caller's own options
+        g.grid_override = saved_grid;
+        if (!same_gen(g.gen, saved_gen)) {
+            JitKernel k;
+            if (load_multi(saved_gen, k) == hipSuccess) {
+                if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
+                g.multi_jit = k;
+            }
+            g.gen = saved_gen;
+        }
+        return err_code(e);
+    }
+    if (!g.tuned) {
+        g.tuned = true;
+        g.user_gen = saved_gen;
+        g.user_mode = g.mode;
+        g.user_grid = saved_grid;
+    }
+    g.grid_override = best_grid;
+    out->mode = 0;
+    out->lanes_per_cell = 1;
+    out->waves = best_grid;
+    out->batch = best_gen.batch;
+    out->reg_window = best_gen.reg_window;
+    out->unroll_stages = best_gen.unroll_stages;
+    out->reuse = best_gen.reuse;
+    u32 lds = 0;
+    for (const Mech &m : g.multi_mechs) {
+        const u32 l = lds_entries(m, resolved(best_gen));
+        if (l > lds) lds = l;
+    }
+    out->lds_slots = lds;
+    out->best_ms = best;
+    out->sample_cells = sampled;
+    return 0;
 }
 
 GPU_API s32 simk_graph_capture_multi(u32 count, const GpuKinLaunch *launches, u32 repeats, u32 steps,

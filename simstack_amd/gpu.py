@@ -68,7 +68,7 @@ EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_s
                      "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
                      "simk_graph_destroy", "simk_jit_precompile_multi", "simk_serve_open", "simk_serve_submit",
                      "simk_serve_step", "simk_serve_stats", "simk_serve_close", "simk_kin_set_cache_budget",
-                     "simk_kin_autotune", "simk_plugflow_f64")
+                     "simk_kin_autotune", "simk_kin_autotune_multi", "simk_plugflow_f64")
 
 
 def _sig(lib, name, res, *args):
@@ -122,6 +122,8 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_plugflow_f64", C.c_int32, V, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, V)
     _sig(lib, "simk_kin_set_cache_budget", C.c_int32, C.c_uint64)
     _sig(lib, "simk_kin_autotune", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, C.c_double, P(SimkTuneResult))
+    _sig(lib, "simk_kin_autotune_multi", C.c_int32, C.c_uint32, P(_ffi.GpuKinLaunch), C.c_uint32, C.c_double,
+         P(SimkTuneResult))
     _sig(lib, "simk_serve_open", C.c_int32, P(SimkServeConfig))
     _sig(lib, "simk_serve_submit", C.c_int32, P(SimkServeRequest), P(C.c_uint32))
     _sig(lib, "simk_serve_step", C.c_int32, P(C.c_uint32), C.c_uint32, P(C.c_uint32))
@@ -196,6 +198,14 @@ class GpuModule:
         ms = C.c_float()
         self._check(self.lib.simk_kin_advance_multi(len(dzs), Ls, repeats, C.byref(ms)), "simk_kin_advance_multi")
         return ms.value
+
+    def autotune_multi(self, dzs, duration=0.0, sample_cells=0):
+        """simk_kin_autotune_multi over the blocks dzs (mechanism i = dzs[i])."""
+        Ls = (_ffi.GpuKinLaunch * len(dzs))(*[d.launch_desc(duration) for d in dzs])
+        res = SimkTuneResult()
+        self._check(self.lib.simk_kin_autotune_multi(len(dzs), Ls, sample_cells, duration, C.byref(res)),
+                    "simk_kin_autotune_multi")
+        return res
 
     def capture_multi(self, dzs, duration, repeats, steps):
         Ls = (_ffi.GpuKinLaunch * len(dzs))(*[d.launch_desc(duration) for d in dzs])

@@ -139,3 +139,29 @@ def test_multi_rejects_bad_launches(gm):
     Ls = (_ffi.GpuKinLaunch * 3)(*[d.launch_desc(0.1, precision=2) for d in dzs])
     assert gm.lib.simk_kin_advance_multi(3, Ls, 1, None) != 0
     assert np.all(np.isfinite(dzs[0].z.y))
+
+
+def test_multi_autotune_keeps_bits_and_restores(gm):
+    """simk_kin_autotune_multi times candidates on a copy of the blocks (the
+    callers' rows are untouched), keeps the fastest, and the tuned kernel's
+    results are the oracle's."""
+    blocks = _blocks((120, 300, 260), first=2000)
+    meth, ctl = _set(gm, blocks)
+    work = [z.copy() for _, z in blocks]
+    dzs = [gpu.DeviceZones(gm, z) for z in work]
+    for d in dzs:
+        d.upload()
+    res = gm.autotune_multi(dzs, duration=0.05, sample_cells=128)
+    assert res.candidates >= 6 and res.lanes_per_cell == 1 and res.best_ms > 0.0
+    assert res.best_ms <= res.default_ms
+    for d, (_, z) in zip(dzs, blocks):
+        d.download(mutable_only=False)
+        assert d.z.block.tobytes() == z.block.tobytes()   # tuning ran on copies
+    gm.advance_multi(dzs, 0.25)
+    for d in dzs:
+        d.download(mutable_only=False)
+    for (mech, z), got in zip(blocks, work):
+        want = z.copy()
+        oracle_run(mech, meth, ctl, want, [0.25])
+        assert compare_bits(got, want) == ZERO
+    _set(gm, blocks)   # restores the default settings for the tests after this one

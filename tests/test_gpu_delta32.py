@@ -103,3 +103,23 @@ def test_delta32_refill_with_few_waves(gm, kind):
     _oracle(mech, meth, ctl, b, [0.01])
     assert compare_bits(a, b) == ZERO
     assert b.failed[:400].sum() > 0
+
+
+@pytest.mark.parametrize("lds", [0xFFFFFFFF, 64, 0xFFFFFFFE], ids=["lds_none", "lds64", "lds_max"])
+def test_delta32_generated_lds_shares_keep_bits(gm, lds):
+    """The generated f32 delta kernel with its LU entirely in the workspace, 64
+    entries in LDS, and as many as fit (all 407 of the 40/130 mechanism's LU,
+    f32 entries of 64 lanes in 104 KB of LDS): the same bits as the oracle."""
+    sp, mech, z = make_problem("c3b", 300)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
+    a, b = z.copy(), z.copy()
+    try:
+        gm.set_kernel(1)
+        gm.set_gen_options(lds_slots=lds)
+        info = _gpu(gm, mech, meth, ctl, a, [0.01])
+    finally:
+        gm.set_gen_options()
+        gm.set_kernel(0)
+    _oracle(mech, meth, ctl, b, [0.01])
+    assert info.kernel == 8
+    assert compare_bits(a, b) == ZERO

@@ -128,7 +128,8 @@ def test_graph_replays_match_eager(gm):
 
 
 def test_multi_rejects_bad_launches(gm):
-    """Count mismatch and FP32 precision are refused with an error code."""
+    """Count mismatch and a precision other than FP64 are refused with an
+    error code, by the advance and by the autotuner."""
     blocks = _blocks((64, 64, 64))
     _set(gm, blocks)
     dzs = [gpu.DeviceZones(gm, z.copy()) for _, z in blocks]
@@ -139,6 +140,15 @@ def test_multi_rejects_bad_launches(gm):
     Ls = (_ffi.GpuKinLaunch * 3)(*[d.launch_desc(0.1, precision=2) for d in dzs])
     assert gm.lib.simk_kin_advance_multi(3, Ls, 1, None) != 0
     assert np.all(np.isfinite(dzs[0].z.y))
+    # the multi-mechanism autotuner refuses the same launches
+    res = gpu.SimkTuneResult()
+    assert gm.lib.simk_kin_autotune_multi(3, Ls, 64, 0.01, C.byref(res)) != 0
+    with pytest.raises(gpu.GpuError):
+        gm.autotune_multi(dzs[:2], duration=0.01, sample_cells=64)
+    # misaligned or null copies are refused before any launch
+    ms = C.c_float()
+    assert gm.lib.simk_copy_device_b64(None, None, 1024, C.byref(ms)) != 0
+    assert gm.lib.simk_copy_device_b64(dzs[0].d_block + 8, dzs[1].d_block, 1024, C.byref(ms)) != 0
 
 
 def test_multi_autotune_keeps_bits_and_restores(gm):

@@ -99,13 +99,13 @@ typedef struct SimkGenOptions {
     u32 waves_per_eu;    /* amdgpu_waves_per_eu hint (0: none) */
     u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
     u32 reg_window;      /* rows back a finished row stays in registers (0: 16 in FP64, 8 mixed; ~0u: all) */
-    u32 batch;           /* loads issued this many statements ahead (0: 16) */
+    u32 batch;           /* loads issued this many statements ahead (0: 16; 24 in unrefined mixed precision) */
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 96; ~0u: never) */
     u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default: 160 f64 entries
-                            = 80 KiB per wave, 320 f32 in mixed precision, 160 f32 in f32 delta lanes, none for
-                            refined mixed kernels; ~0u: none; at most 160 KiB) */
+                            = 80 KiB per wave; 160 f32 = 40 KiB in unrefined mixed precision and f32 delta
+                            lanes; none for refined mixed kernels; ~0u: none; at most 160 KiB) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
@@ -126,7 +126,14 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
  * Every candidate computes the same bits; only time differs. The tuned
  * settings last until the next gpu_kin_set_mech / simk_kin_set_mechs, which
- * restores the caller's own. */
+ * restores the caller's own.
+ * With L->precision 2 it tunes the mixed-precision kernel instead, for later
+ * mixed advances only (FP64 settings untouched): LU entries in LDS (the
+ * default, 320, 240, 200), each at the cache-aware grid and at as many waves
+ * per CU as that LDS share allows, then at the best pair the load-pipeline
+ * depth (8, 12, 20, 24, 28, 32), register window (4, 16), rate-constant reuse
+ * (none, 48) and unrolled stages. simk_kin_set_gen_options drops these
+ * settings. Precision 1 is refused (hipErrorInvalidValue). */
 typedef struct SimkTuneResult {
     u32 mode;            /* 0 one lane per cell, 1 one wave per cell */
     u32 lanes_per_cell;  /* 1 or 64 */

@@ -114,11 +114,15 @@ u32 lds_entries(const Mech &m, const GenOptions &o) {
 }
 
 // Per-lane workspace layout (f64 slots): FP64 [LU entries lds.. | K stages |
-// f | k copy | src copy]; mixed [K stages | f | refinement rhs | FP32 LU
-// entries lds.. (two per slot) | k copy | src copy]. The last stage's vector
-// stays in registers.
+// f | k copy | src copy]; refined mixed [K stages | f | refinement rhs | FP32
+// LU entries lds.. (two per slot) | k copy | src copy]; unrefined mixed [f |
+// FP32 region: LU entries lds.., then the K stages | k copy | src copy]. The
+// last stage's vector stays in registers. Without refinement every stage
+// vector is the f64 image of an FP32 solve result, so storing it as FP32 keeps
+// every bit.
 struct Layout {
-    u32 lds, koff, foff, rhsoff, lu32off, kcp, scp, slots;
+    u32 lds, koff, foff, rhsoff, lu32off, k32off, kcp, scp, slots;
+    bool k32;
 };
 
 static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
@@ -127,11 +131,27 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
     const u32 sweeps = mixed ? o.sweeps : 0;
     const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;
     L.lds = lds_entries(m, o);
-    L.koff = mixed ? 0 : m.nnz - L.lds;
-    L.foff = L.koff + stored_stages * m.n;
-    L.rhsoff = L.foff + m.n;
-    L.lu32off = L.rhsoff + (sweeps ? m.n : 0);
-    const u32 end = mixed ? L.lu32off + (m.nnz - L.lds + 1) / 2 : L.foff + m.n;
+    L.k32 = mixed && sweeps == 0;
+    L.k32off = 0;
+    u32 end;
+    if (!mixed) {
+        L.koff = m.nnz - L.lds;
+        L.foff = L.koff + stored_stages * m.n;
+        L.rhsoff = L.lu32off = L.foff + m.n;
+        end = L.foff + m.n;
+    } else if (!L.k32) {
+        L.koff = 0;
+        L.foff = stored_stages * m.n;
+        L.rhsoff = L.foff + m.n;
+        L.lu32off = L.rhsoff + m.n;
+        end = L.lu32off + (m.nnz - L.lds + 1) / 2;
+    } else {
+        L.koff = 0;
+        L.foff = 0;
+        L.rhsoff = L.lu32off = m.n;
+        L.k32off = m.nnz - L.lds;
+        end = L.lu32off + (L.k32off + stored_stages * m.n + 1) / 2;
+    }
     L.kcp = end;
     L.scp = L.kcp + m.R;
     L.slots = L.scp + m.n;
@@ -143,13 +163,17 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
 // every grid with less LDS per wave by 20-100 %; more than 80 KiB leaves one
 // wave per CU and loses. f32 delta lanes (fewer VGPRs): 40 KiB per wave and
 // four waves per CU run the DBD air workload 1.9x faster than its whole LU
-// (54 KiB, three waves per CU). The refined mixed kernel is slower with LDS
-// entries.
+// (54 KiB, three waves per CU). Unrefined mixed precision (48/150, 262,144
+// cells): 160 f32 entries and four waves per CU with 24-deep load pipelining
+// run 1.5x faster than 320 entries and two waves with 16. The refined mixed
+// kernel is slower with LDS entries.
 u32 auto_lds(const GenOptions &o) {
     if (o.precision == 1) return 160u;
-    if (o.precision == 2) return o.sweeps ? 0u : 320u;
+    if (o.precision == 2) return o.sweeps ? 0u : 160u;
     return 160u;
 }
+
+u32 auto_batch(const GenOptions &o) { return o.precision == 2 && o.sweeps == 0 ? 24u : 16u; }
 
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
@@ -190,10 +214,16 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const u32 kcp = Lo.kcp, scp = Lo.scp, ws_slots = Lo.slots;
     w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
-    w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u;\n", koff, foff, rhsoff,
-      me.stages);
+    w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u, K32OFF = %u;\n", koff, foff,
+      rhsoff, me.stages, Lo.k32off);
+    // jac_factor also evaluates f(y) and leaves it in the f slots.
+    w("    static constexpr bool kFunInJac = true;\n");
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
+    // srcnz: the lane's cell has a source row with a bit set. A wave whose
+    // lanes all have all-zero (+0.0) rows starts f from the literal +0.0,
+    // which is what loading those rows would give.
+    w("    simk_u32 srcnz;\n");
     if (mixed) w("    SimkRows32 w32;\n");
     if (lds) {
         const char *T = mixed ? "float" : "double", *A = mixed ? "simk_lds32" : "simk_lds";
@@ -215,8 +245,11 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
     w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
     w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) WS(%u + r, kr.ld(r));\n", kcp);
-    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) WS(%u + s, sr.ld(s));\n", scp);
+    w("        srcnz = 0;\n");
+    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) { const double v = sr.ld(s); srcnz |= "
+      "(simk_u32)(__builtin_bit_cast(simk_u64, v) != 0); WS(%u + s, v); }\n", scp);
     w("    }\n");
+    w("    SIMK_DEV bool any_src() const { return __builtin_amdgcn_ballot_w64(srcnz != 0) != 0; }\n");
     w("    static SIMK_DEV constexpr simk_u32 n() { return N; }\n");
     w("    SIMK_DEV Vec vec(int) const { return Vec(); }\n");
     w("    SIMK_DEV double K(simk_u32 r) const { return ws.ld(%u + r); }\n", kcp);
@@ -228,17 +261,24 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         w("    SIMK_DEV float W32(simk_u32 i) const { return w32.ld(i); }\n");
         w("    SIMK_DEV void W32S(simk_u32 i, float v) const { w32.st(i, v); }\n");
     }
-    w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(KOFF + st * N + s); }\n");
-    w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
-    w("        if (st + 1 >= STAGES) return;\n");
-    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(KOFF + st * N + s, x.v[s]);\n    }\n");
+    if (Lo.k32) {
+        w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return (double)W32(K32OFF + st * N + s); }\n");
+        w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
+        w("        if (st + 1 >= STAGES) return;\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) W32S(K32OFF + st * N + s, (float)x.v[s]);\n    }\n");
+    } else {
+        w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(KOFF + st * N + s); }\n");
+        w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
+        w("        if (st + 1 >= STAGES) return;\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(KOFF + st * N + s, x.v[s]);\n    }\n");
+    }
     {
         // v += c K_j, loads software-pipelined like the solve's.
         std::vector<Op> ops;
         char buf[96];
         for (u32 s2 = 0; s2 < n; ++s2) {
             Op op;
-            snprintf(buf, sizeof(buf), "W(KOFF + j * N + %u)", s2);
+            snprintf(buf, sizeof(buf), Lo.k32 ? "(double)W32(K32OFF + j * N + %u)" : "W(KOFF + j * N + %u)", s2);
             op.load = buf;
             snprintf(buf, sizeof(buf), "v.v[%u] = simk_fma(c, %%s, v.v[%u]);", s2, s2);
             op.stmt = buf;
@@ -249,23 +289,21 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         emit_ops(w, ops, o.batch, "        ", uid);
         w("    }\n");
     }
+    // fjac: f(y) as jac_factor left it in the f slots. The saved f stays in
+    // workspace slots: keeping it in registers was measured slower (the
+    // register pressure costs more than the slot traffic it saves).
+    w("    SIMK_DEV void fjac(Vec &x) const {\n");
+    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
     w("    SIMK_DEV void fsave(const Vec &x) const {\n");
     w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(FOFF + s, x.v[s]);\n    }\n");
     w("    SIMK_DEV void fload(Vec &x) const {\n");
     w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
 
-    // fun
-    {
+    // fun: one op per reaction, rate = k * y_a [* y_b], then its net entries
+    // in order (kin_codegen.cpp:58-72 order)
+    auto fun_ops = [&](const char *yv, const char *fv) {
         std::vector<Op> ops;
         char buf[96];
-        for (u32 s = 0; s < n; ++s) {
-            Op op;
-            snprintf(buf, sizeof(buf), "S(%u)", s);
-            op.load = buf;
-            snprintf(buf, sizeof(buf), "f.v[%u] = %%s;", s);
-            op.stmt = buf;
-            ops.push_back(op);
-        }
         for (u32 r = 0; r < m.R; ++r) {
             const Reaction &x = m.rx[r];
             Op op;
@@ -274,22 +312,28 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             op.immutable = true;
             std::string st = "{ const double rate = %s";
             for (u32 p = 0; p < x.reactant_count; ++p) {
-                snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[p]);
+                snprintf(buf, sizeof(buf), " * %s.v[%u]", yv, x.reactant[p]);
                 st += buf;
             }
             st += ";";
             for (u32 e = 0; e < x.net_count; ++e) {
-                snprintf(buf, sizeof(buf), " f.v[%u] = simk_fma(%d.0, rate, f.v[%u]);", x.net_species[e],
-                         (int)x.net_coef[e], x.net_species[e]);
+                snprintf(buf, sizeof(buf), " %s.v[%u] = simk_fma(%d.0, rate, %s.v[%u]);", fv, x.net_species[e],
+                         (int)x.net_coef[e], fv, x.net_species[e]);
                 st += buf;
             }
             st += " }";
             op.stmt = st;
             ops.push_back(op);
         }
+        return ops;
+    };
+    const char *src_init = "        if (any_src()) {\n#pragma unroll\n            for (simk_u32 s = 0; s < N; ++s) %s.v[s] = S(s);\n"
+                           "        } else {\n#pragma unroll\n            for (simk_u32 s = 0; s < N; ++s) %s.v[s] = 0.0;\n        }\n";
+    {
         w("    SIMK_DEV void fun(const Vec &y, Vec &f) const {\n");
+        w(src_init, "f", "f");
         u32 uid = 0;
-        emit_ops(w, ops, o.batch, "        ", uid, o.reuse);
+        emit_ops(w, fun_ops("y", "f"), o.batch, "        ", uid, o.reuse);
         w("    }\n");
     }
 
@@ -355,6 +399,12 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 snprintf(buf, sizeof(buf), "double a%u = 0.0;", idx);
                 stmt(buf);
             }
+            // f_i of f(y): the source, then every reaction with a net entry
+            // on species i in ascending order, exactly fun's sequence for
+            // species i (so f_i has fun's bits). Those are the reactions this
+            // row's -J visits, so they share its rate-constant loads.
+            snprintf(buf, sizeof(buf), "double f%u = anysrc ? S(%u) : 0.0;", i, i);
+            stmt(buf);
             // -J contributions, per entry in (r, p, e) order: one op per
             // reaction touching row i, its reactants' terms in p order.
             for (u32 r = 0; r < m.R; ++r) {
@@ -362,12 +412,24 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 bool any = false;
                 for (u32 e = 0; e < x.net_count; ++e)
                     if (x.net_species[e] == i) any = true;
-                if (!any || x.reactant_count == 0) continue;
+                if (!any) continue;
                 Op op;
                 snprintf(buf, sizeof(buf), "K(%u)", r);
                 op.load = buf;
                 op.immutable = true;
                 std::string st = "{ const double k = %s;";
+                st += " { const double rate = k";
+                for (u32 p = 0; p < x.reactant_count; ++p) {
+                    snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[p]);
+                    st += buf;
+                }
+                st += ";";
+                for (u32 e = 0; e < x.net_count; ++e) {
+                    if (x.net_species[e] != i) continue;
+                    snprintf(buf, sizeof(buf), " f%u = simk_fma(%d.0, rate, f%u);", i, (int)x.net_coef[e], i);
+                    st += buf;
+                }
+                st += " }";
                 for (u32 p = 0; p < x.reactant_count; ++p) {
                     st += " { const double d = k";
                     for (u32 q = 0; q < x.reactant_count; ++q)
@@ -435,9 +497,12 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 stmt(buf);
                 for (u32 idx : row) stmt(lu_st(idx, "a"));
             }
+            snprintf(buf, sizeof(buf), "WS(FOFF + %u, f%u);", i, i);
+            stmt(buf);
         }
         barrier();
         w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
+        w("        const bool anysrc = any_src();\n");
         if (mixed) w("        const float gf = (float)g;\n");
         u32 uid = 0;
         emit_ops(w, ops, o.batch, "        ", uid, o.reuse);

@@ -16,7 +16,8 @@ struct GenOptions {
     u32 fence_rows = 0;      // 1: re-read finished rows from the workspace (fence per row); 0: keep them in registers
     u32 waves_per_eu = 0;    // amdgpu_waves_per_eu lower bound (0: compiler default)
     u32 unroll_stages = 0;   // unroll the stage loop (bigger code, constant stage offsets)
-    u32 batch = 16;          // loads issued this many statements ahead (software pipelining)
+    u32 batch = kAutoBatch;  // loads issued this many statements ahead (software pipelining;
+                             // kAutoBatch: resolved by auto_batch before generation)
     u32 precision = 0;       // 0: FP64 throughout; 1: f32 delta lanes; 2: FP32 Jacobian/LU + FP64 residual refinement
     u32 sweeps = 1;          // precision 2: refinement sweeps per stage solve
     u32 reg_window = kDefaultWindow;   // register mode: finished rows read from registers up to this many
@@ -26,6 +27,7 @@ struct GenOptions {
                                 // (kAutoLds: resolved by auto_lds before generation)
     static constexpr u32 kDefaultWindow = 0xFFFF;
     static constexpr u32 kAutoLds = 0xFFFF;
+    static constexpr u32 kAutoBatch = 0xFFFF;
 };
 
 struct GenResult {
@@ -39,10 +41,15 @@ struct GenResult {
 // and to 160 KiB per wave).
 u32 lds_entries(const Mech &m, const GenOptions &o);
 
-// The default LU entries in LDS: 80 KiB per wave (160 f64, or 320 f32 in
-// mixed precision), so two waves share a CU's 160 KiB; 40 KiB (160 f32) for
-// f32 delta lanes; none for mixed-precision kernels with refinement sweeps.
+// The default LU entries in LDS: 80 KiB per wave (160 f64), so two waves
+// share a CU's 160 KiB; 40 KiB (160 f32) for f32 delta lanes and unrefined
+// mixed-precision kernels (four waves per CU); none for mixed-precision
+// kernels with refinement sweeps.
 u32 auto_lds(const GenOptions &o);
+
+// The default load-pipeline depth: 16 statements, 24 for unrefined
+// mixed-precision kernels (their FP32 loads hold one register each).
+u32 auto_batch(const GenOptions &o);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

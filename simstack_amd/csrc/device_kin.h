@@ -240,7 +240,17 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
     // reference's loops (kin_solver.h:238-259); only the loop nesting differs.
     auto stage = [&](const simk_u32 st) {
         const simk_u32 packed = st * (st - 1) / 2;
-        if (st == 0 || me.eval(st)) {
+        bool have_f = false;
+        if constexpr (P::kFunInJac) {
+            // jac_factor left f(y) in the f slots, where a later stage that
+            // reuses stage 0's f also finds it
+            if (st == 0) {
+                p.fjac(x);
+                have_f = true;
+            }
+        }
+        if (have_f) {
+        } else if (st == 0 || me.eval(st)) {
             typename P::Vec pt = p.vec(2);
 #pragma unroll
             for (simk_u32 s = 0; s < n; ++s) pt.set(s, y.get(s));

@@ -32,6 +32,7 @@ using namespace simk;
 // the reference's KinTableOps (src/sim/kinetics/kin_solver.h:152-161).
 
 struct TableOps {
+    static constexpr bool kFunInJac = false;
     using Vec = SimkWsVec;
     const SimkDevTables *T;
     simk_u32 n_, R_, nnz_;
@@ -255,6 +256,11 @@ struct State {
     bool jit_ok = false;
     JitKernel jit_mixed;       // compiled on the first mixed-precision launch
     bool jit_mixed_ok = false;
+    // Mixed-precision settings chosen by the autotuner (simk_kin_autotune with
+    // precision 2); until then mixed launches use gen and grid_override.
+    bool mix_tuned = false;
+    GenOptions mix_gen;
+    u32 mix_grid = 0;
     JitKernel jit_d32;         // generated f32 delta kernel, compiled on the first precision-1 launch
     bool jit_d32_ok = false, jit_d32_tried = false;
     u32 sweeps = 1;
@@ -636,6 +642,7 @@ hipError_t ensure_ws(u64 bytes) {
 // Options with the automatic LDS share resolved.
 GenOptions resolved(GenOptions o) {
     if (o.lds_slots == GenOptions::kAutoLds) o.lds_slots = auto_lds(o);
+    if (o.batch == GenOptions::kAutoBatch) o.batch = auto_batch(o);
     return o;
 }
 
@@ -668,7 +675,17 @@ bool load_generated(const GenOptions &opt, JitKernel &out) {
     return ok;
 }
 
+void unload_mixed() {
+    if (g.jit_mixed.mod) (void)hipModuleUnload(g.jit_mixed.mod);
+    g.jit_mixed = JitKernel();
+    g.jit_mixed_ok = false;
+}
+
 void untune() {
+    if (g.mix_tuned) {
+        g.mix_tuned = false;
+        unload_mixed();
+    }
     if (!g.tuned) return;
     g.gen = g.user_gen;
     g.mode = g.user_mode;
@@ -722,8 +739,8 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
 // Infinity Cache of the MI355X). Every workspace access then stays on chip:
 // for the 48/150 mechanism (449 KB of workspace per wave) 512 waves run 24 %
 // faster than the 1,024 that fit the register file. At least one wave per CU.
-u32 resident_waves(int per_cu, u64 ws_bytes_per_wave) {
-    if (g.grid_override) return g.grid_override;
+u32 resident_waves(int per_cu, u64 ws_bytes_per_wave, u32 grid) {
+    if (grid) return grid;
     const u32 cus = g.cus ? g.cus : 1;
     u64 r = (u64)(per_cu > 0 ? per_cu : 1) * cus;
     if (g.cache_budget && ws_bytes_per_wave) {
@@ -854,7 +871,7 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     }
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const u32 resident = resident_waves(gen ? g.jit_d32.blocks_per_cu : g.blocks_per_cu_d32,
-                                        (u64)slots * SIMK_WAVE * slot_bytes);
+                                        (u64)slots * SIMK_WAVE * slot_bytes, g.grid_override);
     const u32 waves = needed < resident ? needed : resident;
     hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * slot_bytes);
     if (e != hipSuccess) return e;
@@ -915,7 +932,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     if (L->precision == 2) {
         // Mixed precision exists only as a generated kernel.
         if (!g.jit_mixed_ok) {
-            GenOptions mo = g.gen;
+            GenOptions mo = g.mix_tuned ? g.mix_gen : g.gen;
             mo.precision = 2;
             mo.sweeps = g.sweeps;
             g.jit_mixed_ok = load_generated(mo, g.jit_mixed);
@@ -934,7 +951,8 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
         return hipErrorInvalidValue;
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const int per_cu = use_gen ? jk->blocks_per_cu : g.blocks_per_cu_table;
-    const u32 resident = resident_waves(per_cu, (u64)slots * SIMK_WAVE * 8);
+    const u32 grid = L->precision == 2 && g.mix_tuned ? g.mix_grid : g.grid_override;
+    const u32 resident = resident_waves(per_cu, (u64)slots * SIMK_WAVE * 8, grid);
     const u32 waves = needed < resident ? needed : resident;
     hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(double));
     if (e != hipSuccess) return e;
@@ -1015,7 +1033,7 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.unroll_stages = options->unroll_stages;
     o.reg_window = options->reg_window == 0 ? GenOptions::kDefaultWindow
                                             : (options->reg_window == 0xFFFFFFFFu ? 0u : options->reg_window);
-    o.batch = options->batch == 0 ? 16u : options->batch;
+    o.batch = options->batch == 0 ? GenOptions::kAutoBatch : options->batch;
     o.precision = options->precision;
     o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
     o.reuse = options->reuse == 0 ? 96u : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
@@ -1067,7 +1085,7 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
     }
     P.slots = g.multi_jit.ws_slots;
     if ((u64)P.slots * SIMK_WAVE * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
-    const u32 resident = resident_waves(g.multi_jit.blocks_per_cu, (u64)P.slots * SIMK_WAVE * 8);
+    const u32 resident = resident_waves(g.multi_jit.blocks_per_cu, (u64)P.slots * SIMK_WAVE * 8, g.grid_override);
     P.waves = (u32)(total_waves < resident ? total_waves : resident);
     if (P.waves == 0) P.waves = 1;
     const u64 ws_bytes = (u64)P.waves * P.slots * SIMK_WAVE * sizeof(double);
@@ -1125,7 +1143,7 @@ SimkMultiShape simk_multi_shape() {
     if (!g.multi_ok) return S;
     S.count = (u32)g.multi_mechs.size();
     S.slots = g.multi_jit.ws_slots;
-    S.resident_waves = resident_waves(g.multi_jit.blocks_per_cu, (u64)S.slots * SIMK_WAVE * 8);
+    S.resident_waves = resident_waves(g.multi_jit.blocks_per_cu, (u64)S.slots * SIMK_WAVE * 8, g.grid_override);
     for (u32 i = 0; i < S.count; ++i) {
         S.n[i] = g.multi_mechs[i].n;
         S.R[i] = g.multi_mechs[i].R;
@@ -1375,14 +1393,13 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.reg_window = n.reg_window;
     g.gen.reuse = n.reuse;
     g.gen.lds_slots = n.lds_slots;
-    if (g.tuned) g.user_gen = g.gen;
     g.gen.batch = n.batch;
+    // explicit options are what a later untune() restores
+    if (g.tuned) g.user_gen = g.gen;
     const u32 sw = n.sweeps;
-    if (sw != g.sweeps && g.jit_mixed.mod) {
-        (void)hipModuleUnload(g.jit_mixed.mod);
-        g.jit_mixed = JitKernel();
-        g.jit_mixed_ok = false;
-    }
+    // explicit options replace tuned mixed-precision settings
+    if (sw != g.sweeps || g.mix_tuned) unload_mixed();
+    g.mix_tuned = false;
     g.sweeps = sw;
     return 0;
 }
@@ -1463,8 +1480,134 @@ bool same_gen(const GenOptions &a, const GenOptions &b) {
            a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots;
 }
 
+// Mixed precision (precision 2): the same protocol over the mixed kernel's
+// own knobs. The LU entries held in LDS set how many waves fit a CU, so each
+// LDS share is timed at the cache-aware grid and at every per-CU wave count
+// that share allows; then, at the best pair, load-pipeline depth, register
+// window, rate-constant reuse and stage unrolling one at a time. The winner
+// serves later mixed advances only; FP64 settings are untouched.
+s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResult *out) {
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return err_code(e);
+    TuneSample S;
+    e = tune_sample(g.mech, L, cells, S);
+    if (duration > 0.0) S.orig.duration = S.work.duration = duration;
+    S.work.precision = S.orig.precision = 2;
+    const bool saved_tuned = g.mix_tuned;
+    const GenOptions saved_gen = g.mix_tuned ? g.mix_gen : g.gen;
+    const u32 saved_grid = g.mix_tuned ? g.mix_grid : g.grid_override;
+    const u32 cus = g.cus ? g.cus : 1;
+    f32 best = 1e30f, t = 0.0f;
+    GenOptions best_gen = saved_gen, loaded = saved_gen;
+    u32 best_grid = saved_grid;
+    bool have_loaded = false;
+    g.mix_tuned = true;
+    auto consider = [&](u32 grid, const GenOptions &opt) -> hipError_t {
+        if (!have_loaded || !same_gen(opt, loaded)) {
+            GenOptions mo = opt;
+            mo.precision = 2;
+            mo.sweeps = g.sweeps;
+            JitKernel k;
+            if (!load_generated(mo, k)) return hipSuccess;   // variant does not build: skip it
+            unload_mixed();
+            g.jit_mixed = k;
+            g.jit_mixed_ok = true;
+            loaded = opt;
+            have_loaded = true;
+        }
+        g.mix_gen = opt;
+        g.mix_grid = grid;
+        hipError_t r = tune_time(S, 2, t);
+        if (r != hipSuccess) return r;
+        out->candidates++;
+        if (grid == saved_grid && same_gen(opt, saved_gen)) out->default_ms = t;
+        if (t < best * kTuneMargin) {
+            best = t;
+            best_gen = opt;
+            best_grid = grid;
+        }
+        return hipSuccess;
+    };
+    // 1. LDS share x grid
+    e = e == hipSuccess ? consider(saved_grid, saved_gen) : e;
+    const u32 ldss[] = {GenOptions::kAutoLds, 320, 240, 200};
+    for (u32 l : ldss) {
+        GenOptions o = saved_gen;
+        o.lds_slots = l;
+        GenOptions ro = o;
+        ro.precision = 2;
+        ro.sweeps = g.sweeps;
+        const u32 entries = lds_entries(g.mech, resolved(ro));
+        // waves per CU that this LDS share allows (160 KiB per CU, f32 entries of 64 lanes)
+        const u32 per_cu = entries ? (u32)((160u * 1024u) / (entries * 4u * SIMK_WAVE)) : 4u;
+        const u32 grids[] = {0, cus * (per_cu < 4 ? per_cu : 4)};
+        for (u32 gr : grids)
+            if (e == hipSuccess) e = consider(gr, o);
+    }
+    // 2. generator knobs at the best pair, one at a time
+    const u32 batches[] = {8, 12, 20, 24, 28, 32}, windows[] = {4, 16}, reuses[] = {0u, 48};
+    const GenOptions base = best_gen;
+    const u32 base_grid = best_grid;
+    for (u32 b : batches)
+        if (e == hipSuccess) {
+            GenOptions o = base;
+            o.batch = b;
+            e = consider(base_grid, o);
+        }
+    for (u32 w : windows)
+        if (e == hipSuccess) {
+            GenOptions o = best_gen;
+            o.reg_window = w;
+            e = consider(best_grid, o);
+        }
+    for (u32 r : reuses)
+        if (e == hipSuccess) {
+            GenOptions o = best_gen;
+            o.reuse = r;
+            e = consider(best_grid, o);
+        }
+    if (e == hipSuccess) {
+        GenOptions o = best_gen;
+        o.unroll_stages = 1;
+        e = consider(best_grid, o);
+    }
+    if (S.mem) (void)hipFree(S.mem);
+    if (e != hipSuccess) {
+        g.mix_tuned = saved_tuned;
+        g.mix_gen = saved_gen;
+        g.mix_grid = saved_grid;
+        unload_mixed();
+        return err_code(e);
+    }
+    g.mix_gen = best_gen;
+    g.mix_grid = best_grid;
+    if (!have_loaded || !same_gen(best_gen, loaded)) unload_mixed();   // reloaded on the next mixed launch
+    GenOptions ro = best_gen;
+    ro.precision = 2;
+    ro.sweeps = g.sweeps;
+    out->mode = 0;
+    out->lanes_per_cell = 1;
+    out->waves = best_grid;
+    out->batch = resolved(ro).batch;
+    out->reg_window = best_gen.reg_window;
+    out->unroll_stages = best_gen.unroll_stages;
+    out->reuse = best_gen.reuse;
+    out->lds_slots = lds_entries(g.mech, resolved(ro));
+    out->best_ms = best;
+    out->sample_cells = cells;
+    return 0;
+}
+
 GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
-    if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision != 0) return err_code(hipErrorInvalidValue);
+    if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision == 1 || L->precision > 2)
+        return err_code(hipErrorInvalidValue);
+    if (L->precision == 2) {
+        memset(out, 0, sizeof(*out));
+        const u32 lanes = L->lane_count ? L->lane_count : L->padded;
+        const u32 cells = sample_cells && sample_cells < lanes ? sample_cells : lanes;
+        if (cells == 0) return err_code(hipErrorInvalidValue);
+        return autotune_mixed(L, cells, duration, out);
+    }
     memset(out, 0, sizeof(*out));
     const u32 lanes = L->lane_count ? L->lane_count : L->padded;
     const u32 cells = sample_cells && sample_cells < lanes ? sample_cells : lanes;
@@ -1587,7 +1730,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     out->mode = best_mode;
     out->lanes_per_cell = best_mode ? SIMK_WAVE : 1;
     out->waves = best_grid;
-    out->batch = best_gen.batch;
+    out->batch = resolved(best_gen).batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
     out->reuse = best_gen.reuse;
@@ -1897,7 +2040,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     out->mode = 0;
     out->lanes_per_cell = 1;
     out->waves = best_grid;
-    out->batch = best_gen.batch;
+    out->batch = resolved(best_gen).batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
     out->reuse = best_gen.reuse;

@@ -320,10 +320,11 @@ class DeviceZones:
         L.duration = duration
         return L
 
-    def autotune(self, duration=0.0, sample_cells=0):
+    def autotune(self, duration=0.0, sample_cells=0, precision=0):
         """simk_kin_autotune on this block (a device copy of it); the chosen
-        settings stay in effect for later FP64 advances."""
-        L = self.launch_desc(duration or 1.0)
+        settings stay in effect for later advances of that precision (0 FP64,
+        2 mixed)."""
+        L = self.launch_desc(duration or 1.0, precision=precision)
         res = SimkTuneResult()
         self.gm._check(self.gm.lib.simk_kin_autotune(C.byref(L), sample_cells, duration, C.byref(res)),
                        "simk_kin_autotune")

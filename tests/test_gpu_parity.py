@@ -264,3 +264,45 @@ def test_autotune_picks_settings_and_keeps_bits(gm):
     b = z.subset(lanes)
     oracle_run(mech, meth, ctl, b, [0.1])
     assert compare_bits(a.subset(lanes), b) == ZERO
+
+
+def test_mixed_autotune_keeps_bits_and_fp64(gm):
+    """simk_kin_autotune with precision 2 times the mixed kernel's LDS shares,
+    grids and generator knobs on a device copy; every candidate computes the
+    same bits, so the tuned mixed advance equals the untuned one bit for bit,
+    and FP64 advances afterwards are still bit-identical to the oracle."""
+    from simstack_amd import gpu
+    sp, mech, z = make_problem("c2", 4096)
+    meth, ctl = problem.method(2), problem.control()
+    runs = []
+    for tune in (False, True):
+        a = z.copy()
+        gm.arena_reset()
+        gm.set_kernel(0)
+        gm.set_gen_options(sweeps=0xFFFFFFFF)
+        gm.set_mech(mech, meth, ctl)
+        dz = gpu.DeviceZones(gm, a)
+        dz.upload()
+        if tune:
+            res = dz.autotune(duration=0.02, sample_cells=2048, precision=2)
+            assert res.candidates >= 12 and 0 < res.best_ms <= res.default_ms
+            assert res.lds_slots in (160, 200, 240, 320)
+            dz.download(mutable_only=False)
+            assert a.block.tobytes() == z.block.tobytes()     # tuning ran on a copy
+        dz.advance(0.2, precision=2)
+        assert gm.kin_info().kernel == 3
+        dz.download(mutable_only=False)
+        runs.append(a)
+        if tune:
+            # FP64 on the same module is unaffected by the mixed settings
+            f = z.copy()
+            df = gpu.DeviceZones(gm, f)
+            df.upload()
+            df.advance(0.2)
+            df.download(mutable_only=False)
+    gm.set_gen_options()
+    assert runs[0].block.tobytes() == runs[1].block.tobytes()
+    lanes = np.arange(0, 4096, 97)
+    b = z.subset(lanes)
+    oracle_run(mech, meth, ctl, b, [0.2])
+    assert compare_bits(f.subset(lanes), b) == ZERO

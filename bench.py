@@ -55,7 +55,8 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration):
 
 
 def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
-                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0, lds_slots=0, perturb=False):
+                 kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0, lds_slots=0, perturb=False,
+                 reuse=0):
     """Advances `cells` cells of the workload `steps` x 1 s from the seeded initial
     state (after `warmup` untimed seconds on a copy). Returns timings and the block.
     perturb: every initial concentration moved up by one ulp."""
@@ -72,16 +73,18 @@ def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precis
     gm.set_kernel(kernel)
     gm.lib.gpu_kin_set_mode(0)
     gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps,
-                       lds_slots=lds_slots)
+                       lds_slots=lds_slots, reuse=reuse)
     gm.set_grid(grid)
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z0.copy())
     dz.upload()
     tuned = None
     if tune > 0.0:
-        # a sample of at least the resident lanes (1,024 waves of 64), so every candidate fills its grid
+        # The whole workload: a sample of about one cell per resident lane
+        # (65,536) is dominated by its slowest cells and picked settings that
+        # are slower at 262,144 cells.
         t0 = time.perf_counter()
-        tuned = dz.autotune(duration=tune, sample_cells=min(cells, 65536), precision=precision).as_dict()
+        tuned = dz.autotune(duration=tune, sample_cells=cells, precision=precision).as_dict()
         tuned["tuning_wall_s"] = time.perf_counter() - t0
     if warmup:
         dz.advance(1.0, repeats=warmup, precision=precision)
@@ -457,7 +460,7 @@ def main():
             "steps_ratio": pr["accepted"] / f64r["accepted"]}
         out["mixed"] = {"workload": f"same mechanism, {n} cells, {args.steps} x 1 s fused", **mix,
                         "note": "errors vs the FP64 GPU path, which is bit-identical to the FP64 oracle; tuned runs "
-                                f"autotune on a 65,536-cell sample for {tune:g} s before timing"}
+                                f"autotune on the whole workload for {tune:g} s per candidate before timing"}
     if not args.no_ragged:
         out["ragged"] = ragged_block(gpu, problem, args.ragged_steps)
     if not args.no_serve:

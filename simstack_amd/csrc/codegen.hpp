@@ -22,12 +22,14 @@ struct GenOptions {
     u32 sweeps = 1;          // precision 2: refinement sweeps per stage solve
     u32 reg_window = kDefaultWindow;   // register mode: finished rows read from registers up to this many
                                        // rows back (0: all; kDefaultWindow: 16 in FP64, 8 in mixed precision)
-    u32 reuse = 96;          // a rate constant loaded within this many statements is reused, not reloaded (0: never)
+    u32 reuse = kAutoReuse;  // a rate constant loaded within this many statements is reused, not reloaded (0: never;
+                             // kAutoReuse: resolved by auto_reuse before generation)
     u32 lds_slots = kAutoLds;   // the first this many LU entries of each lane live in LDS, not the workspace
                                 // (kAutoLds: resolved by auto_lds before generation)
     static constexpr u32 kDefaultWindow = 0xFFFF;
     static constexpr u32 kAutoLds = 0xFFFF;
     static constexpr u32 kAutoBatch = 0xFFFF;
+    static constexpr u32 kAutoReuse = 0xFFFF;
 };
 
 struct GenResult {
@@ -50,6 +52,10 @@ u32 auto_lds(const GenOptions &o);
 // The default load-pipeline depth: 16 statements, 24 for unrefined
 // mixed-precision kernels (their FP32 loads hold one register each).
 u32 auto_batch(const GenOptions &o);
+
+// The default rate-constant reuse window: 96 statements, 1000 (every earlier
+// load of the pass) for unrefined mixed-precision kernels.
+u32 auto_reuse(const GenOptions &o);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

@@ -643,6 +643,7 @@ hipError_t ensure_ws(u64 bytes) {
 GenOptions resolved(GenOptions o) {
     if (o.lds_slots == GenOptions::kAutoLds) o.lds_slots = auto_lds(o);
     if (o.batch == GenOptions::kAutoBatch) o.batch = auto_batch(o);
+    if (o.reuse == GenOptions::kAutoReuse) o.reuse = auto_reuse(o);
     return o;
 }
 
@@ -1036,7 +1037,7 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.batch = options->batch == 0 ? GenOptions::kAutoBatch : options->batch;
     o.precision = options->precision;
     o.sweeps = options->sweeps == 0 ? 1u : (options->sweeps == 0xFFFFFFFFu ? 0u : options->sweeps);
-    o.reuse = options->reuse == 0 ? 96u : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
+    o.reuse = options->reuse == 0 ? GenOptions::kAutoReuse : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
     o.lds_slots = options->lds_slots == 0 ? GenOptions().lds_slots
                                           : (options->lds_slots == 0xFFFFFFFFu ? 0u : options->lds_slots);
     return o;
@@ -1545,7 +1546,7 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
             if (e == hipSuccess) e = consider(gr, o);
     }
     // 2. generator knobs at the best pair, one at a time
-    const u32 batches[] = {8, 12, 20, 24, 28, 32}, windows[] = {4, 16}, reuses[] = {0u, 48};
+    const u32 batches[] = {8, 12, 20, 24, 28, 32}, windows[] = {4, 16}, reuses[] = {0u, 48, 96, 384};
     const GenOptions base = best_gen;
     const u32 base_grid = best_grid;
     for (u32 b : batches)
@@ -1591,7 +1592,7 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
     out->batch = resolved(ro).batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
-    out->reuse = best_gen.reuse;
+    out->reuse = resolved(ro).reuse;
     out->lds_slots = lds_entries(g.mech, resolved(ro));
     out->best_ms = best;
     out->sample_cells = cells;
@@ -1678,7 +1679,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.unroll_stages = u;
             e = consider(0, best_grid, o);
         }
-    const u32 reuses[] = {0, 48};
+    const u32 reuses[] = {0, 48, 384};
     for (u32 r : reuses)
         if (e == hipSuccess) {
             GenOptions o = best_gen;
@@ -1733,7 +1734,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     out->batch = resolved(best_gen).batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
-    out->reuse = best_gen.reuse;
+    out->reuse = resolved(best_gen).reuse;
     out->lds_slots = lds_entries(g.mech, resolved(best_gen));
     out->best_ms = best;
     out->sample_cells = cells;
@@ -2043,7 +2044,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     out->batch = resolved(best_gen).batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
-    out->reuse = best_gen.reuse;
+    out->reuse = resolved(best_gen).reuse;
     u32 lds = 0;
     for (const Mech &m : g.multi_mechs) {
         const u32 l = lds_entries(m, resolved(best_gen));

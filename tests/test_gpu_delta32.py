@@ -123,3 +123,32 @@ def test_delta32_generated_lds_shares_keep_bits(gm, lds):
     _oracle(mech, meth, ctl, b, [0.01])
     assert info.kernel == 8
     assert compare_bits(a, b) == ZERO
+
+
+def test_delta32_repeats_after_a_previous_call(gm):
+    """One advance, then simk_kin_advance_n(2): the same bits as three single
+    advances and as the oracle, at the tolerance where half the cells fail in
+    f32 (rtol 1e-6). Repeats that start from a previous call's state are run
+    as separate launches in delta lanes."""
+    sp, mech, z = make_problem("c2", 512)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    gm.arena_reset()
+    gm.set_kernel(1)
+    gm.set_mech(mech, meth, ctl)
+    da, db = gpu.DeviceZones(gm, a), gpu.DeviceZones(gm, b)
+    da.upload()
+    db.upload()
+    da.advance(0.2, precision=1)
+    da.advance(0.2, precision=1, repeats=2)
+    for _ in range(3):
+        db.advance(0.2, precision=1)
+    da.download(mutable_only=False)
+    db.download(mutable_only=False)
+    gm.set_kernel(0)
+    assert a.block.tobytes() == b.block.tobytes()
+    assert 0 < a.failed[:512].sum() < 512
+    ok = np.nonzero(a.failed[:512] == 0)[0][:4]
+    c = z.subset(ok)
+    _oracle(mech, meth, ctl, c, [0.2, 0.2, 0.2])
+    assert compare_bits(a.subset(ok), c) == ZERO

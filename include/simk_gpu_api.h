@@ -102,11 +102,13 @@ typedef struct SimkGenOptions {
     u32 batch;           /* loads issued this many statements ahead (0: 16; 24 in unrefined mixed precision) */
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
-    u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 96, 1000 in unrefined mixed
-                            precision; ~0u: never) */
+    u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 384 in one-mechanism FP64
+                            kernels, 1000 in unrefined mixed precision, else 96; ~0u: never) */
     u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default: 160 f64 entries
                             = 80 KiB per wave; 160 f32 = 40 KiB in unrefined mixed precision and f32 delta
                             lanes; none for refined mixed kernels; ~0u: none; at most 160 KiB) */
+    u32 fun_in_jac;      /* 0: default (on); 1: the -J pass also evaluates f(y) for stage 0 (one rate-constant
+                            pass fewer per attempt); ~0u: stage 0 evaluates f(y) itself */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
@@ -122,7 +124,7 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * keeps the fastest for later FP64 advances of the current mechanism:
  *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
  *   2. at the best grid, the generator's load-pipeline depth (8, 24), register
- *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48, 384)
+ *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48, 96)
  *      and LU entries in LDS (none, 80, 128, 160), one knob at a time;
  *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
  * Every candidate computes the same bits; only time differs. The tuned

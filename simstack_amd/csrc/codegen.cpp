@@ -177,8 +177,15 @@ u32 auto_batch(const GenOptions &o) { return o.precision == 2 && o.sweeps == 0 ?
 
 // Unrefined mixed precision, 48/150 at 262,144 cells: reusing every earlier
 // rate-constant load of a pass (the -J pass then loads 152 instead of 274)
-// is 2 % faster than a 96-statement window, with no extra registers.
-u32 auto_reuse(const GenOptions &o) { return o.precision == 2 && o.sweeps == 0 ? 1000u : 96u; }
+// is 2 % faster than a 96-statement window, with no extra registers. FP64,
+// one mechanism: 384 statements, 3 % faster than 96. The multi-mechanism
+// kernel keeps 96 (larger windows spill there and run 25 % slower).
+u32 auto_reuse(const GenOptions &o, bool multi) {
+    if (o.precision == 2 && o.sweeps == 0) return 1000u;
+    return o.precision == 0 && !multi ? 384u : 96u;
+}
+
+u32 auto_fun_in_jac(const GenOptions &) { return 1u; }
 
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
@@ -221,8 +228,9 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
     w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u, K32OFF = %u;\n", koff, foff,
       rhsoff, me.stages, Lo.k32off);
-    // jac_factor also evaluates f(y) and leaves it in the f slots.
-    w("    static constexpr bool kFunInJac = true;\n");
+    // jac_factor also evaluates f(y) and leaves it in the f slots (fij).
+    const bool fij = o.fun_in_jac != 0;
+    w("    static constexpr bool kFunInJac = %s;\n", fij ? "true" : "false");
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
     // srcnz: the lane's cell has a source row with a bit set. A wave whose
@@ -408,8 +416,10 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             // on species i in ascending order, exactly fun's sequence for
             // species i (so f_i has fun's bits). Those are the reactions this
             // row's -J visits, so they share its rate-constant loads.
-            snprintf(buf, sizeof(buf), "double f%u = anysrc ? S(%u) : 0.0;", i, i);
-            stmt(buf);
+            if (fij) {
+                snprintf(buf, sizeof(buf), "double f%u = anysrc ? S(%u) : 0.0;", i, i);
+                stmt(buf);
+            }
             // -J contributions, per entry in (r, p, e) order: one op per
             // reaction touching row i, its reactants' terms in p order.
             for (u32 r = 0; r < m.R; ++r) {
@@ -417,24 +427,26 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 bool any = false;
                 for (u32 e = 0; e < x.net_count; ++e)
                     if (x.net_species[e] == i) any = true;
-                if (!any) continue;
+                if (!any || (!fij && x.reactant_count == 0)) continue;
                 Op op;
                 snprintf(buf, sizeof(buf), "K(%u)", r);
                 op.load = buf;
                 op.immutable = true;
                 std::string st = "{ const double k = %s;";
-                st += " { const double rate = k";
-                for (u32 p = 0; p < x.reactant_count; ++p) {
-                    snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[p]);
-                    st += buf;
+                if (fij) {
+                    st += " { const double rate = k";
+                    for (u32 p = 0; p < x.reactant_count; ++p) {
+                        snprintf(buf, sizeof(buf), " * y.v[%u]", x.reactant[p]);
+                        st += buf;
+                    }
+                    st += ";";
+                    for (u32 e = 0; e < x.net_count; ++e) {
+                        if (x.net_species[e] != i) continue;
+                        snprintf(buf, sizeof(buf), " f%u = simk_fma(%d.0, rate, f%u);", i, (int)x.net_coef[e], i);
+                        st += buf;
+                    }
+                    st += " }";
                 }
-                st += ";";
-                for (u32 e = 0; e < x.net_count; ++e) {
-                    if (x.net_species[e] != i) continue;
-                    snprintf(buf, sizeof(buf), " f%u = simk_fma(%d.0, rate, f%u);", i, (int)x.net_coef[e], i);
-                    st += buf;
-                }
-                st += " }";
                 for (u32 p = 0; p < x.reactant_count; ++p) {
                     st += " { const double d = k";
                     for (u32 q = 0; q < x.reactant_count; ++q)
@@ -502,12 +514,14 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 stmt(buf);
                 for (u32 idx : row) stmt(lu_st(idx, "a"));
             }
-            snprintf(buf, sizeof(buf), "WS(FOFF + %u, f%u);", i, i);
-            stmt(buf);
+            if (fij) {
+                snprintf(buf, sizeof(buf), "WS(FOFF + %u, f%u);", i, i);
+                stmt(buf);
+            }
         }
         barrier();
         w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
-        w("        const bool anysrc = any_src();\n");
+        if (fij) w("        const bool anysrc = any_src();\n");
         if (mixed) w("        const float gf = (float)g;\n");
         u32 uid = 0;
         emit_ops(w, ops, o.batch, "        ", uid, o.reuse);

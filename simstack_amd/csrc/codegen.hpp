@@ -26,10 +26,13 @@ struct GenOptions {
                              // kAutoReuse: resolved by auto_reuse before generation)
     u32 lds_slots = kAutoLds;   // the first this many LU entries of each lane live in LDS, not the workspace
                                 // (kAutoLds: resolved by auto_lds before generation)
+    u32 fun_in_jac = kAutoFunInJac;   // 1: the -J pass also evaluates f(y) (one rate-constant pass fewer);
+                                      // 0: stage 0 evaluates it (kAutoFunInJac: auto_fun_in_jac)
     static constexpr u32 kDefaultWindow = 0xFFFF;
     static constexpr u32 kAutoLds = 0xFFFF;
     static constexpr u32 kAutoBatch = 0xFFFF;
     static constexpr u32 kAutoReuse = 0xFFFF;
+    static constexpr u32 kAutoFunInJac = 0xFFFF;
 };
 
 struct GenResult {
@@ -53,9 +56,13 @@ u32 auto_lds(const GenOptions &o);
 // mixed-precision kernels (their FP32 loads hold one register each).
 u32 auto_batch(const GenOptions &o);
 
-// The default rate-constant reuse window: 96 statements, 1000 (every earlier
-// load of the pass) for unrefined mixed-precision kernels.
-u32 auto_reuse(const GenOptions &o);
+// The default rate-constant reuse window: 384 statements for one-mechanism
+// FP64 kernels, 1000 (every earlier load of the pass) for unrefined
+// mixed-precision kernels, 96 otherwise (multi: the multi-mechanism kernel).
+u32 auto_reuse(const GenOptions &o, bool multi = false);
+
+// The default for f(y) in the -J pass: on.
+u32 auto_fun_in_jac(const GenOptions &o);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

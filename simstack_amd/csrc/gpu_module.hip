@@ -640,10 +640,19 @@ hipError_t ensure_ws(u64 bytes) {
 // Generates, compiles (or loads from the cache) and loads one kernel variant
 // of the current mechanism.
 // Options with the automatic LDS share resolved.
+GenOptions resolved(GenOptions o);
+
+// Options of the multi-mechanism kernel with the automatic values resolved.
+GenOptions resolved_multi(GenOptions o) {
+    if (o.reuse == GenOptions::kAutoReuse) o.reuse = auto_reuse(o, true);
+    return resolved(o);
+}
+
 GenOptions resolved(GenOptions o) {
     if (o.lds_slots == GenOptions::kAutoLds) o.lds_slots = auto_lds(o);
     if (o.batch == GenOptions::kAutoBatch) o.batch = auto_batch(o);
     if (o.reuse == GenOptions::kAutoReuse) o.reuse = auto_reuse(o);
+    if (o.fun_in_jac == GenOptions::kAutoFunInJac) o.fun_in_jac = auto_fun_in_jac(o);
     return o;
 }
 
@@ -1053,6 +1062,7 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.reuse = options->reuse == 0 ? GenOptions::kAutoReuse : (options->reuse == 0xFFFFFFFFu ? 0u : options->reuse);
     o.lds_slots = options->lds_slots == 0 ? GenOptions().lds_slots
                                           : (options->lds_slots == 0xFFFFFFFFu ? 0u : options->lds_slots);
+    o.fun_in_jac = options->fun_in_jac == 0 ? GenOptions::kAutoFunInJac : (options->fun_in_jac == 0xFFFFFFFFu ? 0u : 1u);
     return o;
 }
 
@@ -1407,6 +1417,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.reg_window = n.reg_window;
     g.gen.reuse = n.reuse;
     g.gen.lds_slots = n.lds_slots;
+    g.gen.fun_in_jac = n.fun_in_jac;
     g.gen.batch = n.batch;
     // explicit options are what a later untune() restores
     if (g.tuned) g.user_gen = g.gen;
@@ -1491,7 +1502,8 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes) {
 bool same_gen(const GenOptions &a, const GenOptions &b) {
     return a.fence_rows == b.fence_rows && a.waves_per_eu == b.waves_per_eu && a.unroll_stages == b.unroll_stages &&
            a.batch == b.batch && a.precision == b.precision && a.sweeps == b.sweeps &&
-           a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots;
+           a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots &&
+           a.fun_in_jac == b.fun_in_jac;
 }
 
 // Mixed precision (precision 2): the same protocol over the mixed kernel's
@@ -1692,7 +1704,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.unroll_stages = u;
             e = consider(0, best_grid, o);
         }
-    const u32 reuses[] = {0, 48, 384};
+    const u32 reuses[] = {0, 48, 96};
     for (u32 r : reuses)
         if (e == hipSuccess) {
             GenOptions o = best_gen;
@@ -1805,7 +1817,7 @@ GPU_API s32 simk_jit_precompile_multi(u32 count, const void *const *mechs, const
     o.precision = 0;
     KinMethod me = *method;
     me.name = nullptr;
-    GenResult gr = generate_multi(ptrs, me, resolved(o), kDeviceKinSource);
+    GenResult gr = generate_multi(ptrs, me, resolved_multi(o), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
     std::string text;
@@ -1850,7 +1862,7 @@ hipError_t load_multi(GenOptions o, JitKernel &out) {
     std::vector<const Mech *> ptrs;
     for (const Mech &m : g.multi_mechs) ptrs.push_back(&m);
     o.precision = 0;
-    GenResult gr = generate_multi(ptrs, g.method, resolved(o), kDeviceKinSource);
+    GenResult gr = generate_multi(ptrs, g.method, resolved_multi(o), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
     if (!jit_code(gr, code, cached, g.jit_log)) return hipErrorInvalidImage;
@@ -2054,13 +2066,13 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     out->mode = 0;
     out->lanes_per_cell = 1;
     out->waves = best_grid;
-    out->batch = resolved(best_gen).batch;
+    out->batch = resolved_multi(best_gen).batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
-    out->reuse = resolved(best_gen).reuse;
+    out->reuse = resolved_multi(best_gen).reuse;
     u32 lds = 0;
     for (const Mech &m : g.multi_mechs) {
-        const u32 l = lds_entries(m, resolved(best_gen));
+        const u32 l = lds_entries(m, resolved_multi(best_gen));
         if (l > lds) lds = l;
     }
     out->lds_slots = lds;

@@ -105,8 +105,9 @@ typedef struct SimkGenOptions {
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 384 in one-mechanism FP64
                             kernels, 1000 in unrefined mixed precision, else 96; ~0u: never) */
     u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default: 160 f64 entries
-                            = 80 KiB per wave; 160 f32 = 40 KiB in unrefined mixed precision and f32 delta
-                            lanes; none for refined mixed kernels; ~0u: none; at most 160 KiB) */
+                            = 80 KiB per wave; 200 f32 = 50 KiB in unrefined mixed precision; 160 f32 =
+                            40 KiB in f32 delta lanes; none for refined mixed kernels; ~0u: none; at most
+                            160 KiB) */
     u32 fun_in_jac;      /* 0: default (on); 1: the -J pass also evaluates f(y) for stage 0 (one rate-constant
                             pass fewer per attempt); ~0u: stage 0 evaluates f(y) itself */
 } SimkGenOptions;
@@ -132,7 +133,7 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * restores the caller's own.
  * With L->precision 2 it tunes the mixed-precision kernel instead, for later
  * mixed advances only (FP64 settings untouched): LU entries in LDS (the
- * default, 320, 240, 200), each at the cache-aware grid and at as many waves
+ * default, 320, 240, 160), each at the cache-aware grid and at as many waves
  * per CU as that LDS share allows, then at the best pair the load-pipeline
  * depth (8, 12, 20, 24, 28, 32), register window (4, 16), rate-constant reuse
  * (none, 48, 96, 384) and unrolled stages. simk_kin_set_gen_options drops these

@@ -164,12 +164,13 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
 // wave per CU and loses. f32 delta lanes (fewer VGPRs): 40 KiB per wave and
 // four waves per CU run the DBD air workload 1.9x faster than its whole LU
 // (54 KiB, three waves per CU). Unrefined mixed precision (48/150, 262,144
-// cells): 160 f32 entries and four waves per CU with 24-deep load pipelining
-// run 1.5x faster than 320 entries and two waves with 16. The refined mixed
-// kernel is slower with LDS entries.
+// cells, 24-deep load pipelining, rate constants reused over the pass): 200
+// f32 entries and three waves per CU run 8 % faster than 160 entries and
+// four waves, and 1.5x faster than 320 entries and two waves. The refined
+// mixed kernel is slower with LDS entries.
 u32 auto_lds(const GenOptions &o) {
     if (o.precision == 1) return 160u;
-    if (o.precision == 2) return o.sweeps ? 0u : 160u;
+    if (o.precision == 2) return o.sweeps ? 0u : 200u;
     return 160u;
 }
 

@@ -47,9 +47,9 @@ struct GenResult {
 u32 lds_entries(const Mech &m, const GenOptions &o);
 
 // The default LU entries in LDS: 80 KiB per wave (160 f64), so two waves
-// share a CU's 160 KiB; 40 KiB (160 f32) for f32 delta lanes and unrefined
-// mixed-precision kernels (four waves per CU); none for mixed-precision
-// kernels with refinement sweeps.
+// share a CU's 160 KiB; 40 KiB (160 f32) for f32 delta lanes (four waves per
+// CU); 50 KiB (200 f32) for unrefined mixed-precision kernels (three waves
+// per CU); none for mixed-precision kernels with refinement sweeps.
 u32 auto_lds(const GenOptions &o);
 
 // The default load-pipeline depth: 16 statements, 24 for unrefined

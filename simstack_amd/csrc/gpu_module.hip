@@ -1556,7 +1556,7 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
     };
     // 1. LDS share x grid
     e = e == hipSuccess ? consider(saved_grid, saved_gen) : e;
-    const u32 ldss[] = {GenOptions::kAutoLds, 320, 240, 200};
+    const u32 ldss[] = {GenOptions::kAutoLds, 320, 240, 160};
     for (u32 l : ldss) {
         GenOptions o = saved_gen;
         o.lds_slots = l;

@@ -214,12 +214,13 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
     untuned_wall, untuned_acc, _, untuned_steps = graph_run()
     tuned = None
     if tune > 0.0:
-        # simk_kin_autotune_multi on a copy of up to 65,536 cells of each
-        # mechanism (more than the resident lanes, so every candidate fills
-        # its grid), one advance of `tune` seconds per candidate
+        # simk_kin_autotune_multi on a copy of the whole workload, one advance
+        # of `tune` seconds per candidate (a sample of 65,536 cells per
+        # mechanism picked settings slower at full size: its slowest cells
+        # dominate)
         reset()
         t0 = time.perf_counter()
-        tuned = gm.autotune_multi(dzs, duration=tune, sample_cells=65536).as_dict()
+        tuned = gm.autotune_multi(dzs, duration=tune, sample_cells=0).as_dict()
         tuned["tuning_wall_s"] = time.perf_counter() - t0
     graph_wall, graph_acc, graph_att, graph_steps = graph_run()
     per_cell = np.concatenate(graph_steps)

@@ -98,7 +98,8 @@ typedef struct SimkGenOptions {
     u32 fence_rows;      /* 1: factored rows re-read from the workspace; 0: kept in registers */
     u32 waves_per_eu;    /* amdgpu_waves_per_eu hint (0: none) */
     u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
-    u32 reg_window;      /* rows back a finished row stays in registers (0: 16 in FP64, 8 mixed; ~0u: all) */
+    u32 reg_window;      /* rows back a finished row stays in registers (0: 16 in one-mechanism FP64 kernels, 8 in
+                            mixed precision and the multi-mechanism kernel; ~0u: all) */
     u32 batch;           /* loads issued this many statements ahead (0: 16; 24 in unrefined mixed precision) */
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */

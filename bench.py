@@ -366,8 +366,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 262,144-cell mixed vs FP64 comparison")
     ap.add_argument("--mixed-cells", type=int, default=262144)
-    ap.add_argument("--mixed-tune", type=float, default=1.0,
-                    help="simulated seconds per autotuner candidate for the tuned FP64 and mixed runs (0: none)")
+    ap.add_argument("--mixed-tune", type=float, default=0.0,
+                    help="also autotune FP64 and mixed precision on the whole workload, this many simulated seconds "
+                         "per candidate, and compare the tuned runs (0: the defaults only; they are the tuners' "
+                         "picks on this workload)")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generated, 2 table walk")
     ap.add_argument("--separate", action="store_true",
                     help="one gpu_kin_advance launch per step instead of one fused simk_kin_advance_n launch")
@@ -455,7 +457,7 @@ def main():
         rate = lambda r: r["accepted"] / r["wall"]
         mix = {"fp64_value": rate(f64r), "fp64_tuned_value": rate(f64t), "fp64_tuned": f64t["tuned"]}
         for name, sw, tune_s in (("mixed_norefine", 0xFFFFFFFF, 0.0), ("mixed_norefine_tuned", 0xFFFFFFFF, tune),
-                                 ("mixed_1sweep", 1, 0.0)):
+                                 ("mixed_1sweep", 1, 0.0), ("mixed_2sweeps", 2, 0.0)):
             if name.endswith("_tuned") and tune <= 0.0:
                 continue
             mr = mixed_run(2, sw, tune_s)

@@ -963,14 +963,16 @@ GenResult generate_multi(const std::vector<const Mech *> &ms, const KinMethod &m
         if (g.mech_slots.back() > g.ws_slots) g.ws_slots = g.mech_slots.back();
     }
     emit_method(w, me, o);
-    // Waves start on the mechanism whose wave range holds their index and move
-    // on to the others once its queue is empty, so no SIMD idles while any
+    // A wave starts at the position of MA.seq whose wave range holds its
+    // index and visits the mechanisms' queues in seq order from there
+    // (wrapping), moving on once a queue is empty, so no SIMD idles while any
     // mechanism still has cells.
     w("extern \"C\" __global__ void __launch_bounds__(SIMK_WAVE) %s(SimkMultiArgs MA) {\n", name);
     w("    GenMethod me;\n    simk_u32 first = 0;\n");
-    w("    for (simk_u32 m = 1; m < MA.count; ++m)\n        if (blockIdx.x >= MA.start[m]) first = m;\n");
+    w("    for (simk_u32 k = 1; k < MA.count; ++k)\n        if (blockIdx.x >= MA.start[k]) first = k;\n");
     w("    for (simk_u32 pass = 0; pass < MA.count; ++pass) {\n");
-    w("        simk_u32 m = first + pass;\n        if (m >= MA.count) m -= MA.count;\n");
+    w("        simk_u32 k = first + pass;\n        if (k >= MA.count) k -= MA.count;\n");
+    w("        const simk_u32 m = MA.seq[k];\n");
     w("        switch (m) {\n");
     for (size_t i = 0; i < ms.size(); ++i)
         w("            case %u: simk_persistent_body<GenOps%u, GenMethod, true>(MA.a[%u], me); break;\n", (u32)i, (u32)i,

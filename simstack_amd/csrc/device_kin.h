@@ -549,5 +549,6 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
 struct SimkMultiArgs {
     SimkKinArgs a[4];
     simk_u32 count;
-    simk_u32 start[4];
+    simk_u32 start[4];       // first wave that starts at position k of seq
+    simk_u32 seq[4];         // the order in which a wave visits the mechanisms' queues
 };

@@ -222,6 +222,7 @@ struct JitKernel {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     u32 ws_slots = 0;
+    std::vector<u32> mech_slots;   // multi-mechanism kernels: the slots each mechanism touches
     int blocks_per_cu = 1;
     std::string name;
     bool cached = false;
@@ -765,6 +766,60 @@ u32 resident_waves(int per_cu, u64 ws_bytes_per_wave, u32 grid) {
     return (u32)r;
 }
 
+// Algorithmic flops of one Rodas3-like step attempt of a mechanism (FMA = 2):
+// four f evaluations, -J, the LU factor, four solves and the vector work.
+f64 attempt_flops(const Mech &m) {
+    f64 fun = 0.0, jac = 0.0;
+    for (const Reaction &x : m.rx) {
+        const f64 r = x.reactant_count, e = x.net_count;
+        fun += r + 2.0 * e;
+        jac += r * (r - 1.0) + 2.0 * r * e;
+    }
+    const f64 lu = (f64)m.lu_div.size() + 2.0 * (f64)m.lu_sub.size() + 2.0 * m.n;
+    const f64 solve = 2.0 * ((f64)m.nnz - m.n) + m.n;
+    return 4.0 * fun + jac + lu + 4.0 * solve + 31.0 * m.n + 30.0;
+}
+
+// Longest processing time first across mechanisms: every wave starts at the
+// mechanism whose cells cost the most per attempt and visits the rest in
+// decreasing cost, so the costliest cells start at once on as many lanes as
+// they need instead of queueing behind a small share of the waves and ending
+// the launch on a tail of them. (Splitting the waves in proportion to an
+// estimated work share put 6 of 512 waves on the 1,000 cells of the 60/200
+// mechanism: 19 % slower on the 60/200 + 40/130 + 24/70 batch.)
+void multi_schedule(SimkMultiArgs &MA, u32 count) {
+    f64 cost[4];
+    for (u32 i = 0; i < count; ++i) {
+        cost[i] = attempt_flops(g.multi_mechs[i]);
+        MA.seq[i] = i;
+    }
+    for (u32 i = 1; i < count; ++i)   // stable insertion sort, decreasing cost
+        for (u32 j = i; j > 0 && cost[MA.seq[j]] > cost[MA.seq[j - 1]]; --j) {
+            const u32 t = MA.seq[j];
+            MA.seq[j] = MA.seq[j - 1];
+            MA.seq[j - 1] = t;
+        }
+    MA.start[0] = 0;
+    for (u32 k = 1; k < 4; ++k) MA.start[k] = 0xFFFFFFFFu;
+}
+
+// Workspace bytes per wave the multi-mechanism kernel actually touches: each
+// wave's slots are sized for the largest mechanism, but a wave running a
+// smaller one touches only that mechanism's slots. Weighted by the work
+// share of each mechanism (weight[i]), this is what the cache budget must
+// hold. (Sizing by the largest mechanism left 416 of 512 resident waves on
+// the 60/200 + 40/130 + 24/70 batch, 84 % of whose work is 24/70: 24 % slower.)
+u64 multi_wave_bytes(const f64 *weight, u32 count) {
+    const JitKernel &k = g.multi_jit;
+    f64 acc = 0.0, wsum = 0.0;
+    for (u32 i = 0; i < count && i < k.mech_slots.size(); ++i) {
+        acc += weight[i] * (f64)k.mech_slots[i];
+        wsum += weight[i];
+    }
+    const f64 slots = wsum > 0.0 ? acc / wsum : (f64)k.ws_slots;
+    return (u64)(slots * SIMK_WAVE * 8.0) + 1;
+}
+
 // Method and control in the lane type T (lane_splat's rounding for f32, base_lane.h:98).
 template <class T>
 SimkTConsts<T> t_consts() {
@@ -1099,7 +1154,7 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
     memset(&P.MA, 0, sizeof(P.MA));
     P.MA.count = count;
     u64 total_waves = 0;
-    f64 weight[4], wsum = 0.0;
+    f64 weight[4];   // work share of each mechanism, for the cache footprint
     for (u32 i = 0; i < count; ++i) {
         const GpuKinLaunch *L = Ls + i;
         if (!launch_ok(L) || L->precision != 0) return hipErrorInvalidValue;
@@ -1109,11 +1164,10 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
         const u32 lanes = L->lane_count ? L->lane_count : L->padded;
         total_waves += (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
         weight[i] = (f64)lanes * (f64)(m.nnz + 4 * m.n);   // per-cell cost ~ LU plus stage vectors
-        wsum += weight[i];
     }
     P.slots = g.multi_jit.ws_slots;
     if ((u64)P.slots * SIMK_WAVE * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
-    const u32 resident = resident_waves(g.multi_jit.blocks_per_cu, (u64)P.slots * SIMK_WAVE * 8, g.grid_override);
+    const u32 resident = resident_waves(g.multi_jit.blocks_per_cu, multi_wave_bytes(weight, count), g.grid_override);
     P.waves = (u32)(total_waves < resident ? total_waves : resident);
     if (P.waves == 0) P.waves = 1;
     const u64 ws_bytes = (u64)P.waves * P.slots * SIMK_WAVE * sizeof(double);
@@ -1137,15 +1191,13 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
         P.ws = g.ws;
         P.queue = g.queue;
     }
-    f64 acc = 0.0;
     for (u32 i = 0; i < count; ++i) {
         const GpuKinLaunch *L = Ls + i;
         const u32 lanes = L->lane_count ? L->lane_count : L->padded;
         P.MA.a[i] = make_args(L, lanes, g.multi_mechs[i], g.multi_ctl[i], P.slots, repeats, P.queue + i, 1);
         P.MA.a[i].ws = P.ws;
-        P.MA.start[i] = (u32)(acc / (wsum > 0 ? wsum : 1.0) * P.waves);
-        acc += weight[i];
     }
+    multi_schedule(P.MA, count);
     return hipSuccess;
 }
 
@@ -1171,7 +1223,10 @@ SimkMultiShape simk_multi_shape() {
     if (!g.multi_ok) return S;
     S.count = (u32)g.multi_mechs.size();
     S.slots = g.multi_jit.ws_slots;
-    S.resident_waves = resident_waves(g.multi_jit.blocks_per_cu, (u64)S.slots * SIMK_WAVE * 8, g.grid_override);
+    // serving: equal cells expected of every mechanism
+    f64 weight[4] = {0.0, 0.0, 0.0, 0.0};
+    for (u32 i = 0; i < S.count && i < 4; ++i) weight[i] = (f64)(g.multi_mechs[i].nnz + 4 * g.multi_mechs[i].n);
+    S.resident_waves = resident_waves(g.multi_jit.blocks_per_cu, multi_wave_bytes(weight, S.count), g.grid_override);
     for (u32 i = 0; i < S.count; ++i) {
         S.n[i] = g.multi_mechs[i].n;
         S.R[i] = g.multi_mechs[i].R;
@@ -1191,17 +1246,13 @@ hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *
     P.waves = waves;
     P.ws = ws;
     P.queue = queue;
-    f64 weight[4], wsum = 0.0;
     for (u32 i = 0; i < count; ++i) {
         const GpuKinLaunch *L = Ls + i;
         const Mech &m = g.multi_mechs[i];
         const u64 rows = m.R > m.n ? m.R : m.n;
         if (!launch_ok(L) || lanes[i] > L->padded || rows * L->padded * 8ull >= (1ull << 31))
             return hipErrorInvalidValue;
-        weight[i] = (f64)lanes[i] * (f64)(m.nnz + 4 * m.n);
-        wsum += weight[i];
     }
-    f64 acc = 0.0;
     for (u32 i = 0; i < count; ++i) {
         P.MA.a[i] = make_args(Ls + i, lanes[i], g.multi_mechs[i], g.multi_ctl[i], P.slots, repeats, queue + i, 1);
         P.MA.a[i].ws = ws;
@@ -1213,9 +1264,8 @@ hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *
             P.MA.a[i].carry = slice->carry[i];
             P.MA.a[i].carry_count = queue + 4 + i;
         }
-        P.MA.start[i] = (u32)(acc / (wsum > 0 ? wsum : 1.0) * P.waves);
-        acc += weight[i];
     }
+    multi_schedule(P.MA, count);
     return enqueue_multi(P);
 }
 
@@ -1884,6 +1934,7 @@ hipError_t load_multi(GenOptions o, JitKernel &out) {
     out.mod = mod;
     out.fn = fn;
     out.ws_slots = gr.ws_slots;
+    out.mech_slots = gr.mech_slots;
     out.name = gr.kernel;
     out.cached = cached;
     out.blocks_per_cu = blocks;

@@ -100,7 +100,8 @@ typedef struct SimkGenOptions {
     u32 unroll_stages;   /* 1: unroll the Rosenbrock stage loop */
     u32 reg_window;      /* rows back a finished row stays in registers (0: 16 in one-mechanism FP64 kernels, 8 in
                             mixed precision and the multi-mechanism kernel; ~0u: all) */
-    u32 batch;           /* loads issued this many statements ahead (0: 16; 24 in unrefined mixed precision) */
+    u32 batch;           /* loads issued this many statements ahead (0: 24; 16 in f32 delta lanes and refined mixed
+                            precision) */
     u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 384 in one-mechanism FP64
@@ -125,7 +126,7 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * block is untouched) under candidate settings and
  * keeps the fastest for later FP64 advances of the current mechanism:
  *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
- *   2. at the best grid, the generator's load-pipeline depth (8, 24), register
+ *   2. at the best grid, the generator's load-pipeline depth (8, 16), register
  *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48, 96)
  *      and LU entries in LDS (none, 80, 128, 160), one knob at a time;
  *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.

@@ -174,7 +174,7 @@ u32 auto_lds(const GenOptions &o) {
     return 160u;
 }
 
-u32 auto_batch(const GenOptions &o) { return o.precision == 2 && o.sweeps == 0 ? 24u : 16u; }
+u32 auto_batch(const GenOptions &o) { return o.precision == 1 || (o.precision == 2 && o.sweeps) ? 16u : 24u; }
 
 // Unrefined mixed precision, 48/150 at 262,144 cells: reusing every earlier
 // rate-constant load of a pass (the -J pass then loads 152 instead of 274)

@@ -52,8 +52,8 @@ u32 lds_entries(const Mech &m, const GenOptions &o);
 // per CU); none for mixed-precision kernels with refinement sweeps.
 u32 auto_lds(const GenOptions &o);
 
-// The default load-pipeline depth: 16 statements, 24 for unrefined
-// mixed-precision kernels (their FP32 loads hold one register each).
+// The default load-pipeline depth: 24 statements; 16 for f32 delta lanes and
+// refined mixed-precision kernels.
 u32 auto_batch(const GenOptions &o);
 
 // The default rate-constant reuse window: 384 statements for one-mechanism

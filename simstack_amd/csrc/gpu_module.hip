@@ -1739,7 +1739,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     const u32 grids[] = {0, cus, cus * 3 / 2, cus * 2, cus * 5 / 2, cus * 3, cus * 4};
     for (u32 i = 0; i < sizeof(grids) / sizeof(grids[0]) && e == hipSuccess; ++i) e = consider(0, grids[i], g.gen);
     // 2. generator knobs at the best grid, one at a time
-    const u32 batches[] = {8, 24}, windows[] = {8, 32}, unrolls[] = {1};
+    const u32 batches[] = {8, 16}, windows[] = {8, 32}, unrolls[] = {1};
     for (u32 b : batches)
         if (e == hipSuccess) {
             GenOptions o = best_gen;
@@ -2080,7 +2080,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
         if (!same_gen(o, best_gen)) e = consider(best_grid, o);
     };
     knob([](GenOptions &o) { o.batch = 8; });
-    knob([](GenOptions &o) { o.batch = 24; });
+    knob([](GenOptions &o) { o.batch = 16; });
     knob([](GenOptions &o) { o.reg_window = 16; });
     knob([](GenOptions &o) { o.reg_window = 32; });
     knob([](GenOptions &o) { o.unroll_stages = 1; });

@@ -29,14 +29,17 @@ PEAK_FP64_VALU_TFLOPS = 78.6     # MI355X FP64 vector, dense
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(mech, meth, ctl, sp, cells, duration):
+def cpu_baseline(mech, meth, ctl, sp, cells, duration, threads=0):
     """The reference's own FP64 AVX2 path (oracle/_ref, f64x4 lanes, threads as
     dbd_advance_threads) on the first `cells` cells for one advance; falls back
-    to the oracle's scalar restatement if the reference build is absent."""
+    to the oracle's scalar restatement if the reference build is absent.
+    threads 0: the host's CPU share (OMP_NUM_THREADS, else the CPU count, at
+    most 64)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
     from simstack_amd import problem
-    threads = min(os.cpu_count() or 1, 64)
+    if threads <= 0:
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1, 64)
     z = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
     t0 = time.perf_counter()
     if oracle_ffi.have_ref():
@@ -498,6 +501,8 @@ def main():
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
+        # the same path on one core, on a smaller sample
+        out["cpu_baseline_1core"] = cpu_baseline(mech, meth, ctl, sp, max(args.cpu_cells // 8, 64), 1.0, threads=1)
     print(json.dumps(out), flush=True)
 
 

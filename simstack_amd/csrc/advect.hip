@@ -29,6 +29,7 @@ namespace {
 constexpr unsigned kThreads = 256;
 constexpr unsigned kMaxChunk = 32;   // plug flow: zones per thread, up to 8192 zones
 constexpr unsigned kRowsPerBlock = kThreads / 64u;   // waves per workgroup
+constexpr unsigned kGroupFields = 8;   // fields of a channel advanced by one wave group
 
 template <class T>
 __device__ __forceinline__ T fma_t(T a, T b, T c);
@@ -204,6 +205,85 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_wave_kernel(T *x, unsign
                                    inflow ? inflow[field] : T(0), outlet_sum ? outlet_sum + r : nullptr, active);
 }
 
+// The same row update for G consecutive fields of one channel per wave group:
+// the channel's coefficients are loaded once and kept in registers for the
+// G rows, instead of once per row. Row groups are numbered field group major,
+// channel minor (neighbouring groups touch neighbouring memory). Every wave
+// runs G row iterations with the same barriers (a row past the last field is
+// inactive: no loads, stores or outlet), and the edge buffers' parity follows
+// one substep counter across the rows, so two writes of a buffer are always
+// a barrier apart from the reads between them.
+template <class T, unsigned SLOTS, unsigned WPR, unsigned G>
+__global__ void __launch_bounds__(kThreads) simk_upwind_group_kernel(T *x, unsigned long long padded, unsigned fields,
+                                                                     unsigned channels, unsigned nodes,
+                                                                     const T *courant, unsigned substeps,
+                                                                     const T *inflow, T *outlet_sum) {
+    constexpr unsigned kPart = SLOTS * 64u;
+    __shared__ T edge[2][kRowsPerBlock];
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x / 64u, part = wave % WPR;
+    const unsigned groups = (fields + G - 1) / G;
+    const unsigned gidx = blockIdx.x * (kRowsPerBlock / WPR) + wave / WPR;
+    const bool gactive = gidx < groups * channels;
+    if (WPR == 1 && !gactive) return;   // whole waves, no barriers
+    const unsigned gi = gactive ? gidx : 0;
+    const unsigned ch = gi % channels, fg = gi / channels;
+    const unsigned base = part * kPart;
+    const T *cr = courant + (unsigned long long)ch * nodes;
+    T c[SLOTS];
+#pragma unroll
+    for (unsigned k = 0; k < SLOTS; ++k) {
+        const unsigned j = base + k * 64u + lane;
+        c[k] = (gactive && j < nodes) ? cr[j] : T(0);
+    }
+    const bool outlet_part = outlet_sum && (nodes - 1) / kPart == part;
+    const unsigned oslot = ((nodes - 1) % kPart) / 64u, olane = (nodes - 1) % 64u;
+    unsigned step = 0;
+    for (unsigned i = 0; i < G; ++i) {
+        const unsigned f = fg * G + i;
+        const bool active = gactive && f < fields;
+        T *xr = x + (unsigned long long)(active ? f : 0u) * padded + (unsigned long long)ch * nodes;
+        T v[SLOTS];
+#pragma unroll
+        for (unsigned k = 0; k < SLOTS; ++k) {
+            const unsigned j = base + k * 64u + lane;
+            v[k] = (active && j < nodes) ? xr[j] : T(0);
+        }
+        const bool owns_outlet = active && outlet_part;
+        const unsigned r = (active ? f : 0u) * channels + ch;
+        T out = owns_outlet ? outlet_sum[r] : T(0);
+        const T in = (active && inflow) ? inflow[f] : T(0);
+        for (unsigned s = 0; s < substeps; ++s, ++step) {
+            if (owns_outlet) {
+                T ov = T(0);
+#pragma unroll
+                for (unsigned k = 0; k < SLOTS; ++k)
+                    if (k == oslot) ov = v[k];
+                out = out + lane_value(ov, olane);
+            }
+            T first = in;
+            if constexpr (WPR > 1) {
+                if (lane == 63) edge[step & 1u][wave] = v[SLOTS - 1];
+                __syncthreads();
+                if (part) first = edge[step & 1u][wave - 1];
+            }
+#pragma unroll
+            for (unsigned k = SLOTS; k-- > 0;) {
+                const T prev = k ? dpp<kWaveRor1>(T(0), v[k - 1]) : first;
+                const T up = shift_up(v[k], prev);
+                v[k] = fma_t<T>(c[k], up - v[k], v[k]);
+            }
+        }
+        if (owns_outlet && lane == 0) outlet_sum[r] = out;
+        if (active) {
+#pragma unroll
+            for (unsigned k = 0; k < SLOTS; ++k) {
+                const unsigned j = base + k * 64u + lane;
+                if (j < nodes) xr[j] = v[k];
+            }
+        }
+    }
+}
+
 // Plug flow along a chain of zones (dbd_flow_exchange, dbd.cpp:465-483): one
 // workgroup per species row; zone 0 relaxes toward the feed.
 template <unsigned CHUNK>
@@ -261,8 +341,28 @@ hipError_t launch_upwind(hipStream_t stream, T *x, u64 padded, u32 fields, u32 c
     const WaveShape w = wave_shape(nodes);
     if (!w.wpr) return hipErrorInvalidValue;
     const u32 per_block = kRowsPerBlock / w.wpr;
-    const dim3 grid((rows + per_block - 1) / per_block), block(kThreads);
+    const dim3 block(kThreads);
     const unsigned long long pd = padded;
+    if (fields >= kGroupFields) {
+        // kGroupFields fields of a channel per wave group (coefficients loaded once)
+        const u32 groups = (fields + kGroupFields - 1) / kGroupFields;
+        const u64 units = (u64)groups * channels;
+        const dim3 ggrid((u32)((units + per_block - 1) / per_block));
+        if (w.slots == 16)
+            hipLaunchKernelGGL((simk_upwind_group_kernel<T, 16, 1, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                               fields, channels, nodes, courant, substeps, inflow, outlet_sum);
+        else if (w.wpr == 1)
+            hipLaunchKernelGGL((simk_upwind_group_kernel<T, 32, 1, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                               fields, channels, nodes, courant, substeps, inflow, outlet_sum);
+        else if (w.wpr == 2)
+            hipLaunchKernelGGL((simk_upwind_group_kernel<T, 32, 2, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                               fields, channels, nodes, courant, substeps, inflow, outlet_sum);
+        else
+            hipLaunchKernelGGL((simk_upwind_group_kernel<T, 32, 4, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                               fields, channels, nodes, courant, substeps, inflow, outlet_sum);
+        return hipGetLastError();
+    }
+    const dim3 grid((rows + per_block - 1) / per_block);
     if (w.slots == 16)
         hipLaunchKernelGGL((simk_upwind_wave_kernel<T, 16, 1>), grid, block, 0, stream, x, pd, rows, channels,
                            nodes, courant, substeps, inflow, outlet_sum);

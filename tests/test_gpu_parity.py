@@ -306,3 +306,32 @@ def test_mixed_autotune_keeps_bits_and_fp64(gm):
     b = z.subset(lanes)
     oracle_run(mech, meth, ctl, b, [0.2])
     assert compare_bits(f.subset(lanes), b) == ZERO
+
+
+@pytest.mark.parametrize("sweeps", [0xFFFFFFFF, 0], ids=["0sweeps", "1sweep"])
+def test_mixed_lu_placement_keeps_bits(gm, sweeps):
+    """The mixed kernel keeps its first lds_slots f32 LU entries in LDS and the
+    rest in the workspace. Where they live changes no operation: none in LDS,
+    an odd share and all 439 in LDS (no workspace LU traffic) compute the same
+    bits, with and without a refinement sweep."""
+    from simstack_amd import gpu
+    sp, mech, z = make_problem("c2", 2048)
+    meth, ctl = problem.method(2), problem.control()
+    runs = []
+    try:
+        for lds in (0xFFFFFFFF, 201, 439):
+            a = z.copy()
+            gm.arena_reset()
+            gm.set_kernel(0)
+            gm.set_gen_options(sweeps=sweeps, lds_slots=lds)
+            gm.set_mech(mech, meth, ctl)
+            dz = gpu.DeviceZones(gm, a)
+            dz.upload()
+            dz.advance(0.2, precision=2)
+            assert gm.kin_info().kernel == 3
+            dz.download(mutable_only=False)
+            assert a.failed.sum() == 0
+            runs.append(a.block.tobytes())
+    finally:
+        gm.set_gen_options()
+    assert runs[0] == runs[1] == runs[2]

@@ -256,7 +256,7 @@ SIMK_DEV void simk_d32_body(const SimkKinArgs &A, const SimkD32Consts &C) {
     float h = 0.0f, t = 0.0f, applied = 0.0f, seg_end = 0.0f, period = 0.0f, first = 0.0f, count = 0.0f;
     float steps = 0.0f, rejects = 0.0f, pulses = 0.0f, failed = 0.0f;
     const float duration = (float)A.duration;
-    bool pending = false, reject_last = false, reject_more = false, fresh = !A.queue_first;
+    bool reject_last = false, reject_more = false, fresh = !A.queue_first;
     pos = wave * SIMK_WAVE + lane;
 
     // dbd_advance_group's load (dbd_lanes.h:31-54) for the current cell.
@@ -334,7 +334,7 @@ SIMK_DEV void simk_d32_body(const SimkKinArgs &A, const SimkD32Consts &C) {
                 phase = SIMK_PH_SEGMENT;
             } else if (phase == SIMK_PH_SEGMENT) {
                 // kin_advance's segment (kin_solver.h:327-330) and kin_integrate's entry (:204-208)
-                pending = applied < count;
+                const bool pending = applied < count;
                 const float next_pulse = __builtin_fmaf(applied, period, first);
                 seg_end = pending ? simk_fminf_(next_pulse, duration) : duration;
                 h = h > 0.0f ? h : C.h_start;
@@ -349,6 +349,12 @@ SIMK_DEV void simk_d32_body(const SimkKinArgs &A, const SimkD32Consts &C) {
                     phase = SIMK_PH_ATTEMPT;
                 }
             } else {   // SIMK_PH_END (kin_solver.h:332-345)
+                // The segment's pending flag, recomputed: applied is unchanged
+                // since SEGMENT. A bool carried from SEGMENT to here across
+                // the wave's divergent loop exits was not kept per lane by the
+                // compiler (the lane-mask register was rewritten for lanes
+                // waiting outside the loop), so no lane mask is carried.
+                const bool pending = applied < count;
                 const bool alive = !(failed > 0.0f);
                 const bool hit = pending && alive && (t >= seg_end);
                 if (hit) {
@@ -517,7 +523,7 @@ SIMK_DEV void simk_d32g_body(const SimkKinArgs &A, const SimkD32Consts &C) {
     float steps = 0.0f, rejects = 0.0f, pulses = 0.0f, failed = 0.0f;
     float base[N], y[N];
     const float duration = (float)A.duration;
-    bool pending = false, reject_last = false, reject_more = false, fresh = !A.queue_first;
+    bool reject_last = false, reject_more = false, fresh = !A.queue_first;
     pos = wave * SIMK_WAVE + lane;
 
     auto load = [&](bool first_call) {
@@ -589,7 +595,7 @@ SIMK_DEV void simk_d32g_body(const SimkKinArgs &A, const SimkD32Consts &C) {
                 rep = 0;
                 phase = SIMK_PH_SEGMENT;
             } else if (phase == SIMK_PH_SEGMENT) {
-                pending = applied < count;
+                const bool pending = applied < count;
                 const float next_pulse = __builtin_fmaf(applied, period, first);
                 seg_end = pending ? simk_fminf_(next_pulse, duration) : duration;
                 h = h > 0.0f ? h : C.h_start;
@@ -604,6 +610,12 @@ SIMK_DEV void simk_d32g_body(const SimkKinArgs &A, const SimkD32Consts &C) {
                     phase = SIMK_PH_ATTEMPT;
                 }
             } else {   // SIMK_PH_END
+                // The segment's pending flag, recomputed: applied is unchanged
+                // since SEGMENT. A bool carried from SEGMENT to here across
+                // the wave's divergent loop exits was not kept per lane by the
+                // compiler (the lane-mask register was rewritten for lanes
+                // waiting outside the loop), so no lane mask is carried.
+                const bool pending = applied < count;
                 const bool alive = !(failed > 0.0f);
                 const bool hit = pending && alive && (t >= seg_end);
                 if (hit) {

@@ -384,7 +384,7 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
     simk_u32 cell = 0, phase = SIMK_PH_CELL, it = 0, rep = 0;
     double h = 0.0, t = 0.0, applied = 0.0, seg_end = 0.0, period = 0.0, first = 0.0, count = 0.0;
     double steps = 0.0, rejects = 0.0, pulses = 0.0, failed = 0.0;
-    bool pending = false, reject_last = false, reject_more = false, fresh = !A.queue_first;
+    bool reject_last = false, reject_more = false, fresh = !A.queue_first;
     const simk_u64 t_begin = (SLICED && A.slice) ? __builtin_amdgcn_s_memrealtime() : 0;
     bool expired = false;
 
@@ -451,14 +451,14 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
                         reject_last = (f & 1u) != 0;
                         reject_more = (f & 2u) != 0;
                         rr.st(4, 0.0);
-                        pending = applied < count;
+                        const bool pending = applied < count;
                         const double next_pulse = simk_fma(applied, period, first);
                         seg_end = pending ? simk_min(next_pulse, A.duration) : A.duration;
                         phase = SIMK_PH_CHECK;
                     }
                 }
             } else if (phase == SIMK_PH_SEGMENT) {
-                pending = applied < count;
+                const bool pending = applied < count;
                 const double next_pulse = simk_fma(applied, period, first);
                 seg_end = pending ? simk_min(next_pulse, A.duration) : A.duration;
                 h = h > 0.0 ? h : ctl.h_start;
@@ -502,6 +502,10 @@ SIMK_DEV void simk_persistent_body(const SimkKinArgs &A, const M &me) {
                     phase = SIMK_PH_ATTEMPT;
                 }
             } else {   // SIMK_PH_END
+                // The segment's pending flag, recomputed (applied is unchanged
+                // since SEGMENT) rather than carried as a lane mask across the
+                // wave's divergent loop exits.
+                const bool pending = applied < count;
                 const bool alive = !(failed > 0.0);
                 const bool hit = pending && alive && (t >= seg_end);
                 if (hit) {

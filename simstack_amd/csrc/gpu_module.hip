@@ -995,19 +995,6 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
         !L->period || !L->pulse_first || !L->pulse_count)
         return hipErrorInvalidValue;
     if (L->precision > 2) return hipErrorNotSupported;
-    if (L->precision == 1 && repeats > 1) {
-        // f32 delta lanes run each repeat as a launch of its own. Repeats fused
-        // in one launch (store, then re-read the stored rows for the next
-        // repeat) were measured to differ in some bits from separate calls
-        // when the launch starts from the state a previous launch left, and
-        // the cause was not found; separate launches are the form checked
-        // bit for bit against the oracle.
-        for (u32 r = 0; r < repeats; ++r) {
-            hipError_t e = launch(L, 1);
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
-    }
     if (g.mode == 1) return launch_wave(L, lanes, repeats);
     if (L->precision == 1) return launch_delta32(L, lanes, repeats);
     const JitKernel *jk = nullptr;

@@ -126,10 +126,11 @@ def test_delta32_generated_lds_shares_keep_bits(gm, lds):
 
 
 def test_delta32_repeats_after_a_previous_call(gm):
-    """One advance, then simk_kin_advance_n(2): the same bits as three single
-    advances and as the oracle, at the tolerance where half the cells fail in
-    f32 (rtol 1e-6). Repeats that start from a previous call's state are run
-    as separate launches in delta lanes."""
+    """One advance, then simk_kin_advance_n(2) fused in one launch: the same
+    bits as three single advances and as the oracle, at the tolerance where
+    half the cells fail in f32 (rtol 1e-6). The fused launch stores each
+    repeat's cell and re-reads the stored rows for the next one, starting
+    here from the state a previous launch left."""
     sp, mech, z = make_problem("c2", 512)
     meth, ctl = problem.method(2), problem.control()
     a, b = z.copy(), z.copy()

@@ -30,22 +30,15 @@ def _oracle(mech, meth, ctl, z, duration, precision=0, lanes=None):
     adv(mech, meth, ctl, z, duration, lanes=lanes)
 
 
-# Known parity bug of the device f32 delta kernels (generated and table walk
-# alike): when pulse schedules fall after the end of the advance, so that the
-# pulsed cells apply their pulses at the clipped segment end, the pulse-free
-# cells of the run also record one pulse and end with h = h_start. The oracle
-# agrees with the reference's own scalar f32 and f32x8 lanes on these inputs;
-# FP64 and mixed precision are correct. Schedules inside the advance (the DBD
-# workload, the other delta-lane tests) are unaffected.
-_D32_LATE_PULSES = pytest.mark.xfail(strict=True, reason="f32 delta lanes: pulses due after the advance end "
-                                                        "mark pulse-free cells (open bug)")
-
-
+# Pulse schedules due after the end of the advance (the pulsed cells apply
+# their pulses at the clipped segment end while the pulse-free cells of the
+# same waves integrate to it) once left pulse-free f32 delta lanes with one
+# recorded pulse and h = h_start: the segment's pending flag, carried as a
+# lane mask across the wave's divergent loop exits, was rewritten for lanes
+# waiting outside the loop. The flag is now recomputed where it is tested.
 @pytest.mark.parametrize("cells", [1, 63, 65, 130])
 @pytest.mark.parametrize("kernel,precision", [(1, 0), (2, 0), (1, 1)], ids=["generated", "table", "f32delta"])
-def test_single_cell_and_partial_waves(gm, cells, kernel, precision, request):
-    if precision == 1 and cells > 1:
-        request.applymarker(_D32_LATE_PULSES)
+def test_single_cell_and_partial_waves(gm, cells, kernel, precision):
     sp, mech, z = make_problem("c3c", cells)
     add_pulses(z, seed=cells)
     meth, ctl = problem.method(2), problem.control(rtol=1e-5, atol=1e-10)

@@ -81,6 +81,21 @@ GPU_API s32 simk_kin_set_mech_caps(const void *mech, const SimkCaps *caps, const
  * host work in between). */
 GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *launch, u32 repeats, f32 *kernel_ms);
 
+/* simk_kin_advance_n without any device allocation. It has no counterpart in
+ * the reference. It is for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc. The persistent kernels' workspace and queue counter come from a
+ * fixed 64 MiB array in the library's code object (one copy per GPU, held
+ * until the process ends, its address resolved at gpu_init). The grid is
+ * lowered to the waves whose workspace fits it, which changes no result.
+ * Errors: hipErrorInvalidValue when one wave's workspace alone exceeds
+ * 64 MiB - 256 B (a mechanism of about 130,000 workspace slots per lane);
+ * hipErrorNotReady when the launch would first have to load a kernel (the
+ * mixed-precision and f32 delta kernels load on their first ordinary
+ * advance; call gpu_kin_advance once with that precision beforehand). Calls
+ * share the one array per GPU, so two calls that overlap on a GPU race. */
+GPU_API s32 simk_kin_advance_static(const GpuKinLaunch *launch, u32 repeats, f32 *kernel_ms);
+
 /* Persistent grid size in waves (0: as many as are resident on the device). */
 GPU_API s32 simk_kin_set_grid(u32 waves);
 

@@ -208,6 +208,15 @@ __global__ void simk_queues_zero_kernel(u32 *queue, u32 count) {
     if (blockIdx.x == 0 && threadIdx.x < count) queue[threadIdx.x] = 0;
 }
 
+// Device memory of simk_kin_advance_static: the queue counter in the first 256
+// bytes, the persistent kernels' workspace after it. Every GPU the library
+// runs on holds its own copy from the first call until the process ends. Host
+// code never passes this name to a kernel (on the host it stands for no device
+// address): gpu_init resolves each device's copy with hipGetSymbolAddress and
+// launches use that pointer.
+#define SIMK_SCRATCH_BYTES (64ull << 20)
+static __device__ __align__(256) double scratch[SIMK_SCRATCH_BYTES / sizeof(double)];
+
 // ---------------------------------------------------------------------------
 // Module state (one device, one stream; calls come from one thread at a time).
 
@@ -285,6 +294,11 @@ struct State {
     double *ws = nullptr;
     u64 ws_bytes = 0;
     u32 *queue = nullptr;
+    // scratch's address on each device (resolved at gpu_init), and whether the
+    // launch being prepared must use it instead of allocating
+    static constexpr int kMaxDevices = 64;
+    u8 *scratch_at[kMaxDevices] = {};
+    bool use_scratch = false;
     u32 cus = 0;
     u32 grid_override = 0;     // persistent waves per launch (0: resident waves)
     int blocks_per_cu_table = 0, blocks_per_cu_d32 = 0;
@@ -620,6 +634,38 @@ void fill_dev_method(const KinMethod &me, SimkDevMethod &dm) {
     stage_plan(me, dm.eval, dm.save_f);
 }
 
+// The workspace and queue counter for a persistent launch of `waves` waves
+// with `wave_bytes` of workspace each. Ordinary launches grow the module's
+// workspace as needed. Inside simk_kin_advance_static nothing is allocated:
+// the device's scratch array serves both, `waves` is lowered to what it holds
+// (results do not depend on the grid), and a wave whose workspace alone
+// exceeds it is an error.
+hipError_t ensure_ws(u64 bytes);
+hipError_t launch_buffers(u64 wave_bytes, u32 &waves, double *&ws, u32 *&queue) {
+    if (g.use_scratch) {
+        u8 *base = (g.device >= 0 && g.device < State::kMaxDevices) ? g.scratch_at[g.device] : nullptr;
+        if (!base) return hipErrorNotInitialized;
+        const u64 room = SIMK_SCRATCH_BYTES - 256;
+        if (wave_bytes > room) return hipErrorInvalidValue;
+        if (wave_bytes) {
+            const u64 fit = room / wave_bytes;
+            if (fit < waves) waves = (u32)fit;
+        }
+        queue = (u32 *)base;
+        ws = (double *)(base + 256);
+        return hipSuccess;
+    }
+    hipError_t e = ensure_ws((u64)waves * wave_bytes);
+    if (e != hipSuccess) return e;
+    if (!g.queue) {
+        e = hipMalloc(&g.queue, 256);
+        if (e != hipSuccess) return e;
+    }
+    ws = g.ws;
+    queue = g.queue;
+    return hipSuccess;
+}
+
 hipError_t ensure_ws(u64 bytes) {
     if (bytes <= g.ws_bytes) return hipSuccess;
     if (g.ws) {
@@ -875,12 +921,12 @@ hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
         blocks = 1;
     u32 resident = (u32)blocks * (g.cus ? g.cus : 1);
     if (g.grid_override) resident = g.grid_override;
-    const u32 waves = lanes < resident ? lanes : resident;
-    if (!g.queue) {
-        e = hipMalloc(&g.queue, 256);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, g.queue, waves);
+    u32 waves = lanes < resident ? lanes : resident;
+    double *ws = nullptr;
+    u32 *queue = nullptr;
+    e = launch_buffers(0, waves, ws, queue);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, queue, waves);
     SimkKinArgs A;
     memset(&A, 0, sizeof(A));
     A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
@@ -892,7 +938,7 @@ hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     A.n_species = m.n;
     A.n_reactions = m.R;
     A.repeats = repeats;
-    A.queue = g.queue;
+    A.queue = queue;
     A.order = g.order;
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
@@ -919,6 +965,7 @@ hipError_t launch_wave(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
 hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     const Mech &m = g.mech;
     if (g.kernel_kind != 2 && !g.jit_d32_tried) {
+        if (g.use_scratch) return hipErrorNotReady;   // loading a kernel allocates
         g.jit_d32_tried = true;
         GenOptions o = g.gen;
         o.precision = 1;
@@ -941,14 +988,12 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const u32 resident = resident_waves(gen ? g.jit_d32.blocks_per_cu : g.blocks_per_cu_d32,
                                         (u64)slots * SIMK_WAVE * slot_bytes, g.grid_override);
-    const u32 waves = needed < resident ? needed : resident;
-    hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * slot_bytes);
+    u32 waves = needed < resident ? needed : resident;
+    double *ws = nullptr;
+    u32 *queue = nullptr;
+    hipError_t e = launch_buffers((u64)slots * SIMK_WAVE * slot_bytes, waves, ws, queue);
     if (e != hipSuccess) return e;
-    if (!g.queue) {
-        e = hipMalloc(&g.queue, 256);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, g.queue, waves * SIMK_WAVE);
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, queue, waves * SIMK_WAVE);
     SimkKinArgs A;
     memset(&A, 0, sizeof(A));
     A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
@@ -957,12 +1002,12 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     A.padded = L->padded;
     A.lanes = lanes;
     A.duration = L->duration;
-    A.ws = g.ws;
+    A.ws = ws;
     A.ws_slots = slots;
     A.n_species = m.n;
     A.n_reactions = m.R;
     A.repeats = repeats;
-    A.queue = g.queue;
+    A.queue = queue;
     A.order = g.order;
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
@@ -1001,6 +1046,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     if (L->precision == 2) {
         // Mixed precision exists only as a generated kernel.
         if (!g.jit_mixed_ok) {
+            if (g.use_scratch) return hipErrorNotReady;   // loading a kernel allocates
             GenOptions mo = g.mix_tuned ? g.mix_gen : g.gen;
             mo.precision = 2;
             mo.sweeps = g.sweeps;
@@ -1022,14 +1068,12 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     const int per_cu = use_gen ? jk->blocks_per_cu : g.blocks_per_cu_table;
     const u32 grid = L->precision == 2 && g.mix_tuned ? g.mix_grid : g.grid_override;
     const u32 resident = resident_waves(per_cu, (u64)slots * SIMK_WAVE * 8, grid);
-    const u32 waves = needed < resident ? needed : resident;
-    hipError_t e = ensure_ws((u64)waves * slots * SIMK_WAVE * sizeof(double));
+    u32 waves = needed < resident ? needed : resident;
+    double *ws = nullptr;
+    u32 *queue = nullptr;
+    hipError_t e = launch_buffers((u64)slots * SIMK_WAVE * sizeof(double), waves, ws, queue);
     if (e != hipSuccess) return e;
-    if (!g.queue) {
-        e = hipMalloc(&g.queue, 256);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, g.queue, waves * SIMK_WAVE);
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, queue, waves * SIMK_WAVE);
     SimkKinArgs A;
     memset(&A, 0, sizeof(A));
     A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
@@ -1038,12 +1082,12 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     A.padded = L->padded;
     A.lanes = lanes;
     A.duration = L->duration;
-    A.ws = g.ws;
+    A.ws = ws;
     A.ws_slots = slots;
     A.n_species = g.mech.n;
     A.n_reactions = g.mech.R;
     A.repeats = repeats;
-    A.queue = g.queue;
+    A.queue = queue;
     A.order = g.order;
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
@@ -1301,6 +1345,14 @@ GPU_API s32 gpu_init(u32 device_index, u64 arena_bytes, GpuInfo *info) {
     g.arena.used = 0;
     g.device = (int)device_index;
     g.cus = (u32)prop.multiProcessorCount;
+    // simk_kin_advance_static's scratch array: its address on this device,
+    // resolved once per device with the device current.
+    if (device_index < (u32)State::kMaxDevices && !g.scratch_at[device_index]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return err_code(e);
+        g.scratch_at[device_index] = (u8 *)p;
+    }
     if (const char *mb = getenv("SIMK_CACHE_BUDGET_MB")) g.cache_budget = (u64)strtoull(mb, nullptr, 10) << 20;
     g.ready = true;
     return 0;
@@ -1421,6 +1473,23 @@ GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *L, u32 repeats, f32 *kernel_m
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
     (void)hipEventRecord(g.kin_start, g.stream);
     hipError_t e = launch(L, repeats);
+    if (e != hipSuccess) return err_code(e);
+    (void)hipEventRecord(g.kin_stop, g.stream);
+    e = hipEventSynchronize(g.kin_stop);
+    if (e != hipSuccess) return err_code(e);
+    f32 ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, g.kin_start, g.kin_stop);
+    g.last_ms = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    return err_code(hipGetLastError());
+}
+
+GPU_API s32 simk_kin_advance_static(const GpuKinLaunch *L, u32 repeats, f32 *kernel_ms) {
+    if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
+    (void)hipEventRecord(g.kin_start, g.stream);
+    g.use_scratch = true;
+    hipError_t e = launch(L, repeats);
+    g.use_scratch = false;
     if (e != hipSuccess) return err_code(e);
     (void)hipEventRecord(g.kin_stop, g.stream);
     e = hipEventSynchronize(g.kin_stop);

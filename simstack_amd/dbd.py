@@ -135,3 +135,57 @@ def reference_zones(mech, count):
         z.power_W_cm3[i] = 0.05 + 0.1 * (i % 3)
         z.freq_Hz[i] = 5000.0 + 2500.0 * (i % 4)
     return z.setup()
+
+
+def fnv1a64(data, seed=0xCBF29CE484222325):
+    """64-bit FNV-1a of a bytes-like object."""
+    h = seed
+    for b in bytes(data):
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+class Scene:
+    """The chained DBD reactor as the reference's scene runs it
+    (src/sim/dbd_scene.cpp:82-90 operating point, :226-298 master step):
+    `count` zones (64, the command line's default, src/platform/
+    platform_common.cpp:190) in a chain at T 310 K, 1 atm, 2 % water, 0.1 W/cm^3,
+    10 kHz, fed with humid air and exchanging by plug flow over a 2-s
+    residence. Each master step of dt prepares the zones' source and pulse
+    schedule over (t, t + dt], advances the chemistry, commits the pulses and
+    relaxes each zone toward its upstream neighbour (zone 0 toward the feed)
+    with a = dt * count / residence.
+
+    The chemistry advance and the plug flow are supplied by the caller:
+    `advance(zones, dt)` integrates the block in place and
+    `flow(zones, a, feed)` applies the plug-flow update to its y rows."""
+
+    def __init__(self, mech, count=64, residence=2.0, mode=PULSED, source=None):
+        if count % 8:
+            raise ValueError("the scene's state hash covers whole groups of 8 zones (the reference's row padding)")
+        self.mech, self.count, self.residence, self.mode = mech, count, residence, mode
+        self.source = source if source is not None else source_defaults()
+        self.z = DbdZones(mech, count, T=310.0, p=101325.0, x_H2O=0.02, power=0.1, freq=10000.0).setup()
+        self.t = 0.0
+        self.master_steps = 0
+
+    def feed(self):
+        y = np.zeros(self.mech.n)
+        _ffi.host().simk_air_feed(C.byref(self.mech.sp), self.mech.n, float(self.z.T_K[0]), float(self.z.p_Pa[0]),
+                                  float(self.z.x_H2O[0]), y.ctypes.data_as(_ffi.f64p))
+        return y
+
+    def step(self, advance, flow, dt=1e-3):
+        t0, t1 = self.t, self.t + dt
+        self.z.prepare_source(self.source, self.mode, t0, t1)
+        advance(self.z, dt)
+        self.z.commit_pulses()
+        if self.residence > 0.0:
+            flow(self.z, dt * self.count / self.residence, self.feed())
+        self.t = t1
+        self.master_steps += 1
+
+    def state_hash(self):
+        """FNV-1a 64 of every zone's species row (dbd_scene_hash,
+        src/sim/dbd_scene.cpp:388-391: n rows of the zones, in row order)."""
+        return fnv1a64(np.ascontiguousarray(self.z.y[:, :self.count]).tobytes())

@@ -1,9 +1,16 @@
 // The DBD reactor workload on the host (include/simk_dbd.h): SimStack's
-// humid-air mechanism, feed gas, discharge source and the per-step zone-block
-// preparation, restated from src/sim/kinetics/air_mech.cpp and
-// src/sim/dbd.cpp. The arithmetic follows the reference expression by
-// expression (host libm), so the inputs handed to the device are the
-// reference's bits.
+// humid-air mechanism, its feed gas and discharge source, and the per-step
+// preparation of a zone block for the device advance.
+//
+// The discharge source is data-driven: a table of primary electron-impact
+// yields, then one table per metastable (N2(A), N(2D), O(1D)) of quenching
+// channels and of the stoichiometric updates their shares make. The
+// evaluator applies the updates in table order with a fixed grouping of
+// every sum and product, so the source increments handed to the solver are
+// the reference's bits (src/sim/kinetics/air_mech.cpp:185-288, checked
+// against the reference's own build). Zone preparation follows
+// src/sim/dbd.cpp:84-158: feed gas, rate constants from T and the third-body
+// density, and the absolute pulse clock of each zone.
 #include <math.h>
 #include <string.h>
 
@@ -83,40 +90,195 @@ const AirReaction kAirReactions[] = {
     {"O3 + M -> O2 + O + M", kM, 7.2e-10, 0.0, 11400.0, 0, 0, 0, 0, 0, 0},
 };
 
-// Quasi-steady distribution of `amount` of a metastable over its channels
-// (air_branch, air_mech.cpp:185-201): share_c = k_c [partner_c] / sum, no
-// channel consuming more than half its partner (the excess goes to the
-// dominant channel).
-struct Channel {
-    f64 k;
-    u32 partner;
+// Metastable bookkeeping.
+//
+// A discharge pulse leaves electronically excited species (N2(A), N(2D),
+// O(1D)) that are not species of the mechanism: each is spent at once on its
+// quenching channels. A channel's weight is its rate coefficient times its
+// collision partner's density; the metastable's amount is split in
+// proportion to the weights. A channel may use at most half of its partner's
+// density; whatever a capped channel cannot take is handed to the channel of
+// largest weight (the first such channel on ties). With no weight at all,
+// everything goes to channel 0.
+enum RateForm : u32 {
+    kConstant = 0,   // k0
+    kPowerT300 = 1,  // k0 (T / 300)^q
+    kExpOverT = 2,   // k0 exp(q / T)
+};
+struct Quench {
+    f64 k0;
+    RateForm form;
+    f64 q;
+    u32 partner;   // SimkAirSpecies field index (species_of below)
 };
 
-void branch(const Channel *ch, u32 count, const f64 *y, f64 amount, f64 *share) {
-    f64 sum = 0.0;
-    u32 big = 0;
-    for (u32 c = 0; c < count; ++c) {
-        share[c] = ch[c].k * y[ch[c].partner];
-        if (share[c] > share[big]) big = c;
-        sum += share[c];
+f64 quench_rate(const Quench &c, f64 T_K) {
+    switch (c.form) {
+    case kPowerT300: return c.k0 * pow(T_K / 300.0, c.q);
+    case kExpOverT: return c.k0 * exp(c.q / T_K);
+    default: return c.k0;
     }
-    if (sum <= 0.0) {
-        for (u32 c = 0; c < count; ++c) share[c] = c == big ? 1.0 : 0.0;
-        return;
-    }
-    f64 moved = 0.0;
-    for (u32 c = 0; c < count; ++c) {
-        share[c] /= sum;
-        f64 cap = amount > 0.0 ? 0.5 * y[ch[c].partner] / amount : 1.0;
-        if (c != big && share[c] > cap) {
-            moved += share[c] - cap;
-            share[c] = cap;
-        }
-    }
-    share[big] += moved;
 }
 
-void zone_rows(const SimkDbdZones *z, u64 off[14]) { simk_zones_layout(z->n_species, z->n_reactions, z->padded, off); }
+// Shares of `amount` over the channels, written to share[0..count).
+void split_metastable(const Quench *ch, u32 count, const u32 *partner, const f64 *y, f64 amount, f64 T_K,
+                      f64 *share) {
+    u32 lead = 0;
+    f64 weight_sum = 0.0;
+    for (u32 c = 0; c < count; ++c) {
+        share[c] = quench_rate(ch[c], T_K) * y[partner[c]];
+        weight_sum += share[c];
+        if (share[c] > share[lead]) lead = c;
+    }
+    if (weight_sum <= 0.0) {
+        for (u32 c = 0; c < count; ++c) share[c] = c == lead ? 1.0 : 0.0;
+        return;
+    }
+    f64 surplus = 0.0;
+    for (u32 c = 0; c < count; ++c) {
+        share[c] /= weight_sum;
+        if (c == lead) continue;
+        const f64 limit = amount > 0.0 ? 0.5 * y[partner[c]] / amount : 1.0;
+        if (share[c] > limit) {
+            surplus += share[c] - limit;
+            share[c] = limit;
+        }
+    }
+    share[lead] += surplus;
+}
+
+// One stoichiometric update of the source vector:
+//   delta[species] += sign * ((scale * (c_0 v_0 + c_1 v_1 + ...)) * post)
+// where the v are shares of the current stage (or, in the primary stage,
+// the primary amounts), the sum runs left to right, scale is the stage's
+// amount (1: none) and post a trailing factor (1: none). The updates of a
+// stage are applied in table order; that order, and the grouping above,
+// fix every rounding, so they are part of the data.
+struct Term {
+    f64 c;
+    u32 v;
+};
+struct Update {
+    u32 species;
+    s32 sign;
+    u32 nterms;
+    Term t[7];
+    f64 post;
+};
+
+f64 update_value(const Update &u, const f64 *v, f64 scale, bool scaled) {
+    f64 sum = u.t[0].c == 1.0 ? v[u.t[0].v] : u.t[0].c * v[u.t[0].v];
+    for (u32 i = 1; i < u.nterms; ++i) sum = sum + (u.t[i].c == 1.0 ? v[u.t[i].v] : u.t[i].c * v[u.t[i].v]);
+    f64 x = scaled ? scale * sum : sum;
+    if (u.post != 1.0) x = x * u.post;
+    return x;
+}
+
+void apply(const Update *ups, u32 count, const u32 *species_of, const f64 *v, f64 scale, bool scaled, f64 *delta) {
+    for (u32 i = 0; i < count; ++i) {
+        const f64 x = update_value(ups[i], v, scale, scaled);
+        f64 &d = delta[species_of[ups[i].species]];
+        d = ups[i].sign > 0 ? d + x : d - x;
+    }
+}
+
+// Species by SimkAirSpecies field position.
+enum AirField : u32 { O, O3, N, NO, NO2, NO3, N2O5, N2O, HNO2, HNO3, OH, HO2, H2O2, H, N2, O2, H2O };
+
+// Primary electron-impact yields, per 100 eV deposited, scaled by the
+// parent's mole fraction over the reference gas's (x / x_ref) and consumed
+// from the parent. amounts: 0 O2 dissociations, 1 N2(A), 2 N2
+// dissociations, 3 H2O dissociations, 4 O(1D) = f_O1D x amount 0.
+struct Primary {
+    u32 parent;
+    f64 x_ref;
+};
+const Primary kPrimary[4] = {{O2, 0.21}, {N2, 0.78}, {N2, 0.78}, {H2O, 0.02}};
+enum { kDissO2 = 0, kN2A = 1, kDissN2 = 2, kDissH2O = 3, kO1D = 4 };
+const Update kPrimaryUpdates[] = {
+    {O2, -1, 1, {{1.0, kDissO2}}, 1.0},                     // O2 -> O + O, O + O(1D)
+    {O, +1, 2, {{2.0, kDissO2}, {-1.0, kO1D}}, 1.0},        // the ground-state O atoms
+    {N2, -1, 2, {{1.0, kN2A}, {1.0, kDissN2}}, 1.0},        // N2 -> N2(A); N2 -> N + N(2D)
+    {N, +1, 1, {{1.0, kDissN2}}, 1.0},
+    {OH, +1, 1, {{1.0, kDissH2O}}, 1.0},                    // H2O -> OH + H
+    {H, +1, 1, {{1.0, kDissH2O}}, 1.0},
+    {H2O, -1, 1, {{1.0, kDissH2O}}, 1.0},
+};
+
+// N2(A) quenching (Kossyi et al. 1992). Every channel gives the N2 back.
+const Quench kN2AQ[9] = {
+    {2.5e-12, kPowerT300, 0.55, O2},   // 0: N2 + O + O
+    {1.3e-12, kPowerT300, 0.55, O2},   // 1: N2 + O2
+    {7.8e-14, kPowerT300, 0.55, O2},   // 2: N2O + O
+    {7.0e-12, kConstant, 0.0, O},      // 3: NO + N(2D)
+    {7.0e-11, kConstant, 0.0, NO},     // 4: N2 + NO
+    {1.0e-12, kConstant, 0.0, NO2},    // 5: N2 + NO + O
+    {8.0e-11, kConstant, 0.0, N2O},    // 6: N2 + N + NO
+    {5.0e-14, kConstant, 0.0, H2O},    // 7: N2 + H + OH
+    {3.0e-16, kConstant, 0.0, N2},     // 8: N2 + N2
+};
+const Update kN2AUpdates[] = {
+    {N2, +1, 7, {{1, 0}, {1, 1}, {1, 4}, {1, 5}, {1, 6}, {1, 7}, {1, 8}}, 1.0},
+    {O2, -1, 2, {{1, 0}, {1, 2}}, 1.0},
+    {O, +1, 3, {{2, 0}, {1, 2}, {1, 5}}, 1.0},
+    {N2O, +1, 2, {{1, 2}, {-1, 6}}, 1.0},
+    {O, -1, 1, {{1, 3}}, 1.0},
+    {NO, +1, 3, {{1, 3}, {1, 5}, {1, 6}}, 1.0},
+    {NO2, -1, 1, {{1, 5}}, 1.0},
+    {N, +1, 1, {{1, 6}}, 1.0},
+    {H2O, -1, 1, {{1, 7}}, 1.0},
+    {H, +1, 1, {{1, 7}}, 1.0},
+    {OH, +1, 1, {{1, 7}}, 1.0},
+};
+// N(2D) quenching (Herron 1999). Its amount is the primary N2 dissociations
+// plus N2(A)'s channel 3.
+const Quench kN2DQ[5] = {
+    {5.2e-12, kConstant, 0.0, O2},    // 0: NO + O
+    {1.7e-14, kConstant, 0.0, N2},    // 1: N + N2
+    {6.9e-13, kConstant, 0.0, O},     // 2: N + O
+    {6.0e-11, kConstant, 0.0, NO},    // 3: N2 + O
+    {2.2e-12, kConstant, 0.0, N2O},   // 4: NO + N2
+};
+const Update kN2DUpdates[] = {
+    {O2, -1, 1, {{1, 0}}, 1.0},
+    {NO, +1, 3, {{1, 0}, {-1, 3}, {1, 4}}, 1.0},
+    {O, +1, 2, {{1, 0}, {1, 3}}, 1.0},
+    {N, +1, 2, {{1, 1}, {1, 2}}, 1.0},
+    {N2, +1, 2, {{1, 3}, {1, 4}}, 1.0},
+    {N2O, -1, 1, {{1, 4}}, 1.0},
+};
+// O(1D) quenching (JPL 19-5). Channel 5 (O3) splits evenly between O2 + O2
+// and O2 + O + O: per O(1D), -1 O3, +1.5 O2, +1 O.
+const Quench kO1DQ[6] = {
+    {2.15e-11, kExpOverT, 110.0, N2},   // 0: O + N2
+    {3.3e-11, kExpOverT, 55.0, O2},     // 1: O + O2
+    {1.63e-10, kExpOverT, 60.0, H2O},   // 2: OH + OH
+    {7.26e-11, kExpOverT, 20.0, N2O},   // 3: NO + NO
+    {4.63e-11, kExpOverT, 20.0, N2O},   // 4: N2 + O2
+    {2.4e-10, kConstant, 0.0, O3},      // 5: see above
+};
+const Update kO1DUpdates[] = {
+    {O, +1, 2, {{1, 0}, {1, 1}}, 1.0},
+    {O3, -1, 1, {{1, 5}}, 1.0},
+    {O2, +1, 1, {{1, 5}}, 1.5},
+    {O, +1, 1, {{1, 5}}, 1.0},
+    {H2O, -1, 1, {{1, 2}}, 1.0},
+    {OH, +1, 1, {{1, 2}}, 2.0},
+    {N2O, -1, 2, {{1, 3}, {1, 4}}, 1.0},
+    {NO, +1, 1, {{1, 3}}, 2.0},
+    {N2, +1, 1, {{1, 4}}, 1.0},
+    {O2, +1, 1, {{1, 4}}, 1.0},
+};
+
+template <u32 C>
+void quench_stage(const Quench (&ch)[C], const Update *ups, u32 nups, const u32 *species_of, const f64 *y,
+                  f64 amount, f64 T_K, f64 *share, f64 *delta) {
+    u32 partner[C];
+    for (u32 c = 0; c < C; ++c) partner[c] = species_of[ch[c].partner];
+    split_metastable(ch, C, partner, y, amount, T_K, share);
+    apply(ups, nups, species_of, share, amount, true, delta);
+}
+
 
 }  // namespace
 
@@ -161,184 +323,159 @@ SIMK_HOST_API void simk_air_source_defaults(SimkAirSource *s) {
     s->filament_J_cm3 = 0.0;
 }
 
+// Feed gas: the ideal-gas number density at T and p (in cm^-3), split into
+// water at x_H2O and dry air of 79 % N2 (argon counted as N2) and 21 % O2.
 SIMK_HOST_API f64 simk_air_feed(const SimkAirSpecies *sp, u32 n, f64 T_K, f64 p_Pa, f64 x_H2O, f64 *y) {
-    f64 total = p_Pa / (kBoltzmann * T_K) * 1e-6;   // m^-3 -> cm^-3
-    for (u32 s = 0; s < n; ++s) y[s] = 0.0;
-    f64 dry = 1.0 - x_H2O;
-    y[sp->N2] = total * dry * 0.79;   // argon lumped into N2
-    y[sp->O2] = total * dry * 0.21;
-    y[sp->H2O] = total * x_H2O;
-    return total;
+    const f64 density = p_Pa / (kBoltzmann * T_K) * 1e-6;
+    memset(y, 0, sizeof(f64) * n);
+    const f64 dry_fraction = 1.0 - x_H2O;
+    y[sp->N2] = density * dry_fraction * 0.79;
+    y[sp->O2] = density * dry_fraction * 0.21;
+    y[sp->H2O] = density * x_H2O;
+    return density;
 }
 
+// The species change per cm^3 caused by depositing energy_J_cm3 into gas of
+// composition y at T: the primary electron-impact yields, then the quenching
+// of N2(A), N(2D) and O(1D), in that order (N2(A) feeds N(2D)).
 SIMK_HOST_API s32 simk_air_source_delta(const SimkAirSpecies *sp, const SimkAirSource *src, const f64 *y, u32 n,
                                         f64 energy_J_cm3, f64 T_K, f64 *delta) {
-    for (u32 s = 0; s < n; ++s) delta[s] = 0.0;
-    if (src->boltzmann) return 1;
-    f64 total = 0.0;
-    for (u32 s = 0; s < n; ++s) total += y[s];
-    if (total <= 0.0 || energy_J_cm3 <= 0.0) return 0;
-    const f64 hundreds_eV = energy_J_cm3 * kEvPerJoule * 0.01;
-    const f64 g_o2 = src->g_O2_diss, f_o1d = src->f_O1D, g_n2a = src->g_N2A, g_n2d = src->g_N2_diss;
-    // electron-impact channels, scaled by the parents' mole fractions
-    const f64 dO2 = g_o2 * (y[sp->O2] / total / 0.21) * hundreds_eV;
-    const f64 dA = g_n2a * (y[sp->N2] / total / 0.78) * hundreds_eV;
-    const f64 dN2 = g_n2d * (y[sp->N2] / total / 0.78) * hundreds_eV;
-    const f64 dOH = src->g_OH_per_100eV * (y[sp->H2O] / total / 0.02) * hundreds_eV;
-    const f64 d1D = f_o1d * dO2;
-    delta[sp->O2] -= dO2;
-    delta[sp->O] += 2.0 * dO2 - d1D;
-    delta[sp->N2] -= dA + dN2;
-    delta[sp->N] += dN2;
-    delta[sp->OH] += dOH;
-    delta[sp->H] += dOH;
-    delta[sp->H2O] -= dOH;
-    const f64 t3 = T_K / 300.0;
-    const f64 k_o2 = pow(t3, 0.55);
-    f64 f[10];
-    // N2(A): every channel returns the N2 (Kossyi et al. 1992)
-    const Channel a_ch[] = {
-        {2.5e-12 * k_o2, sp->O2},   // -> N2 + O + O
-        {1.3e-12 * k_o2, sp->O2},   // -> N2 + O2
-        {7.8e-14 * k_o2, sp->O2},   // -> N2O + O
-        {7.0e-12, sp->O},           // -> NO + N(2D)
-        {7.0e-11, sp->NO},          // -> N2 + NO
-        {1.0e-12, sp->NO2},         // -> N2 + NO + O
-        {8.0e-11, sp->N2O},         // -> N2 + N + NO
-        {5.0e-14, sp->H2O},         // -> N2 + H + OH
-        {3.0e-16, sp->N2},          // -> N2 + N2
-    };
-    branch(a_ch, 9, y, dA, f);
-    delta[sp->N2] += dA * (f[0] + f[1] + f[4] + f[5] + f[6] + f[7] + f[8]);
-    delta[sp->O2] -= dA * (f[0] + f[2]);
-    delta[sp->O] += dA * (2.0 * f[0] + f[2] + f[5]);
-    delta[sp->N2O] += dA * (f[2] - f[6]);
-    delta[sp->O] -= dA * f[3];
-    delta[sp->NO] += dA * (f[3] + f[5] + f[6]);
-    const f64 n2d = dN2 + dA * f[3];
-    delta[sp->NO2] -= dA * f[5];
-    delta[sp->N] += dA * f[6];
-    delta[sp->H2O] -= dA * f[7];
-    delta[sp->H] += dA * f[7];
-    delta[sp->OH] += dA * f[7];
-    // N(2D) (Herron 1999)
-    const Channel n_ch[] = {
-        {5.2e-12, sp->O2},    // -> NO + O
-        {1.7e-14, sp->N2},    // -> N + N2
-        {6.9e-13, sp->O},     // -> N + O
-        {6.0e-11, sp->NO},    // -> N2 + O
-        {2.2e-12, sp->N2O},   // -> NO + N2
-    };
-    branch(n_ch, 5, y, n2d, f);
-    delta[sp->O2] -= n2d * f[0];
-    delta[sp->NO] += n2d * (f[0] - f[3] + f[4]);
-    delta[sp->O] += n2d * (f[0] + f[3]);
-    delta[sp->N] += n2d * (f[1] + f[2]);
-    delta[sp->N2] += n2d * (f[3] + f[4]);
-    delta[sp->N2O] -= n2d * f[4];
-    // O(1D) (JPL 19-5)
-    const Channel o_ch[] = {
-        {2.15e-11 * exp(110.0 / T_K), sp->N2},    // -> O + N2
-        {3.3e-11 * exp(55.0 / T_K), sp->O2},      // -> O + O2
-        {1.63e-10 * exp(60.0 / T_K), sp->H2O},    // -> OH + OH
-        {7.26e-11 * exp(20.0 / T_K), sp->N2O},    // -> NO + NO
-        {4.63e-11 * exp(20.0 / T_K), sp->N2O},    // -> N2 + O2
-        {2.4e-10, sp->O3},                        // -> O2 + O2 or O2 + O + O
-    };
-    branch(o_ch, 6, y, d1D, f);
-    delta[sp->O] += d1D * (f[0] + f[1]);
-    delta[sp->O3] -= d1D * f[5];
-    delta[sp->O2] += d1D * f[5] * 1.5;
-    delta[sp->O] += d1D * f[5];
-    delta[sp->H2O] -= d1D * f[2];
-    delta[sp->OH] += d1D * f[2] * 2.0;
-    delta[sp->N2O] -= d1D * (f[3] + f[4]);
-    delta[sp->NO] += d1D * f[3] * 2.0;
-    delta[sp->N2] += d1D * f[4];
-    delta[sp->O2] += d1D * f[4];
+    memset(delta, 0, sizeof(f64) * n);
+    if (src->boltzmann) return 1;   // the tabulated Boltzmann G-values are not provided
+    f64 density = 0.0;
+    for (u32 s = 0; s < n; ++s) density += y[s];
+    if (density <= 0.0 || energy_J_cm3 <= 0.0) return 0;
+    u32 species_of[17];
+    memcpy(species_of, sp, sizeof(species_of));
+    const f64 per_100eV = energy_J_cm3 * kEvPerJoule * 0.01;
+    const f64 g[4] = {src->g_O2_diss, src->g_N2A, src->g_N2_diss, src->g_OH_per_100eV};
+    f64 amount[5];
+    for (u32 k = 0; k < 4; ++k)
+        amount[k] = g[k] * (y[species_of[kPrimary[k].parent]] / density / kPrimary[k].x_ref) * per_100eV;
+    amount[kO1D] = src->f_O1D * amount[kDissO2];
+    apply(kPrimaryUpdates, sizeof(kPrimaryUpdates) / sizeof(Update), species_of, amount, 1.0, false, delta);
+    f64 share[9];
+    quench_stage(kN2AQ, kN2AUpdates, sizeof(kN2AUpdates) / sizeof(Update), species_of, y, amount[kN2A], T_K, share,
+                 delta);
+    const f64 n2d = amount[kDissN2] + amount[kN2A] * share[3];
+    quench_stage(kN2DQ, kN2DUpdates, sizeof(kN2DUpdates) / sizeof(Update), species_of, y, n2d, T_K, share, delta);
+    quench_stage(kO1DQ, kO1DUpdates, sizeof(kO1DUpdates) / sizeof(Update), species_of, y, amount[kO1D], T_K, share,
+                 delta);
     return 0;
 }
 
+// ---- zone blocks ------------------------------------------------------------
+// The row offsets of a zone block, by name.
+struct Rows {
+    u64 p;
+    f64 *y, *h, *steps, *rejects, *pulses, *failed, *k, *src, *pulse, *period, *first, *count;
+    explicit Rows(const SimkDbdZones *z) {
+        u64 off[14];
+        simk_zones_layout(z->n_species, z->n_reactions, z->padded, off);
+        p = z->padded;
+        f64 *b = z->block;
+        y = b + off[0];
+        h = b + off[1];
+        steps = b + off[2];
+        rejects = b + off[3];
+        pulses = b + off[4];
+        failed = b + off[5];
+        k = b + off[6];
+        src = b + off[7];
+        pulse = b + off[8];
+        period = b + off[9];
+        first = b + off[10];
+        count = b + off[11];
+    }
+};
+
 SIMK_HOST_API void simk_dbd_fill_padding(SimkDbdZones *z) {
-    f64 *arrays[5] = {z->T_K, z->p_Pa, z->x_H2O, z->power_W_cm3, z->freq_Hz};
-    for (f64 *a : arrays)
-        for (u32 i = z->count; i < z->padded; ++i) a[i] = a[0];
+    f64 *per_zone[5] = {z->T_K, z->p_Pa, z->x_H2O, z->power_W_cm3, z->freq_Hz};
+    for (u32 i = z->count; i < z->padded; ++i)
+        for (f64 *a : per_zone) a[i] = a[0];
 }
 
 SIMK_HOST_API void simk_dbd_reset(SimkDbdZones *z, const SimkAirSpecies *sp) {
-    u64 off[14];
-    zone_rows(z, off);
-    const u64 p = z->padded;
-    f64 *b = z->block;
-    f64 feed[64];
+    const Rows r(z);
+    f64 gas[64];
     for (u32 i = 0; i < z->padded; ++i) {
-        simk_air_feed(sp, z->n_species, z->T_K[i], z->p_Pa[i], z->x_H2O[i], feed);
-        for (u32 s = 0; s < z->n_species; ++s) b[off[0] + s * p + i] = feed[s];
-        b[off[1] + i] = 0.0;   // h
+        simk_air_feed(sp, z->n_species, z->T_K[i], z->p_Pa[i], z->x_H2O[i], gas);
+        for (u32 s = 0; s < z->n_species; ++s) r.y[s * r.p + i] = gas[s];
+        r.h[i] = r.steps[i] = r.rejects[i] = r.pulses[i] = r.failed[i] = 0.0;
         z->pulse_index[i] = 0.0;
-        b[off[2] + i] = b[off[3] + i] = b[off[4] + i] = b[off[5] + i] = 0.0;   // steps, rejects, pulses, failed
     }
 }
 
 SIMK_HOST_API void simk_dbd_prepare_rates(SimkDbdZones *z, const SimkModel *model) {
-    u64 off[14];
-    zone_rows(z, off);
-    const u64 p = z->padded;
+    const Rows r(z);
     f64 k[256];
     for (u32 i = 0; i < z->padded; ++i) {
-        const f64 M = z->p_Pa[i] / (kBoltzmann * z->T_K[i]) * 1e-6;
-        simk_model_rate_constants(model, z->T_K[i], M, k);
-        for (u32 r = 0; r < z->n_reactions; ++r) z->block[off[6] + r * p + i] = k[r];
+        const f64 third_body = z->p_Pa[i] / (kBoltzmann * z->T_K[i]) * 1e-6;   // cm^-3
+        simk_model_rate_constants(model, z->T_K[i], third_body, k);
+        for (u32 q = 0; q < z->n_reactions; ++q) r.k[q * r.p + i] = k[q];
     }
 }
+
+namespace {
+// The discharge of one zone during a master step. Pulsed: a burst every
+// `period` seconds of absolute time (bursts numbered from 1; the zone's
+// pulse_index counts those already applied), each depositing power * period;
+// the bursts with absolute time in (t0, t1] are handed to the solver as a
+// count and the first one's local time. A filament energy density sets the
+// burst rate to power / filament energy; otherwise there are two bursts per
+// voltage cycle. Averaged: the same power as a steady source per second.
+struct Discharge {
+    bool on;
+    f64 bursts_per_s;
+};
+Discharge discharge_of(const SimkDbdZones *z, u32 i, const SimkAirSource *src) {
+    Discharge d;
+    d.on = z->power_W_cm3[i] > 0.0 && z->freq_Hz[i] > 0.0;
+    d.bursts_per_s = (d.on && src->filament_J_cm3 > 0.0) ? z->power_W_cm3[i] / src->filament_J_cm3
+                                                          : 2.0 * z->freq_Hz[i];
+    return d;
+}
+}  // namespace
 
 SIMK_HOST_API s32 simk_dbd_prepare_source(SimkDbdZones *z, const SimkAirSpecies *sp, const SimkAirSource *src,
                                           u32 mode, f64 t0, f64 t1) {
     if (src->boltzmann || z->n_species > 64) return 1;
-    u64 off[14];
-    zone_rows(z, off);
-    const u64 p = z->padded;
+    const Rows r(z);
     const u32 n = z->n_species;
-    f64 *b = z->block;
-    f64 y[64], delta[64];
+    f64 gas[64], change[64];
     for (u32 i = 0; i < z->padded; ++i) {
-        for (u32 s = 0; s < n; ++s) y[s] = b[off[0] + s * p + i];
-        const f64 freq = z->freq_Hz[i];
-        const bool on = z->power_W_cm3[i] > 0.0 && freq > 0.0;
-        // one micro-discharge burst per half-cycle, or a filament of
-        // filament_J_cm3 at rate power / filament_J_cm3
-        const f64 pulses_per_s =
-            src->filament_J_cm3 > 0.0 && on ? z->power_W_cm3[i] / src->filament_J_cm3 : 2.0 * freq;
-        if (mode == SimkDbd_Pulsed && on) {
-            simk_air_source_delta(sp, src, y, n, z->power_W_cm3[i] / pulses_per_s, z->T_K[i], delta);
-            for (u32 s = 0; s < n; ++s) {
-                b[off[8] + s * p + i] = delta[s];   // pulse
-                b[off[7] + s * p + i] = 0.0;        // src
-            }
-            const f64 period = 1.0 / pulses_per_s;
-            b[off[9] + i] = period;
-            f64 next = z->pulse_index[i] + 1.0, count = 0.0;
-            while ((next + count) * period <= t1) count += 1.0;
-            b[off[10] + i] = next * period - t0;   // pulse_first, local time
-            b[off[11] + i] = count;                // pulse_count
-        } else {
-            simk_air_source_delta(sp, src, y, n, on ? z->power_W_cm3[i] : 0.0, z->T_K[i], delta);   // per second
-            for (u32 s = 0; s < n; ++s) {
-                b[off[7] + s * p + i] = delta[s];
-                b[off[8] + s * p + i] = 0.0;
-            }
-            b[off[9] + i] = 0.0;
-            b[off[10] + i] = 0.0;
-            b[off[11] + i] = 0.0;
+        for (u32 s = 0; s < n; ++s) gas[s] = r.y[s * r.p + i];
+        const Discharge d = discharge_of(z, i, src);
+        const bool pulsed = mode == SimkDbd_Pulsed && d.on;
+        // energy of one burst, or the power itself as a rate
+        const f64 energy = pulsed ? z->power_W_cm3[i] / d.bursts_per_s : (d.on ? z->power_W_cm3[i] : 0.0);
+        simk_air_source_delta(sp, src, gas, n, energy, z->T_K[i], change);
+        f64 *to = pulsed ? r.pulse : r.src, *zeroed = pulsed ? r.src : r.pulse;
+        for (u32 s = 0; s < n; ++s) {
+            to[s * r.p + i] = change[s];
+            zeroed[s * r.p + i] = 0.0;
         }
+        if (!pulsed) {
+            r.period[i] = r.first[i] = r.count[i] = 0.0;
+            continue;
+        }
+        const f64 period = 1.0 / d.bursts_per_s;
+        const f64 first_burst = z->pulse_index[i] + 1.0;
+        f64 due = 0.0;   // bursts first_burst, first_burst + 1, ... with absolute time <= t1
+        for (;;) {
+            if (!((first_burst + due) * period <= t1)) break;
+            due += 1.0;
+        }
+        r.period[i] = period;
+        r.first[i] = first_burst * period - t0;
+        r.count[i] = due;
     }
     return 0;
 }
 
 SIMK_HOST_API void simk_dbd_commit_pulses(SimkDbdZones *z) {
-    u64 off[14];
-    zone_rows(z, off);
-    for (u32 i = 0; i < z->padded; ++i) z->pulse_index[i] += z->block[off[11] + i];
+    const Rows r(z);
+    for (u32 i = 0; i < z->padded; ++i) z->pulse_index[i] = z->pulse_index[i] + r.count[i];
 }
 
 SIMK_HOST_API void simk_dbd_default_control(KinControl *c) {

@@ -34,21 +34,45 @@ struct SimkModel {
     }
 };
 
-// Element counts of a formula made of N, O, H and digits; "(...)" labels a state.
+// Atoms of each element in a species name: a run of element letters, each
+// optionally followed by a count ("HNO3", "N2O5"); anything from '(' on names
+// an excited state and carries no atoms ("N2(A)", "O(1D)"). False when a
+// letter is not an element.
 static bool parse_formula(const char *name, u8 counts[3]) {
     counts[0] = counts[1] = counts[2] = 0;
-    const char *p = name;
-    while (*p && *p != '(') {
-        int el = -1;
-        for (int e = 0; e < 3; ++e)
-            if (*p == kElementLetters[e]) el = e;
-        if (el < 0) return false;
-        ++p;
-        u32 count = 0;
-        while (*p >= '0' && *p <= '9') count = count * 10 + (u32)(*p++ - '0');
-        counts[el] = (u8)(counts[el] + (count ? count : 1));
+    for (const char *c = name; *c && *c != '(';) {
+        const char *hit = strchr(kElementLetters, *c);
+        if (!hit || !*c) return false;
+        const int element = (int)(hit - kElementLetters);
+        ++c;
+        u32 atoms = 0;
+        for (; *c >= '0' && *c <= '9'; ++c) atoms = atoms * 10 + (u32)(*c - '0');
+        counts[element] = (u8)(counts[element] + (atoms == 0 ? 1 : atoms));
     }
     return true;
+}
+
+// The species names of one side of an equation: '+'-separated, surrounding
+// blanks ignored, names of at most 15 characters (a longer run is cut into
+// 15-character pieces). The third body "M" is dropped: its density is part of
+// the rate constant.
+static std::vector<std::string> side_species(const std::string &side) {
+    std::vector<std::string> out;
+    std::string name;
+    auto flush = [&]() {
+        if (!name.empty() && name != "M") out.push_back(name);
+        name.clear();
+    };
+    for (char ch : side) {
+        if (ch == ' ' || ch == '+') {
+            flush();
+            continue;
+        }
+        if (name.size() == 15) flush();
+        name += ch;
+    }
+    flush();
+    return out;
 }
 
 extern "C" {
@@ -78,55 +102,56 @@ static int find_species(const SimkModel *m, const std::string &name) {
 }
 
 SIMK_HOST_API u32 simk_model_add_reaction(SimkModel *m, const char *equation, const SimkRateLaw *law) {
-    u32 r = m->mech.R++;
+    const u32 r = m->mech.R++;
     m->mech.rx.emplace_back();
     Reaction &rx = m->mech.rx.back();
-    SimkRateLaw l{};
-    if (law) l = *law;
-    m->laws.push_back(l);
+    m->laws.push_back(law ? *law : SimkRateLaw{});
     m->equations.emplace_back(equation);
+    const SimkRateLaw &l = m->laws.back();
 
+    // "reactants -> products"; a later "->" only separates products
+    const std::string eq(equation);
+    const size_t arrow = eq.find("->");
+    if (arrow == std::string::npos) {
+        m->fail("no '->' between reactants and products", equation);
+        return r;
+    }
+    std::string right = eq.substr(arrow + 2);
+    for (size_t at; (at = right.find("->")) != std::string::npos;) right.replace(at, 2, " ");
+    const std::vector<std::string> lhs = side_species(eq.substr(0, arrow)), rhs = side_species(right);
+
+    // stoichiometry: -1 per reactant occurrence, +1 per product occurrence;
+    // atoms must balance element by element
     std::vector<int> net(m->mech.n, 0);
-    int balance[3] = {0, 0, 0};
-    bool products = false;
-    const char *p = equation;
-    while (*p) {
-        while (*p == ' ' || *p == '+') ++p;
-        if (!*p) break;
-        if (p[0] == '-' && p[1] == '>') {
-            products = true;
-            p += 2;
-            continue;
-        }
-        std::string tok;
-        while (*p && *p != ' ' && *p != '+' && !(p[0] == '-' && p[1] == '>') && tok.size() < 15) tok += *p++;
-        if (tok == "M") continue;
-        int s = find_species(m, tok);
-        if (s < 0) {
-            m->fail("unknown species in reaction", equation);
-            return r;
-        }
-        if (!products) {
-            if (rx.reactant_count >= SIMK_MAX_REACTANTS) {
-                m->fail("too many reactants", equation);
+    int atoms[3] = {0, 0, 0};
+    for (int side = 0; side < 2; ++side) {
+        const int sign = side == 0 ? -1 : 1;
+        for (const std::string &name : side == 0 ? lhs : rhs) {
+            const int s = find_species(m, name);
+            if (s < 0) {
+                m->fail("unknown species in reaction", equation);
                 return r;
             }
-            rx.reactant[rx.reactant_count++] = (u8)s;
-            net[s] -= 1;
-            for (int e = 0; e < 3; ++e) balance[e] -= m->elements[s][e];
-        } else {
-            net[s] += 1;
-            for (int e = 0; e < 3; ++e) balance[e] += m->elements[s][e];
+            if (side == 0) {
+                if (rx.reactant_count == SIMK_MAX_REACTANTS) {
+                    m->fail("more reactants than a reaction may have", equation);
+                    return r;
+                }
+                rx.reactant[rx.reactant_count++] = (u8)s;
+            }
+            net[s] += sign;
+            for (int e = 0; e < 3; ++e) atoms[e] += sign * m->elements[s][e];
         }
     }
-    if (!products || rx.reactant_count == 0) m->fail("reaction needs reactants and '->'", equation);
-    for (int e = 0; e < 3; ++e)
-        if (balance[e] != 0) m->fail("element balance violated", equation);
-    if (l.kind == SimkRate_EqReverse && l.ref >= r) m->fail("reverse reaction must follow its forward reaction", equation);
+    if (rx.reactant_count == 0) m->fail("reaction without reactants", equation);
+    if (atoms[0] || atoms[1] || atoms[2]) m->fail("element balance violated", equation);
+    if (l.kind == SimkRate_EqReverse && l.ref >= r)
+        m->fail("an equilibrium reverse must come after the reaction it reverses", equation);
+    // species whose count changes, in species order
     for (u32 s = 0; s < m->mech.n; ++s) {
-        if (net[s] == 0) continue;
-        if (rx.net_count >= SIMK_MAX_NET) {
-            m->fail("too many net species", equation);
+        if (!net[s]) continue;
+        if (rx.net_count == SIMK_MAX_NET) {
+            m->fail("more changed species than a reaction may have", equation);
             return r;
         }
         rx.net_species[rx.net_count] = (u8)s;

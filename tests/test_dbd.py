@@ -84,6 +84,20 @@ def test_air_feed_and_source_equal_reference(air):
                 lib.ref_air_source_delta(yy.ctypes.data, energy, T, 0, 0.0, dw.ctypes.data)
                 assert d.tobytes() == dw.tobytes()
 
+    # trace partners under a large burst: channels hit the half-the-partner
+    # cap and hand their surplus to the leading channel; a gas with no
+    # partners at all sends everything to channel 0
+    sp = air.sp
+    for O2, NO, O, N2O, H2O in ((1e10, 1e14, 1e11, 1e9, 0.0), (0.0, 0.0, 0.0, 0.0, 0.0), (1e8, 3e12, 1e13, 5e12, 1e9)):
+        yy = np.zeros(17)
+        yy[sp.N2], yy[sp.O2], yy[sp.NO], yy[sp.O], yy[sp.N2O], yy[sp.H2O] = 2.4e19, O2, NO, O, N2O, H2O
+        for energy in (2e-5, 3e-2):
+            d, dw = np.zeros(17), np.zeros(17)
+            _ffi.host().simk_air_source_delta(C.byref(air.sp), C.byref(src), yy.ctypes.data_as(_ffi.f64p), 17,
+                                              energy, 330.0, d.ctypes.data_as(_ffi.f64p))
+            lib.ref_air_source_delta(yy.ctypes.data, energy, 330.0, 0, 0.0, dw.ctypes.data)
+            assert d.tobytes() == dw.tobytes()
+
 
 def test_air_source_conserves_atoms(air):
     """Every channel of the discharge source conserves N, O and H atoms

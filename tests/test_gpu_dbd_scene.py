@@ -17,8 +17,11 @@ pytestmark = pytest.mark.gpu
 HASHES = {0: 0x9BEB634425FAF2C2, 1: 0x32256818F6F35BC9}
 
 
-@pytest.mark.parametrize("kernel", [1, 2], ids=["generated", "table"])
-@pytest.mark.parametrize("precision", [0, 1], ids=["f64", "f32"])
+# The f32 table walk is left out: one launch per master step of a single
+# wave, each vector update a dependent workspace round trip, takes minutes
+# for the 500 steps (it is checked bit for bit on 5-step runs in
+# test_gpu_dbd.py).
+@pytest.mark.parametrize("precision,kernel", [(0, 1), (0, 2), (1, 1)], ids=["f64-generated", "f64-table", "f32-generated"])
 def test_scene_hash_known_answer_device(gm, precision, kernel):
     print(f"[case] dbd scene, 64 zones x 500 steps, precision {precision}, kernel {kernel}", flush=True)
     air = dbd.AirMechanism(caps=_ffi.DEFAULT_CAPS)

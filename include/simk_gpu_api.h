@@ -96,6 +96,11 @@ GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *launch, u32 repeats, f32 *ker
  * share the one array per GPU, so two calls that overlap on a GPU race. */
 GPU_API s32 simk_kin_advance_static(const GpuKinLaunch *launch, u32 repeats, f32 *kernel_ms);
 
+/* Lanes per cell in cooperative mode (gpu_kin_set_mode(1)): 64 (one wave per
+ * cell, the default), 32, 16 or 8 (that many cells share a wave, each team
+ * with its own LDS region and control flow). Results do not depend on it. */
+GPU_API s32 simk_kin_set_team(u32 lanes_per_cell);
+
 /* Persistent grid size in waves (0: as many as are resident on the device). */
 GPU_API s32 simk_kin_set_grid(u32 waves);
 
@@ -267,7 +272,8 @@ typedef struct SimkServeConfig {
     u32 max_requests;    /* queued + admitted at once (0: 4096) */
     u32 steps;           /* operator-split steps per request (0: 20) */
     u32 substeps;        /* advection substeps per step (0: 4) */
-    u32 slice_us;        /* chemistry time slice per iteration, microseconds (0: 20000; ~0u: no slicing) */
+    u32 slice_us;        /* chemistry time slice per iteration, microseconds (0: 20000; ~0u: no slicing); four
+                            times as long while no submitted request waits for admission */
     f64 dt;              /* chemistry seconds per step (0: 1e-3) */
 } SimkServeConfig;
 

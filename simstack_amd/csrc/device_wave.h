@@ -15,9 +15,16 @@
 //           order of kin_solve
 //   vectors stage combinations, y_new and the error terms in parallel over
 //           species; the error norm is summed in species order by one lane
-// Control (step size, accept/reject, pulses) is wave-uniform and computed
-// identically by every lane. T = double (FP64 lanes) or float with DELTA (the
-// reference's f32 delta lanes, dbd_lanes.h:21-72).
+// Control (step size, accept/reject, pulses) is uniform over the lanes of a
+// cell and computed identically by each of them. T = double (FP64 lanes) or
+// float with DELTA (the reference's f32 delta lanes, dbd_lanes.h:21-72).
+//
+// Teams: L lanes per cell (8, 16, 32 or 64), 64 / L cells per wave, each team
+// with its own LDS region, queue position and control flow. Teams of one wave
+// diverge (different step counts), so teams narrower than the wave order
+// their LDS traffic with wave-scope fences instead of workgroup barriers: the
+// lanes of a wave issue LDS operations in order, and the fences keep the
+// compiler from moving a team's loads and stores across a phase boundary.
 #pragma once
 
 #include "device_delta32.h"
@@ -85,37 +92,49 @@ struct SimkWaveState {
     }
 };
 
-// LDS writes by one lane visible to the others (the block is one wave).
-SIMK_DEV void simk_wave_sync() { __syncthreads(); }
+// LDS writes by one lane visible to the other lanes of its team: a workgroup
+// barrier when the team is the whole wave (uniform control), else wave-scope
+// fences around a wave barrier (a compiler-level ordering point; the wave
+// executes its LDS operations in order).
+template <simk_u32 L>
+SIMK_DEV void simk_team_sync() {
+    if constexpr (L == SIMK_WAVE) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
 
 // f = src + sum_r net_r k_r prod y at pt (kin_fun, kin_solver.h:39-53).
-template <class T>
+template <class T, simk_u32 L>
 SIMK_DEV void simk_wave_fun(const SimkDevTables *D, const SimkWaveTables *W, SimkWaveState<T> &S, const T *pt,
                             T *f) {
-    const simk_u32 lane = threadIdx.x;
-    for (simk_u32 r = lane; r < S.R; r += SIMK_WAVE) {
+    const simk_u32 lane = threadIdx.x % L;
+    for (simk_u32 r = lane; r < S.R; r += L) {
         T rate = S.k[r];
         const simk_u32 rc = D->reactant_count[r];
         for (simk_u32 p = 0; p < rc; ++p) rate = rate * pt[D->reactant[r * 3 + p]];
         S.rate[r] = rate;
     }
-    simk_wave_sync();
-    for (simk_u32 s = lane; s < S.n; s += SIMK_WAVE) {
+    simk_team_sync<L>();
+    for (simk_u32 s = lane; s < S.n; s += L) {
         T v = S.src[s];
         for (simk_u32 e = W->fun_begin[s]; e < W->fun_begin[s + 1]; ++e)
             v = simk_t_fma((T)W->fun_coef[e], S.rate[W->fun_r[e]], v);
         f[s] = v;
     }
-    simk_wave_sync();
+    simk_team_sync<L>();
 }
 
 // a = -J at pt, then the in-place LU of I/(h gamma) - J (kin_neg_jac,
 // kin_factor, kin_solver.h:75-113).
-template <class T>
+template <class T, simk_u32 L>
 SIMK_DEV void simk_wave_jac_factor(const SimkDevTables *D, const SimkWaveTables *W, SimkWaveState<T> &S,
                                    const T *pt, const T ghinv) {
-    const simk_u32 lane = threadIdx.x;
-    for (simk_u32 item = lane; item < 3 * S.R; item += SIMK_WAVE) {
+    const simk_u32 lane = threadIdx.x % L;
+    for (simk_u32 item = lane; item < 3 * S.R; item += L) {
         const simk_u32 r = item / 3, p = item % 3;
         const simk_u32 rc = D->reactant_count[r];
         if (p >= rc) continue;
@@ -124,65 +143,65 @@ SIMK_DEV void simk_wave_jac_factor(const SimkDevTables *D, const SimkWaveTables 
             if (q != p) d = d * pt[D->reactant[r * 3 + q]];
         S.deriv[item] = d;
     }
-    simk_wave_sync();
-    for (simk_u32 idx = lane; idx < S.nnz; idx += SIMK_WAVE) {
+    simk_team_sync<L>();
+    for (simk_u32 idx = lane; idx < S.nnz; idx += L) {
         T v = T(0);
         for (simk_u32 e = W->jac_begin[idx]; e < W->jac_begin[idx + 1]; ++e)
             v = simk_t_fma((T)W->jac_coef[e], S.deriv[W->jac_rp[e]], v);
         S.lu[idx] = v;
     }
-    simk_wave_sync();
+    simk_team_sync<L>();
     for (simk_u32 i = 0; i < S.n; ++i) {
         for (simk_u32 d = D->div_begin[i]; d < D->div_begin[i + 1]; ++d) {
             const unsigned short *dv = D->lu_div + 4 * d;
             const simk_u32 ik = dv[0], kk = dv[1], s0 = dv[2], s1 = dv[3];
             const T l = S.lu[ik] * S.lu[kk];
-            simk_wave_sync();   // every lane has read lu[ik] before it is overwritten
+            simk_team_sync<L>();   // every lane has read lu[ik] before it is overwritten
             if (lane == 0) S.lu[ik] = l;
-            for (simk_u32 s = s0 + lane; s < s1; s += SIMK_WAVE) {
+            for (simk_u32 s = s0 + lane; s < s1; s += L) {
                 const simk_u32 ij = D->lu_sub[2 * s], kj = D->lu_sub[2 * s + 1];
                 S.lu[ij] = simk_t_fma(-l, S.lu[kj], S.lu[ij]);
             }
-            simk_wave_sync();
+            simk_team_sync<L>();
         }
         if (lane == 0) {
             const simk_u32 ii = D->diag[i];
             S.lu[ii] = T(1) / (S.lu[ii] + ghinv);
         }
-        simk_wave_sync();
+        simk_team_sync<L>();
     }
 }
 
 // x = (LU)^-1 x by columns (kin_solve, kin_solver.h:122-139).
-template <class T>
+template <class T, simk_u32 L>
 SIMK_DEV void simk_wave_solve(const SimkDevTables *D, const SimkWaveTables *W, SimkWaveState<T> &S, T *x) {
-    const simk_u32 lane = threadIdx.x;
+    const simk_u32 lane = threadIdx.x % L;
     for (simk_u32 c = 0; c < S.n; ++c) {
         const T xc = x[c];
-        for (simk_u32 e = W->lcol_begin[c] + lane; e < W->lcol_begin[c + 1]; e += SIMK_WAVE) {
+        for (simk_u32 e = W->lcol_begin[c] + lane; e < W->lcol_begin[c + 1]; e += L) {
             const simk_u32 row = W->lcol_row[e];
             x[row] = simk_t_fma(-S.lu[W->lcol_idx[e]], xc, x[row]);
         }
-        simk_wave_sync();
+        simk_team_sync<L>();
     }
     for (simk_u32 c = S.n; c-- > 0;) {
         if (lane == 0) x[c] = x[c] * S.lu[D->diag[c]];
-        simk_wave_sync();
+        simk_team_sync<L>();
         const T xc = x[c];
-        for (simk_u32 e = W->ucol_begin[c] + lane; e < W->ucol_begin[c + 1]; e += SIMK_WAVE) {
+        for (simk_u32 e = W->ucol_begin[c] + lane; e < W->ucol_begin[c + 1]; e += L) {
             const simk_u32 row = W->ucol_row[e];
             x[row] = simk_t_fma(-S.lu[W->ucol_idx[e]], xc, x[row]);
         }
-        simk_wave_sync();
+        simk_team_sync<L>();
     }
 }
 
 // One step attempt (kin_integrate's loop body, kin_solver.h:216-311).
-template <class T, bool DELTA>
+template <class T, bool DELTA, simk_u32 L>
 SIMK_DEV void simk_wave_attempt(const SimkDevTables *D, const SimkWaveTables *W, SimkWaveState<T> &S,
                                 const SimkTConsts<T> &C, T &t, const T t_end, T &h, T &steps, T &rejects,
                                 bool &reject_last, bool &reject_more) {
-    const simk_u32 lane = threadIdx.x, n = S.n;
+    const simk_u32 lane = threadIdx.x % L, n = S.n;
     const T one = T(1);
     const T remaining = t_end - t;
     T h_cap = remaining;
@@ -193,40 +212,40 @@ SIMK_DEV void simk_wave_attempt(const SimkDevTables *D, const SimkWaveTables *W,
 
     const T *pt = S.y;
     if (DELTA) {
-        for (simk_u32 s = lane; s < n; s += SIMK_WAVE) S.yeval[s] = S.base[s] + S.y[s];
-        simk_wave_sync();
+        for (simk_u32 s = lane; s < n; s += L) S.yeval[s] = S.base[s] + S.y[s];
+        simk_team_sync<L>();
         pt = S.yeval;
     }
-    simk_wave_fun(D, W, S, pt, S.fcn);
+    simk_wave_fun<T, L>(D, W, S, pt, S.fcn);
     const T ghinv = one / (hs * C.gamma0);
-    simk_wave_jac_factor(D, W, S, pt, ghinv);
+    simk_wave_jac_factor<T, L>(D, W, S, pt, ghinv);
     const T hinv = one / hs;
 
     for (simk_u32 st = 0; st < C.stages; ++st) {
         const simk_u32 packed = st * (st - 1) / 2;
         T *Kst = S.K + st * n;
         if (st > 0 && ((C.new_f >> st) & 1u)) {
-            for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+            for (simk_u32 s = lane; s < n; s += L) {
                 T v = S.y[s];
                 for (simk_u32 j = 0; j < st; ++j)
                     if ((C.a_nz >> (packed + j)) & 1u) v = simk_t_fma(C.a[packed + j], S.K[j * n + s], v);
                 S.ytmp[s] = v;
                 if (DELTA) S.yeval[s] = S.base[s] + v;
             }
-            simk_wave_sync();
-            simk_wave_fun(D, W, S, DELTA ? S.yeval : S.ytmp, S.fcn);
+            simk_team_sync<L>();
+            simk_wave_fun<T, L>(D, W, S, DELTA ? S.yeval : S.ytmp, S.fcn);
         }
-        for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+        for (simk_u32 s = lane; s < n; s += L) {
             T v = S.fcn[s];
             for (simk_u32 j = 0; j < st; ++j)
                 if ((C.c_nz >> (packed + j)) & 1u) v = simk_t_fma(C.c[packed + j] * hinv, S.K[j * n + s], v);
             Kst[s] = v;
         }
-        simk_wave_sync();
-        simk_wave_solve(D, W, S, Kst);
+        simk_team_sync<L>();
+        simk_wave_solve<T, L>(D, W, S, Kst);
     }
 
-    for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+    for (simk_u32 s = lane; s < n; s += L) {
         T ynew = S.y[s], yerr = T(0);
         for (simk_u32 st = 0; st < C.stages; ++st) {
             const T kv = S.K[st * n + s];
@@ -241,13 +260,13 @@ SIMK_DEV void simk_wave_attempt(const SimkDevTables *D, const SimkWaveTables *W,
         S.q[s] = yerr / scale;
         S.ytmp[s] = ynew;
     }
-    simk_wave_sync();
+    simk_team_sync<L>();
     if (lane == 0) {
         T err = T(0);
         for (simk_u32 s = 0; s < n; ++s) err = simk_t_fma(S.q[s], S.q[s], err);
         S.scal[0] = err;
     }
-    simk_wave_sync();
+    simk_team_sync<L>();
     T err = simk_t_sqrt(S.scal[0] * C.inv_n);
     if (!simk_t_finite(err)) err = T(1e10);
 
@@ -258,7 +277,7 @@ SIMK_DEV void simk_wave_attempt(const SimkDevTables *D, const SimkWaveTables *W,
         fac = one;
     }
     if (ok) {
-        for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+        for (simk_u32 s = lane; s < n; s += L) {
             T v = S.ytmp[s];
             if (C.clamp_negative) v = DELTA ? simk_t_max(v, -S.base[s]) : simk_t_max(v, T(0));
             S.y[s] = v;
@@ -278,35 +297,49 @@ SIMK_DEV void simk_wave_attempt(const SimkDevTables *D, const SimkWaveTables *W,
     reject_last = !ok;
     steps = steps + (ok ? one : T(0));
     rejects = rejects + (ok ? T(0) : one);
-    simk_wave_sync();
+    simk_team_sync<L>();
 }
 
-// Persistent: each wave takes cells from the queue one at a time and runs
+// Bytes of LDS one cell's team uses: the cell's state (simk_wave_lds_elems
+// values of T), then n doubles, the delta form's f64 storage between repeats.
+template <class T>
+SIMK_DEV simk_u32 simk_team_lds_bytes(simk_u32 n, simk_u32 R, simk_u32 nnz) {
+    return ((simk_wave_lds_elems(n, R, nnz) * (simk_u32)sizeof(T) + 7u) & ~7u) + 8u * n;
+}
+
+// Persistent: each team takes cells from the queue one at a time and runs
 // dbd_advance_group (load, kin_advance with pulse segments, store) on it.
-// y64: n doubles of LDS, the delta form's f64 storage between repeats.
-template <class T, bool DELTA>
-SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, const SimkWaveTables *W, T *lds,
-                             double *y64) {
+// lds: the wave's LDS, 64 / L consecutive team regions.
+template <class T, bool DELTA, simk_u32 L>
+SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, const SimkWaveTables *W,
+                             unsigned char *lds) {
+    constexpr simk_u32 kTeams = SIMK_WAVE / L;
     const SimkDevTables *D = A.tables;
-    const simk_u32 lane = threadIdx.x;
+    const simk_u32 lane = threadIdx.x % L, team = threadIdx.x / L;
+    const simk_u32 team_bytes = simk_team_lds_bytes<T>(D->n, D->R, D->nnz);
+    unsigned char *mine = lds + team * team_bytes;
+    double *y64 = (double *)(mine + ((simk_wave_lds_elems(D->n, D->R, D->nnz) * (simk_u32)sizeof(T) + 7u) & ~7u));
     SimkWaveState<T> S;
-    S.init(lds, D->n, D->R, D->nnz);
+    S.init((T *)mine, D->n, D->R, D->nnz);
     const simk_u32 n = S.n, R = S.R;
     const simk_u64 p = A.padded;
     const T duration = (T)A.duration;
-    simk_u32 pos = blockIdx.x;   // the queue head starts at gridDim.x
+    simk_u32 pos = blockIdx.x * kTeams + team;   // the queue head starts at gridDim.x * kTeams
     bool first_cell = !A.queue_first;
     for (;;) {
         if (!first_cell) {
             simk_u32 got = 0;
             if (lane == 0) got = atomicAdd(A.queue, 1u);
-            pos = (simk_u32)__builtin_amdgcn_readfirstlane((int)got);
+            if constexpr (L == SIMK_WAVE)
+                pos = (simk_u32)__builtin_amdgcn_readfirstlane((int)got);
+            else   // the team's first lane took the position: broadcast it to the team
+                pos = (simk_u32)__builtin_amdgcn_ds_bpermute((int)((team * L) << 2), (int)got);
         }
         first_cell = false;
         if (pos >= A.lanes) break;
         const simk_u64 cell = A.order ? A.order[pos] : pos;
         // dbd_advance_group's load (dbd_lanes.h:31-54)
-        for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+        for (simk_u32 s = lane; s < n; s += L) {
             const double v = A.y[s * p + cell];
             if (DELTA) {
                 y64[s] = v;
@@ -318,19 +351,19 @@ SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, cons
             S.src[s] = (T)A.src[s * p + cell];
             S.pulse[s] = (T)A.pulse[s * p + cell];
         }
-        for (simk_u32 r = lane; r < R; r += SIMK_WAVE) S.k[r] = (T)A.k[r * p + cell];
+        for (simk_u32 r = lane; r < R; r += L) S.k[r] = (T)A.k[r * p + cell];
         const T period = (T)A.period[cell], first = (T)A.pulse_first[cell], count = (T)A.pulse_count[cell];
         T h = (T)A.h[cell];
         T failed = (T)A.failed[cell];
         double acc_steps = 0.0, acc_rejects = 0.0, acc_pulses = 0.0;
-        simk_wave_sync();
+        simk_team_sync<L>();
         for (simk_u32 rep = 0; rep < A.repeats; ++rep) {
             if (DELTA && rep > 0) {   // the next call restarts from the f64 storage
-                for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+                for (simk_u32 s = lane; s < n; s += L) {
                     S.base[s] = (T)y64[s];
                     S.y[s] = T(0);
                 }
-                simk_wave_sync();
+                simk_team_sync<L>();
             }
             T steps = T(0), rejects = T(0), pulses = T(0), t = T(0), applied = T(0);
             // kin_advance (kin_solver.h:322-347)
@@ -348,14 +381,14 @@ SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, cons
                         failed = T(1);
                         break;
                     }
-                    simk_wave_attempt<T, DELTA>(D, W, S, C, t, seg_end, h, steps, rejects, reject_last, reject_more);
+                    simk_wave_attempt<T, DELTA, L>(D, W, S, C, t, seg_end, h, steps, rejects, reject_last, reject_more);
                     active = t < seg_end;
                 }
                 const bool alive = !(failed > T(0));
                 const bool hit = pending && alive && (t >= seg_end);
                 if (hit) {
-                    for (simk_u32 s = lane; s < n; s += SIMK_WAVE) S.y[s] = S.y[s] + S.pulse[s];
-                    simk_wave_sync();
+                    for (simk_u32 s = lane; s < n; s += L) S.y[s] = S.y[s] + S.pulse[s];
+                    simk_team_sync<L>();
                     applied = applied + T(1);
                     pulses = pulses + T(1);
                     h = C.h_start;
@@ -365,12 +398,12 @@ SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, cons
                 if (((t >= duration) && (applied >= count || !pending)) || !alive) break;
             }
             if (DELTA) {   // dbd_lanes.h:58-67
-                for (simk_u32 s = lane; s < n; s += SIMK_WAVE) {
+                for (simk_u32 s = lane; s < n; s += L) {
                     double v = y64[s] + (double)S.y[s];
                     if (C.clamp_negative) v = v > 0.0 ? v : 0.0;
                     y64[s] = v;
                 }
-                simk_wave_sync();
+                simk_team_sync<L>();
             }
             // per-call counters added to the f64 totals (integers: exact in any grouping)
             acc_steps = acc_steps + (double)steps;
@@ -378,7 +411,7 @@ SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, cons
             acc_pulses = acc_pulses + (double)pulses;
         }
         // dbd_advance_group's store (dbd_lanes.h:56-77)
-        for (simk_u32 s = lane; s < n; s += SIMK_WAVE) A.y[s * p + cell] = DELTA ? y64[s] : (double)S.y[s];
+        for (simk_u32 s = lane; s < n; s += L) A.y[s * p + cell] = DELTA ? y64[s] : (double)S.y[s];
         if (lane == 0) {
             A.h[cell] = (double)h;
             A.failed[cell] = (double)failed;
@@ -386,6 +419,6 @@ SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, cons
             A.rejects[cell] = A.rejects[cell] + acc_rejects;
             A.pulses[cell] = A.pulses[cell] + acc_pulses;
         }
-        simk_wave_sync();
+        simk_team_sync<L>();
     }
 }

@@ -157,13 +157,11 @@ extern "C" __global__ void __launch_bounds__(SIMK_WAVE) simk_delta32_kernel(Simk
 }
 
 // Cooperative mode: one wave per cell, cell state in LDS.
-template <class T, bool DELTA>
+template <class T, bool DELTA, simk_u32 L>
 __global__ void __launch_bounds__(SIMK_WAVE) simk_wave_kernel(SimkKinArgs A, SimkTConsts<T> C,
                                                              const SimkWaveTables *W) {
     extern __shared__ __align__(16) unsigned char simk_wave_lds[];
-    const simk_u32 elems = simk_wave_lds_elems(A.tables->n, A.tables->R, A.tables->nnz);
-    double *y64 = (double *)(simk_wave_lds + ((elems * (simk_u32)sizeof(T) + 7u) & ~7u));
-    simk_wave_body<T, DELTA>(A, C, W, (T *)simk_wave_lds, y64);
+    simk_wave_body<T, DELTA, L>(A, C, W, simk_wave_lds);
 }
 
 // The reference's M1 point demo (a spring/swirl particle step), kept so every
@@ -252,6 +250,7 @@ struct State {
     bool tuned = false;
     GenOptions user_gen;
     u32 user_mode = 0, user_grid = 0;
+    u32 team_lanes = SIMK_WAVE, user_team = SIMK_WAVE;   // lanes per cell in cooperative mode
     // current mechanism
     bool have_mech = false;
     Mech mech;
@@ -750,6 +749,7 @@ void untune() {
     if (!g.tuned) return;
     g.gen = g.user_gen;
     g.mode = g.user_mode;
+    g.team_lanes = g.user_team;
     g.grid_override = g.user_grid;
     g.tuned = false;
 }
@@ -901,17 +901,25 @@ SimkTConsts<T> t_consts() {
     return C;
 }
 
-// Cooperative mode: one wave per cell (gpu_kin_set_mode(1)); FP64 or f32 delta.
-template <class T, bool DELTA>
+// LDS bytes of one cooperative team's cell (simk_team_lds_bytes).
+template <class T>
+u64 team_cell_bytes(const Mech &m) {
+    const u64 elems = 13ull * m.n + 5ull * m.R + m.nnz + 2;
+    return ((elems * sizeof(T) + 7) & ~7ull) + 8ull * m.n;
+}
+
+// Cooperative mode (gpu_kin_set_mode(1)): teams of TL lanes per cell, 64 / TL
+// cells per wave; FP64 or f32 delta.
+template <class T, bool DELTA, u32 TL>
 hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     const Mech &m = g.mech;
-    const u64 elems = 13ull * m.n + 5ull * m.R + m.nnz + 2;
-    const u64 lds = ((elems * sizeof(T) + 7) & ~7ull) + 8ull * m.n;
+    constexpr u32 teams = SIMK_WAVE / TL;
+    const u64 lds = teams * team_cell_bytes<T>(m);
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, g.device);
     if (e != hipSuccess) return e;
-    if (lds > prop.sharedMemPerBlock) return hipErrorInvalidValue;   // the cell does not fit one workgroup's LDS
-    const auto kernel = simk_wave_kernel<T, DELTA>;
+    if (lds > prop.sharedMemPerBlock) return hipErrorInvalidValue;   // the wave's cells do not fit its LDS
+    const auto kernel = simk_wave_kernel<T, DELTA, TL>;
     if (lds > 65536) {
         e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -921,12 +929,13 @@ hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
         blocks = 1;
     u32 resident = (u32)blocks * (g.cus ? g.cus : 1);
     if (g.grid_override) resident = g.grid_override;
-    u32 waves = lanes < resident ? lanes : resident;
+    const u32 needed = (lanes + teams - 1) / teams;
+    u32 waves = needed < resident ? needed : resident;
     double *ws = nullptr;
     u32 *queue = nullptr;
     e = launch_buffers(0, waves, ws, queue);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, queue, waves);
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, queue, waves * teams);
     SimkKinArgs A;
     memset(&A, 0, sizeof(A));
     A.y = L->y; A.h = L->h; A.steps = L->steps; A.rejects = L->rejects; A.pulses = L->pulses; A.failed = L->failed;
@@ -945,7 +954,9 @@ hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     A.method = g.dev_method;
     const SimkTConsts<T> C = t_consts<T>();
     g.last_kernel = DELTA ? 7 : 6;
-    g.last_name = DELTA ? "simk_wave_kernel<f32,delta>" : "simk_wave_kernel<f64>";
+    char name[64];
+    snprintf(name, sizeof(name), "simk_wave_kernel<%s,%u lanes per cell>", DELTA ? "f32 delta" : "f64", TL);
+    g.last_name = name;
     g.last_slots = 0;
     g.last_waves = waves;
     hipLaunchKernelGGL(kernel, dim3(waves), dim3(SIMK_WAVE), (u32)lds, g.stream, A, C,
@@ -953,10 +964,20 @@ hipError_t launch_wave_t(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     return hipGetLastError();
 }
 
+template <class T, bool DELTA>
+hipError_t launch_wave_lanes(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
+    switch (g.team_lanes) {
+    case 8: return launch_wave_t<T, DELTA, 8>(L, lanes, repeats);
+    case 16: return launch_wave_t<T, DELTA, 16>(L, lanes, repeats);
+    case 32: return launch_wave_t<T, DELTA, 32>(L, lanes, repeats);
+    default: return launch_wave_t<T, DELTA, SIMK_WAVE>(L, lanes, repeats);
+    }
+}
+
 hipError_t launch_wave(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     if (!g.d_wave) return hipErrorInvalidValue;
-    if (L->precision == 0) return launch_wave_t<f64, false>(L, lanes, repeats);
-    if (L->precision == 1) return launch_wave_t<f32, true>(L, lanes, repeats);
+    if (L->precision == 0) return launch_wave_lanes<f64, false>(L, lanes, repeats);
+    if (L->precision == 1) return launch_wave_lanes<f32, true>(L, lanes, repeats);
     return hipErrorNotSupported;   // mixed precision has no cooperative form
 }
 
@@ -1469,6 +1490,14 @@ GPU_API s32 gpu_kin_set_mode(u32 mode) {
     return 0;
 }
 
+GPU_API s32 simk_kin_set_team(u32 lanes_per_cell) {
+    if (lanes_per_cell != 8 && lanes_per_cell != 16 && lanes_per_cell != 32 && lanes_per_cell != SIMK_WAVE)
+        return err_code(hipErrorInvalidValue);
+    g.team_lanes = lanes_per_cell;
+    g.user_team = lanes_per_cell;
+    return 0;
+}
+
 GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *L, u32 repeats, f32 *kernel_ms) {
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
     (void)hipEventRecord(g.kin_start, g.stream);
@@ -1754,10 +1783,11 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     e = tune_sample(g.mech, L, cells, S);
     if (duration > 0.0) S.orig.duration = S.work.duration = duration;
     const u32 saved_mode = g.mode, saved_grid = g.grid_override, saved_kind = g.kernel_kind;
+    const u32 saved_team = g.team_lanes;
     const GenOptions saved_gen = g.gen;
     const u32 cus = g.cus ? g.cus : 1;
     f32 best = 1e30f, t = 0.0f;
-    u32 best_mode = 0, best_grid = 0;
+    u32 best_mode = 0, best_grid = 0, best_team = saved_team;
     GenOptions best_gen = g.gen;
     auto consider = [&](u32 mode, u32 grid, const GenOptions &opt, u32 sub = 0) -> hipError_t {
         g.mode = mode;
@@ -1787,6 +1817,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             best_mode = mode;
             best_grid = grid;
             best_gen = opt;
+            best_team = g.team_lanes;
         }
         return hipSuccess;
     };
@@ -1821,6 +1852,12 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.reuse = r;
             e = consider(0, best_grid, o);
         }
+    // register budget: at least two waves per SIMD (at most 256 VGPRs a lane)
+    if (e == hipSuccess) {
+        GenOptions o = best_gen;
+        o.waves_per_eu = 2;
+        e = consider(0, best_grid, o);
+    }
     // LU entries in LDS: none, and shares for 4, 3 and 2 waves per CU
     const u32 lds_now = lds_entries(g.mech, resolved(best_gen));
     const u32 ldss[] = {0, 80, 128, 160};
@@ -1830,19 +1867,36 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
             o.lds_slots = l;
             e = consider(0, best_grid, o);
         }
-    // 3. one wave per cell, timed on at most 16 resident rounds of its grid
-    if (e == hipSuccess) {
-        int blocks = 1;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_wave_kernel<f64, false>, SIMK_WAVE,
-                                                           (13ull * g.mech.n + 5ull * g.mech.R + g.mech.nnz + 3) * 8 +
-                                                               8ull * g.mech.n);
-        e = consider(1, 0, best_gen, 16u * (u32)(blocks > 0 ? blocks : 1) * cus);
+    // 3. teams of 64, 32, 16 and 8 lanes per cell (cooperative mode), each
+    // timed on at most 16 resident rounds of its grid
+    {
+        hipDeviceProp_t prop;
+        const bool have_prop = hipGetDeviceProperties(&prop, g.device) == hipSuccess;
+        const u32 team_sizes[] = {SIMK_WAVE, 32, 16, 8};
+        for (u32 tl : team_sizes) {
+            if (e != hipSuccess) break;
+            const u64 lds = (SIMK_WAVE / tl) * team_cell_bytes<f64>(g.mech);
+            if (!have_prop || lds > prop.sharedMemPerBlock) continue;
+            int blocks = 1;
+            hipError_t oc = hipErrorInvalidValue;
+            switch (tl) {
+            case 8: oc = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_wave_kernel<f64, false, 8>, SIMK_WAVE, lds); break;
+            case 16: oc = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_wave_kernel<f64, false, 16>, SIMK_WAVE, lds); break;
+            case 32: oc = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_wave_kernel<f64, false, 32>, SIMK_WAVE, lds); break;
+            default: oc = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, simk_wave_kernel<f64, false, SIMK_WAVE>, SIMK_WAVE, lds); break;
+            }
+            if (oc != hipSuccess || blocks < 1) blocks = 1;
+            g.team_lanes = tl;
+            e = consider(1, 0, best_gen, 16u * (u32)blocks * cus * (SIMK_WAVE / tl));
+        }
+        g.team_lanes = saved_team;
     }
     if (S.mem) (void)hipFree(S.mem);
     g.kernel_kind = saved_kind;
     if (e != hipSuccess) {
         g.mode = saved_mode;
         g.grid_override = saved_grid;
+        g.team_lanes = saved_team;
         return err_code(e);
     }
     // apply the winner
@@ -1850,10 +1904,12 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
         g.tuned = true;
         g.user_gen = saved_gen;
         g.user_mode = saved_mode;
+        g.user_team = saved_team;
         g.user_grid = saved_grid;
     }
     g.mode = best_mode;
     g.grid_override = best_grid;
+    if (best_mode == 1) g.team_lanes = best_team;
     if (!same_gen(best_gen, g.gen)) {
         JitKernel k;
         if (load_generated(best_gen, k)) {
@@ -1864,7 +1920,7 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
         }
     }
     out->mode = best_mode;
-    out->lanes_per_cell = best_mode ? SIMK_WAVE : 1;
+    out->lanes_per_cell = best_mode ? g.team_lanes : 1;
     out->waves = best_grid;
     out->batch = resolved(best_gen).batch;
     out->reg_window = best_gen.reg_window;

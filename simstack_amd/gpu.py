@@ -62,7 +62,7 @@ REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_use
                      "gpu_copy_to_device", "gpu_copy_to_host", "gpu_sync", "gpu_step_points", "gpu_kin_set_mech",
                      "gpu_kin_advance", "gpu_host_register", "gpu_host_unregister", "gpu_kin_advance_async",
                      "gpu_kin_wait", "gpu_kin_set_mode")
-EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_advance_static", "simk_kin_set_grid", "simk_kin_set_order",
+EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_advance_static", "simk_kin_set_team", "simk_kin_set_grid", "simk_kin_set_order",
                      "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
                      "simk_jit_precompile", "simk_copy_device", "simk_copy_device_b64", "simk_upwind_f64",
                      "simk_upwind_f32",
@@ -111,6 +111,7 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_kin_advance_n", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, P(C.c_float))
     _sig(lib, "simk_kin_advance_static", C.c_int32, P(_ffi.GpuKinLaunch), C.c_uint32, P(C.c_float))
     _sig(lib, "simk_kin_set_grid", C.c_int32, C.c_uint32)
+    _sig(lib, "simk_kin_set_team", C.c_int32, C.c_uint32)
     _sig(lib, "simk_kin_set_order", C.c_int32, V)
     _sig(lib, "simk_kin_set_mechs", C.c_int32, C.c_uint32, V, V, P(_ffi.KinMethod), P(_ffi.KinControl))
     _sig(lib, "simk_jit_precompile_multi", C.c_int32, C.c_uint32, V, V, P(_ffi.KinMethod), P(SimkGenOptions),

@@ -16,10 +16,11 @@ pytestmark = pytest.mark.gpu
 ZERO = {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
 
 
-def _run(gm, mech, meth, ctl, z, durations, precision=0, mode=1, repeats=1):
+def _run(gm, mech, meth, ctl, z, durations, precision=0, mode=1, repeats=1, team=64):
     gm.arena_reset()
     gm.set_mech(mech, meth, ctl)
     gm._check(gm.lib.gpu_kin_set_mode(mode), "gpu_kin_set_mode")
+    gm._check(gm.lib.simk_kin_set_team(team), "simk_kin_set_team")
     try:
         dz = gpu.DeviceZones(gm, z)
         dz.upload()
@@ -29,6 +30,7 @@ def _run(gm, mech, meth, ctl, z, durations, precision=0, mode=1, repeats=1):
         return gm.kin_info()
     finally:
         gm.lib.gpu_kin_set_mode(0)
+        gm.lib.simk_kin_set_team(64)
 
 
 @pytest.mark.parametrize("mid", [0, 1, 2], ids=["ros2", "ros3", "rodas3"])
@@ -89,3 +91,44 @@ def test_wave_mode_rejects_mixed_precision(gm):
     meth, ctl = problem.method(2), problem.control()
     with pytest.raises(gpu.GpuError):
         _run(gm, mech, meth, ctl, z.copy(), [0.01], precision=2)
+
+
+@pytest.mark.parametrize("team", [8, 16, 32])
+def test_teams_fp64_bit_exact_with_pulses(gm, team):
+    """Teams of 8, 16 and 32 lanes per cell (64 / team cells per wave, each
+    team with its own control flow): every cell bit-identical to the oracle,
+    with pulse segments, two calls, a cell count that leaves the last wave's
+    teams partly empty, and one fused pair of repeats."""
+    print(f"[case] teams of {team} lanes, FP64, c1 x 203 cells", flush=True)
+    sp, mech, z = make_problem("c1", 203)
+    add_pulses(z, seed=team, duration=0.05)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-5, atol=1e-10)
+    a, b, c = z.copy(), z.copy(), z.copy()
+    info = _run(gm, mech, meth, ctl, a, [0.05, 0.05], team=team)
+    _run(gm, mech, meth, ctl, c, [0.05], team=team, repeats=2)
+    oracle_run(mech, meth, ctl, b, [0.05, 0.05])
+    assert info.kernel == 6 and f"{team} lanes".encode() in info.kernel_name
+    assert compare_bits(a, b) == ZERO
+    assert compare_bits(c, b) == ZERO
+    assert b.pulses[:203].sum() > 0
+
+
+@pytest.mark.parametrize("team", [8, 16, 32])
+def test_teams_delta32_and_48x150(gm, team):
+    """f32 delta lanes in teams (the c1 mechanism, pulses) and the 48/150
+    mechanism in FP64 teams: bit-identical to the oracle."""
+    print(f"[case] teams of {team} lanes, f32 delta c1 and FP64 c2", flush=True)
+    sp, mech, z = make_problem("c1", 150)
+    add_pulses(z, seed=team + 1, duration=0.01)
+    meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
+    a, b = z.copy(), z.copy()
+    _run(gm, mech, meth, ctl, a, [0.01, 0.01], precision=1, team=team)
+    for d in (0.01, 0.01):
+        oracle_ffi.oracle_advance_delta32(mech, meth, ctl, b, d, threads=16)
+    assert compare_bits(a, b) == ZERO
+    sp, mech, z = make_problem("c2", 100)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    _run(gm, mech, meth, ctl, a, [0.1], team=team)
+    oracle_run(mech, meth, ctl, b, [0.1])
+    assert compare_bits(a, b) == ZERO

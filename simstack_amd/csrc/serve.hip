@@ -74,6 +74,11 @@ struct Server {
     double *d_inflow = nullptr;      // [max_requests][kMaxSpecies]
     SimkUpwindSeg *d_segs = nullptr, *h_segs = nullptr;
     hipEvent_t ev[5] = {};
+    // Admission uploads run on a stream of their own (default priority), so
+    // a request admitted while a chemistry launch runs is copied in during it;
+    // the module stream waits for `admitted` before it touches those lanes.
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t admitted = nullptr;
     SimkServeStats st;
 };
 
@@ -138,6 +143,11 @@ void release_all() {
     if (S.h_segs) (void)hipHostFree(S.h_segs);
     for (hipEvent_t &e : S.ev)
         if (e) (void)hipEventDestroy(e);
+    if (S.admitted) (void)hipEventDestroy(S.admitted);
+    if (S.copy_stream) {
+        (void)hipStreamSynchronize(S.copy_stream);
+        (void)hipStreamDestroy(S.copy_stream);
+    }
     S = Server();
 }
 
@@ -325,6 +335,8 @@ GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
     if (e == hipSuccess) e = hipHostMalloc(&S.h_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg), 0);
     for (hipEvent_t &ev : S.ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S.admitted, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S.copy_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         release_all();
@@ -367,7 +379,11 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
     if (!same_shape(S.shape, simk_multi_shape())) return code(hipErrorInvalidValue);   // mechanisms changed
     hipStream_t st = simk_module_stream();
     hipError_t e = hipEventRecord(S.ev[0], st);
-    if (e == hipSuccess) e = admit(st);
+    // requests submitted since the last launch: their uploads, then the
+    // module stream waits for every upload enqueued so far
+    if (e == hipSuccess) e = admit(S.copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(S.admitted, S.copy_stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, S.admitted, 0);
     if (e == hipSuccess) e = hipEventRecord(S.ev[1], st);
     if (e == hipSuccess) e = advect(st);
     if (e == hipSuccess) e = hipEventRecord(S.ev[2], st);
@@ -383,7 +399,11 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
         const u32 *orders[4];
         SimkSlice slice;
         memset(&slice, 0, sizeof(slice));
-        slice.ticks = S.cfg.slice_us == 0xFFFFFFFFu ? 0 : (u64)S.cfg.slice_us * 100;   // 100 MHz counter
+        // 100 MHz counter. With nobody waiting for admission the slice is 4x
+        // longer: fewer launch boundaries (each costs a host round trip and
+        // the launch's ramp) while no arrival can use the lanes sooner.
+        const u64 slice_us = S.queued.empty() ? 4ull * S.cfg.slice_us : (u64)S.cfg.slice_us;
+        slice.ticks = S.cfg.slice_us == 0xFFFFFFFFu ? 0 : slice_us * 100;
         for (u32 i = 0; i < S.shape.count; ++i) {
             Slab &sl = S.slabs[i];
             u64 off[12];
@@ -407,6 +427,10 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
         if (waves > S.shape.resident_waves) waves = S.shape.resident_waves;
         e = simk_enqueue_multi_ext(S.shape.count, Ls, lanes, orders, 1, S.d_ws, S.d_queue, (u32)waves, &slice);
         if (e == hipSuccess) e = hipEventRecord(S.ev[3], st);
+        // Uploads of requests that fit now overlap the launch: their lanes
+        // are outside every order array of this launch, and the next
+        // iteration's advection waits for them.
+        if (e == hipSuccess) e = admit(S.copy_stream);
         if (e == hipSuccess) e = hipMemcpyAsync(S.h_queue, S.d_queue, 8 * 4, hipMemcpyDeviceToHost, st);
         for (u32 i = 0; i < S.shape.count && e == hipSuccess; ++i)
             if (slice.ticks)

@@ -107,6 +107,36 @@ class Mechanism:
     def density(self):
         return self.nnz / float(self.n * self.n)
 
+    def _flop_terms(self):
+        nr = (self.reactants != 0xFF).sum(axis=1)
+        net = np.zeros(self.R, np.int64)
+        for r in range(self.R):
+            d = {}
+            for s in self.reactants[r]:
+                if s != 0xFF:
+                    d[s] = d.get(s, 0) - 1
+            for s in self.products[r]:
+                if s != 0xFF:
+                    d[s] = d.get(s, 0) + 1
+            net[r] = sum(1 for v in d.values() if v != 0)
+        return nr, net
+
+    def flops_mixed(self, sweeps=0, stages=4, fun_evals=4):
+        """(FP64, FP32) algorithmic flops of one mixed-precision step attempt
+        (FMA = 2). FP32: the LU factor and every triangular solve, (1 + sweeps)
+        per stage. FP64: f, -J, the vector work, and per sweep and stage the
+        residual b - g x + J x (J x matrix-free from the reactions) and the
+        correction x += z."""
+        nr, net = self._flop_terms()
+        f_fun = int((nr + 2 * net).sum())
+        f_jac = int((nr * (nr - 1) + 2 * nr * net).sum())
+        f_jx = int((nr * (nr + 2 * net)).sum())
+        f_lu = self.lu_div + 2 * self.lu_sub + 2 * self.n
+        f_solve = 2 * (self.nnz - self.n) + self.n
+        fp32 = f_lu + stages * (1 + sweeps) * f_solve
+        fp64 = fun_evals * f_fun + f_jac + 31 * self.n + 30 + sweeps * stages * (f_jx + 2 * self.n + self.n)
+        return fp64, fp32
+
     def flops_per_attempt(self, stages=4, fun_evals=4):
         """Algorithmic FP64 flops of one Rodas3 step attempt (FMA = 2), from the tables:
         fun_evals * F_fun + F_jac + F_lu + stages * F_solve + 31 n + 30."""

@@ -3,11 +3,15 @@
     python -m simstack_amd.profiling --name NAME --cmd "COMMAND" \\
         --trace KT.db --fetch F.db --write W.db --copy-fetch CF.db --copy-write CW.db \\
         [--l2 L2.db] [--sq SQ.db] --kernel simk_gen --copy-bytes 1073741824 \\
-        --out profiles/NAME.md [--traffic-json profiles/traffic_NAME.json]
+        --out profiles/NAME.md [--traffic-json profiles/traffic_NAME.json] \\
+        [--dispatch I] [--attempts A]
 
 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by what the same
 counters read for the module's plain copy kernel of known size
-(simstack_amd.calibrate), so traffic is in bytes.
+(simstack_amd.calibrate), so traffic is in bytes. --dispatch picks one
+dispatch of the kernel (in dispatch order) instead of the average of all;
+--attempts is that dispatch's step attempts (from the bench line), giving
+bytes per attempt.
 """
 import argparse
 import json
@@ -55,9 +59,12 @@ def main():
     ap.add_argument("--algo-bytes-per-launch", type=float, default=0.0)
     ap.add_argument("--out", required=True)
     ap.add_argument("--traffic-json")
+    ap.add_argument("--dispatch", type=int, default=-1)
+    ap.add_argument("--attempts", type=float, default=0.0)
     a = ap.parse_args()
 
-    dur = kernel_durations(a.trace, a.kernel)
+    pick = (lambda v: [v[a.dispatch]]) if a.dispatch >= 0 else (lambda v: v)
+    dur = pick(kernel_durations(a.trace, a.kernel))
     avg_ns = sum(dur) / len(dur)
     if not a.fetch:   # kernel trace only: every kernel of the run
         rows = _q(a.trace, "select name, count(*), avg(duration), sum(duration) from kernels group by name "
@@ -73,8 +80,8 @@ def main():
     cw = counter(a.copy_write, a.copy_kernel, "WRITE_SIZE")
     f_scale = a.copy_bytes / (min(cf) * 1024.0)
     w_scale = a.copy_bytes / (min(cw) * 1024.0)
-    fetch = counter(a.fetch, a.kernel, "FETCH_SIZE")
-    write = counter(a.write, a.kernel, "WRITE_SIZE")
+    fetch = pick(counter(a.fetch, a.kernel, "FETCH_SIZE"))
+    write = pick(counter(a.write, a.kernel, "WRITE_SIZE"))
     rd = sum(fetch) / len(fetch) * 1024.0 * f_scale
     wr = sum(write) / len(write) * 1024.0 * w_scale
     lines = [f"# {a.name}", "", "Command (kernel trace; the counter passes run the same command under",
@@ -84,7 +91,8 @@ def main():
              f"--mib {a.copy_bytes >> 20} --reps 2{' --width 8' if a.copy_kernel == 'simk_copy8_kernel' else ''}` "
              f"(`{a.copy_kernel}`: a copy of {a.copy_bytes} bytes reads and writes that many).", "",
              "| quantity | value |", "|---|---|",
-             f"| kernel `{a.kernel}*` dispatches | {len(dur)} |",
+             f"| kernel `{a.kernel}*` dispatches | {len(dur)}"
+             f"{f' (dispatch {a.dispatch} of the run)' if a.dispatch >= 0 else ''} |",
              f"| average duration per dispatch | {avg_ns / 1e6:.3f} ms |",
              f"| FETCH_SIZE scale (copy kernel) | {f_scale:.3f} |",
              f"| WRITE_SIZE scale (copy kernel) | {w_scale:.3f} |",
@@ -102,6 +110,9 @@ def main():
         extra.append(f"| algorithmic bytes per dispatch | {a.algo_bytes_per_launch:.4e} |")
         extra.append(f"| achieved (algorithmic bytes / avg duration) | {a.algo_bytes_per_launch / avg_ns:.1f} GB/s |")
         extra.append(f"| counted traffic / algorithmic bytes | {(rd + wr) / a.algo_bytes_per_launch:.3f} |")
+    if a.attempts:
+        extra.append(f"| step attempts in the dispatch | {a.attempts:.4e} |")
+        extra.append(f"| HBM traffic per attempt | {(rd + wr) / a.attempts:.1f} B |")
     if a.l2:
         hit = sum(counter(a.l2, a.kernel, "TCC_HIT_sum"))
         miss = sum(counter(a.l2, a.kernel, "TCC_MISS_sum"))
@@ -118,6 +129,8 @@ def main():
     if a.traffic_json:
         json.dump({"kernel": a.kernel, "bytes_per_launch": rd + wr, "bytes_read": rd, "bytes_written": wr,
                    "avg_duration_ms": avg_ns / 1e6, "fetch_scale": f_scale, "write_scale": w_scale,
+                   "attempts": a.attempts or None,
+                   "bytes_per_attempt": (rd + wr) / a.attempts if a.attempts else None,
                    "command": a.cmd}, open(a.traffic_json, "w"), indent=1)
     print("\n".join(lines))
 

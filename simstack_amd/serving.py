@@ -91,13 +91,14 @@ def run(gm, pools, arrivals, steps=20, substeps=4, dt=1e-3, hbm_budget=16 << 30,
     gm.serve_open(hbm_budget=hbm_budget, max_requests=max(len(arrivals.items), 1), steps=steps,
                   substeps=substeps, dt=dt)
     feeds = [pools.feed(m) for m in range(len(pools.mechs))]
+    flops_per_attempt = [m.flops_per_attempt() for m in pools.mechs]
     n = len(arrivals.items)
     lat = np.zeros(n)
     cells_total = 0
     ticket_of = {}
     results = {}
     i = done = failed = 0
-    accepted = attempts = 0.0
+    accepted = attempts = flops = 0.0
     t0 = time.perf_counter()
     try:
         while done < n:
@@ -125,7 +126,9 @@ def run(gm, pools, arrivals, steps=20, substeps=4, dt=1e-3, hbm_budget=16 << 30,
                 res = results.pop(k)
                 ns = pools.mechs[arrivals.items[k][1]].n
                 accepted += float(res[ns + 1].sum())
-                attempts += float(res[ns + 1].sum() + res[ns + 2].sum())
+                att = float(res[ns + 1].sum() + res[ns + 2].sum())
+                attempts += att
+                flops += att * flops_per_attempt[arrivals.items[k][1]]
                 failed += int(res[ns + 4].sum())
                 if on_done:
                     on_done(k, res)
@@ -135,4 +138,4 @@ def run(gm, pools, arrivals, steps=20, substeps=4, dt=1e-3, hbm_budget=16 << 30,
     finally:
         gm.serve_close()
     return {"latency": lat, "wall": wall, "cells": cells_total, "requests": n, "stats": st, "accepted": accepted,
-            "attempts": attempts, "failed": failed}
+            "attempts": attempts, "flops": flops, "failed": failed}

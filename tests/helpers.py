@@ -1,4 +1,7 @@
 """Shared test plumbing: seeded problems, GPU and oracle runs, block comparison."""
+import os
+import time
+
 import numpy as np
 
 from simstack_amd import _ffi, problem
@@ -77,3 +80,25 @@ def max_rel_err(a, b, lanes=None, floor=0.0):
     s = np.maximum(np.abs(x), np.abs(y))
     mask = s > floor
     return float((d[mask] / s[mask]).max()) if mask.any() else 0.0
+
+
+def _threads():
+    return max(1, min(int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1, 32))
+
+
+def checker(mech, meth_id, ctl, z, durations):
+    """Advances z in place once per duration with the CPU checker; returns its name."""
+    t0 = time.perf_counter()
+    if oracle_ffi.have_ref():
+        mb = mech.rebuild(oracle_ffi.ref_caps()).bytes
+        meth_b = oracle_ffi.ref_method_bytes(meth_id)
+        for d in durations:
+            oracle_ffi.ref_advance(mb, meth_b, ctl, z, d, z.padded, width=4, threads=_threads())
+        kind = "reference f64x4"
+    else:
+        for d in durations:
+            oracle_ffi.oracle_advance(mech, problem.method(meth_id), ctl, z, d, threads=_threads())
+        kind = "oracle"
+    print(f"[checker] {kind}, {z.count} cells x {len(durations)} advances, {_threads()} threads, "
+          f"{time.perf_counter() - t0:.1f} s", flush=True)
+    return kind

@@ -15,41 +15,15 @@ threaded on the host's CPU share.
 - the ragged batch: 60/200, 40/130 and 24/70 at 20,000 cells each, one
   multi-mechanism launch of 1 s.
 """
-import os
-import time
-
 import numpy as np
 import pytest
 
-import oracle_ffi
 from simstack_amd import gpu, problem
-from helpers import compare_bits, make_problem
+from helpers import checker, compare_bits, make_problem
 
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 
 ZERO = {k: 0 for k in ("y", "h", "steps", "rejects", "pulses", "failed")}
-
-
-def _threads():
-    return max(1, min(int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1, 32))
-
-
-def checker(mech, meth_id, ctl, z, durations):
-    """Advances z in place once per duration with the CPU checker; returns its name."""
-    t0 = time.perf_counter()
-    if oracle_ffi.have_ref():
-        mb = mech.rebuild(oracle_ffi.ref_caps()).bytes
-        meth_b = oracle_ffi.ref_method_bytes(meth_id)
-        for d in durations:
-            oracle_ffi.ref_advance(mb, meth_b, ctl, z, d, z.padded, width=4, threads=_threads())
-        kind = "reference f64x4"
-    else:
-        for d in durations:
-            oracle_ffi.oracle_advance(mech, problem.method(meth_id), ctl, z, d, threads=_threads())
-        kind = "oracle"
-    print(f"[checker] {kind}, {z.count} cells x {len(durations)} advances, {_threads()} threads, "
-          f"{time.perf_counter() - t0:.1f} s", flush=True)
-    return kind
 
 
 def _device_fused(gm, mech, meth, ctl, z, duration, repeats):

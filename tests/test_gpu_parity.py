@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from simstack_amd import _ffi, problem
-from helpers import add_pulses, compare_bits, gpu_run, make_problem, oracle_run
+from helpers import checker, add_pulses, compare_bits, gpu_run, make_problem, oracle_run
 
 pytestmark = pytest.mark.gpu
 
@@ -202,38 +202,58 @@ def species_groups(ref):
             "trace 1e-11..1e-9": (ref > 1e-11) & (ref <= 1e-9)}
 
 
-@pytest.mark.parametrize("sweeps", [0xFFFFFFFF, 1, 2], ids=["0sweeps", "1sweep", "2sweeps"])
-def test_mixed_precision_accuracy(gm, sweeps):
-    """Mixed precision (FP32 Jacobian and LU, FP64 residual refinement, FP64
-    state and error control) against the FP64 oracle on the 48/150 workload:
-    max relative error per species group, over cells checked by the oracle."""
-    sp, mech, z = make_problem("c2", 2048)
+CRITERION_P999 = 1e-6
+
+
+def cell_errors(got, ref, atol, rtol):
+    """Each cell's max relative error |x - x64| / max(|x|, |x64|) over the
+    species whose FP64 value is above atol/rtol."""
+    d = np.abs(got - ref)
+    s = np.maximum(np.abs(got), np.abs(ref))
+    r = np.divide(d, s, out=np.zeros_like(d), where=s > 0)
+    return np.where(ref > atol / rtol, r, 0.0).max(axis=0)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("sweeps,meets", [(0xFFFFFFFF, False), (1, True), (2, True)],
+                         ids=["0sweeps", "1sweep", "2sweeps"])
+def test_mixed_precision_accuracy(gm, sweeps, meets):
+    """Mixed precision (FP32 Jacobian and LU, FP64 state and error control,
+    0-2 FP64-residual refinement sweeps per stage) against the FP64 checker on
+    8,192 cells of the 48/150 workload over 10 x 1 s. The accuracy criterion:
+    the p99.9 over cells of each cell's max relative error (species above
+    atol/rtol) is at most 1e-6. The refined variants meet it; the unrefined one
+    does not (its errors reach the tolerance level through flipped
+    accept/reject decisions) and is held only to p99.9 <= 1e-5."""
+    cells = 8192
+    sp, mech, z = make_problem("c2", cells)
     meth, ctl = problem.method(2), problem.control()
     a = z.copy()
     from simstack_amd import gpu
     gm.arena_reset()
     gm.set_kernel(0)
     gm.set_gen_options(sweeps=sweeps)
-    gm.set_mech(mech, meth, ctl)
-    dz = gpu.DeviceZones(gm, a)
-    dz.upload()
-    for _ in range(2):
-        dz.advance(1.0, precision=2)
-    dz.download(mutable_only=False)
-    gm.set_gen_options()
-    assert gm.kin_info().kernel == 3
-    lanes = np.arange(0, 2048, 16)
-    b = z.subset(lanes)
-    oracle_run(mech, meth, ctl, b, [1.0, 1.0])
-    got, ref = a.y[:, lanes], b.y[:, :len(lanes)]
-    worst = {}
-    for gname, mask in species_groups(ref).items():
-        if mask.any():
-            worst[gname] = float((np.abs(got - ref)[mask] / ref[mask]).max())
-    print(f"mixed precision ({sweeps} sweeps): max relative error vs FP64 oracle by group {worst}; "
-          f"steps mixed/fp64 {a.steps[lanes].sum() / b.steps[:len(lanes)].sum():.4f}")
-    assert a.failed[lanes].sum() == 0
-    assert worst["major >1e-6"] < 1e-4
+    try:
+        gm.set_mech(mech, meth, ctl)
+        dz = gpu.DeviceZones(gm, a)
+        dz.upload()
+        dz.advance(1.0, precision=2, repeats=10)
+        dz.download(mutable_only=False)
+        assert gm.kin_info().kernel == 3
+    finally:
+        gm.set_gen_options()
+    b = z.copy()
+    checker(mech, 2, ctl, b, [1.0] * 10)
+    e = cell_errors(a.y[:, :cells], b.y[:, :cells], ctl.atol, ctl.rtol)
+    p999 = float(np.percentile(e, 99.9))
+    same = float(np.mean((a.steps[:cells] == b.steps[:cells]) & (a.rejects[:cells] == b.rejects[:cells])))
+    print(f"mixed precision ({sweeps} sweeps), {cells} cells x 10 s: cell p99.9 {p999:.3e}, max {e.max():.3e}, "
+          f"cells above 1e-6 {np.mean(e > 1e-6):.4f}, same step counts {same:.3f}", flush=True)
+    assert a.failed[:cells].sum() == 0
+    if meets:
+        assert p999 <= CRITERION_P999
+    else:
+        assert p999 <= 1e-5
 
 
 def test_autotune_picks_settings_and_keeps_bits(gm):

@@ -60,11 +60,15 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--traffic-json")
     ap.add_argument("--dispatch", type=int, default=-1)
+    ap.add_argument("--trace-dispatch", type=int, default=None,
+                    help="the kernel trace's dispatch index when it differs from the counter runs' (warm-up launches)")
     ap.add_argument("--attempts", type=float, default=0.0)
     a = ap.parse_args()
 
     pick = (lambda v: [v[a.dispatch]]) if a.dispatch >= 0 else (lambda v: v)
-    dur = pick(kernel_durations(a.trace, a.kernel))
+    td = a.dispatch if a.trace_dispatch is None else a.trace_dispatch
+    dur = kernel_durations(a.trace, a.kernel)
+    dur = [dur[td]] if td >= 0 else dur
     avg_ns = sum(dur) / len(dur)
     if not a.fetch:   # kernel trace only: every kernel of the run
         rows = _q(a.trace, "select name, count(*), avg(duration), sum(duration) from kernels group by name "
@@ -92,7 +96,7 @@ def main():
              f"(`{a.copy_kernel}`: a copy of {a.copy_bytes} bytes reads and writes that many).", "",
              "| quantity | value |", "|---|---|",
              f"| kernel `{a.kernel}*` dispatches | {len(dur)}"
-             f"{f' (dispatch {a.dispatch} of the run)' if a.dispatch >= 0 else ''} |",
+             f"{f' (dispatch {td} of the traced run, {a.dispatch} of the counter runs)' if a.dispatch >= 0 else ''} |",
              f"| average duration per dispatch | {avg_ns / 1e6:.3f} ms |",
              f"| FETCH_SIZE scale (copy kernel) | {f_scale:.3f} |",
              f"| WRITE_SIZE scale (copy kernel) | {w_scale:.3f} |",

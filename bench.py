@@ -359,7 +359,7 @@ def serving_run(gpu, problem, rate, window, steps=20, substeps=4, dt=1e-3, seed=
 serving_run.pools = None
 
 
-def serving_block(gpu, problem, rate=200.0, window=0.25, sustained_rate=2.0, sustained_window=10.0):
+def serving_block(gpu, problem, rate=200.0, window=0.25, sustained_rate=3.0, sustained_window=10.0):
     """The serving loop (the ragged batch's three mechanisms, Poisson arrivals,
     cells log-uniform 64-262,144 in 4,096-node channels, 20 operator-split
     steps of upwind advection then 1 ms of chemistry per request). `sustained`
@@ -369,7 +369,7 @@ def serving_block(gpu, problem, rate=200.0, window=0.25, sustained_rate=2.0, sus
     out = {"workload": "Poisson arrivals, cells log-uniform 64-262144 in 4096-node channels, mechanism uniform over "
                        "60/200, 40/130, 24/70; 20 x (upwind advection 4 substeps, then chemistry 0.001 s)"}
     if sustained_rate > 0.0:
-        out["sustained"] = serving_run(gpu, problem, sustained_rate, sustained_window, seed=6)
+        out["sustained"] = serving_run(gpu, problem, sustained_rate, sustained_window, seed=7)
     out["overload"] = serving_run(gpu, problem, rate, window)
     return out
 
@@ -453,13 +453,14 @@ def main():
     ap.add_argument("--no-advect", action="store_true", help="skip the upwind advection bandwidth block")
     ap.add_argument("--serve-window", type=float, default=0.25, help="seconds of Poisson arrivals, overload rate")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="the overload arrival rate")
-    ap.add_argument("--serve-sustained", type=float, default=2.0, help="a sustainable arrival rate (req/s)")
+    ap.add_argument("--serve-sustained", type=float, default=3.0,
+                    help="the highest arrival rate the GPU sustains (req/s; profiles/serving_latency_curve.md)")
     ap.add_argument("--serve-curve", default="",
                     help="comma-separated arrival rates: latency at each over a 10-s window (adds minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (cells shard with no "
-              "communication, see DESIGN.md)", file=sys.stderr)
+              "communication: each GPU would run the same kernel on its own share)", file=sys.stderr)
         sys.exit(2)
 
     from simstack_amd import gpu, problem
@@ -575,8 +576,12 @@ def main():
         out["p99_latency_s"] = head["p99_latency_s"]
         out["latency_at_req_per_s"] = head["rate_req_per_s"]
     if args.serve_curve:
-        out["serving_curve"] = [serving_run(gpu, problem, float(r), 10.0, seed=7)
-                                for r in args.serve_curve.split(",") if r]
+        out["serving_curve"] = []
+        for r in (x for x in args.serve_curve.split(",") if x):
+            out["serving_curve"].append(serving_run(gpu, problem, float(r), 10.0, seed=7))
+            c = out["serving_curve"][-1]
+            print(f"[serving] {c['rate_req_per_s']:g} req/s: p50 {c['p50_latency_s']:.2f} s, p99 "
+                  f"{c['p99_latency_s']:.2f} s", file=sys.stderr, flush=True)
     if not args.no_dbd:
         out["dbd"] = dbd_block(gpu, problem)
     if not args.no_advect:

@@ -307,118 +307,165 @@ SIMK_DEV simk_u32 simk_team_lds_bytes(simk_u32 n, simk_u32 R, simk_u32 nnz) {
     return ((simk_wave_lds_elems(n, R, nnz) * (simk_u32)sizeof(T) + 7u) & ~7u) + 8u * n;
 }
 
-// Persistent: each team takes cells from the queue one at a time and runs
-// dbd_advance_group (load, kin_advance with pulse segments, store) on it.
-// lds: the wave's LDS, 64 / L consecutive team regions.
+// One cell advanced by a team: dbd_advance_group (load, kin_advance with
+// pulse segments, store) with the cell's state in the team's LDS region
+// `mine` (simk_team_lds_bytes of this mechanism).
 template <class T, bool DELTA, simk_u32 L>
-SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, const SimkWaveTables *W,
-                             unsigned char *lds) {
-    constexpr simk_u32 kTeams = SIMK_WAVE / L;
+SIMK_DEV void simk_wave_cell(const SimkKinArgs &A, const SimkTConsts<T> &C, const SimkWaveTables *W,
+                             unsigned char *mine, const simk_u64 cell) {
     const SimkDevTables *D = A.tables;
-    const simk_u32 lane = threadIdx.x % L, team = threadIdx.x / L;
-    const simk_u32 team_bytes = simk_team_lds_bytes<T>(D->n, D->R, D->nnz);
-    unsigned char *mine = lds + team * team_bytes;
+    const simk_u32 lane = threadIdx.x % L;
     double *y64 = (double *)(mine + ((simk_wave_lds_elems(D->n, D->R, D->nnz) * (simk_u32)sizeof(T) + 7u) & ~7u));
     SimkWaveState<T> S;
     S.init((T *)mine, D->n, D->R, D->nnz);
     const simk_u32 n = S.n, R = S.R;
     const simk_u64 p = A.padded;
     const T duration = (T)A.duration;
+    // dbd_advance_group's load (dbd_lanes.h:31-54)
+    for (simk_u32 s = lane; s < n; s += L) {
+        const double v = A.y[s * p + cell];
+        if (DELTA) {
+            y64[s] = v;
+            S.base[s] = (T)v;
+            S.y[s] = T(0);
+        } else {
+            S.y[s] = (T)v;
+        }
+        S.src[s] = (T)A.src[s * p + cell];
+        S.pulse[s] = (T)A.pulse[s * p + cell];
+    }
+    for (simk_u32 r = lane; r < R; r += L) S.k[r] = (T)A.k[r * p + cell];
+    const T period = (T)A.period[cell], first = (T)A.pulse_first[cell], count = (T)A.pulse_count[cell];
+    T h = (T)A.h[cell];
+    T failed = (T)A.failed[cell];
+    double acc_steps = 0.0, acc_rejects = 0.0, acc_pulses = 0.0;
+    simk_team_sync<L>();
+    for (simk_u32 rep = 0; rep < A.repeats; ++rep) {
+        if (DELTA && rep > 0) {   // the next call restarts from the f64 storage
+            for (simk_u32 s = lane; s < n; s += L) {
+                S.base[s] = (T)y64[s];
+                S.y[s] = T(0);
+            }
+            simk_team_sync<L>();
+        }
+        T steps = T(0), rejects = T(0), pulses = T(0), t = T(0), applied = T(0);
+        // kin_advance (kin_solver.h:322-347)
+        for (;;) {
+            const bool pending = applied < count;
+            const T next_pulse = simk_t_fma(applied, period, first);
+            const T seg_end = pending ? simk_t_min(next_pulse, duration) : duration;
+            // kin_integrate (kin_solver.h:189-313)
+            h = h > T(0) ? h : C.h_start;
+            bool reject_last = false, reject_more = false;
+            bool active = (t < seg_end) && !(failed > T(0));
+            simk_u32 it = 0;
+            while (active) {
+                if (++it > C.max_steps) {
+                    failed = T(1);
+                    break;
+                }
+                simk_wave_attempt<T, DELTA, L>(D, W, S, C, t, seg_end, h, steps, rejects, reject_last, reject_more);
+                active = t < seg_end;
+            }
+            // pending recomputed (applied is unchanged since the segment
+            // began): with teams narrower than the wave the loop above exits
+            // divergently, and no lane-mask value is carried across it
+            const bool pending_end = applied < count;
+            const bool alive = !(failed > T(0));
+            const bool hit = pending_end && alive && (t >= seg_end);
+            if (hit) {
+                for (simk_u32 s = lane; s < n; s += L) S.y[s] = S.y[s] + S.pulse[s];
+                simk_team_sync<L>();
+                applied = applied + T(1);
+                pulses = pulses + T(1);
+                h = C.h_start;
+            }
+            // "!pending" ends the loop for a NaN count as well; for every
+            // finite count it is the reference's exit test
+            if (((t >= duration) && (applied >= count || !pending_end)) || !alive) break;
+        }
+        if (DELTA) {   // dbd_lanes.h:58-67
+            for (simk_u32 s = lane; s < n; s += L) {
+                double v = y64[s] + (double)S.y[s];
+                if (C.clamp_negative) v = v > 0.0 ? v : 0.0;
+                y64[s] = v;
+            }
+            simk_team_sync<L>();
+        }
+        // per-call counters added to the f64 totals (integers: exact in any grouping)
+        acc_steps = acc_steps + (double)steps;
+        acc_rejects = acc_rejects + (double)rejects;
+        acc_pulses = acc_pulses + (double)pulses;
+    }
+    // dbd_advance_group's store (dbd_lanes.h:56-77)
+    for (simk_u32 s = lane; s < n; s += L) A.y[s * p + cell] = DELTA ? y64[s] : (double)S.y[s];
+    if (lane == 0) {
+        A.h[cell] = (double)h;
+        A.failed[cell] = (double)failed;
+        A.steps[cell] = A.steps[cell] + acc_steps;
+        A.rejects[cell] = A.rejects[cell] + acc_rejects;
+        A.pulses[cell] = A.pulses[cell] + acc_pulses;
+    }
+    simk_team_sync<L>();
+}
+
+// The next queue position of A for the calling team: its first lane takes
+// it, the team shares it.
+template <simk_u32 L>
+SIMK_DEV simk_u32 simk_team_take(const SimkKinArgs &A) {
+    const simk_u32 lane = threadIdx.x % L, team = threadIdx.x / L;
+    simk_u32 got = 0;
+    if (lane == 0) got = atomicAdd(A.queue, 1u);
+    if constexpr (L == SIMK_WAVE)
+        return (simk_u32)__builtin_amdgcn_readfirstlane((int)got);
+    else
+        return (simk_u32)__builtin_amdgcn_ds_bpermute((int)((team * L) << 2), (int)got);
+}
+
+// Persistent: each team takes cells from the queue one at a time and
+// advances them. lds: the wave's LDS, 64 / L consecutive team regions.
+template <class T, bool DELTA, simk_u32 L>
+SIMK_DEV void simk_wave_body(const SimkKinArgs &A, const SimkTConsts<T> &C, const SimkWaveTables *W,
+                             unsigned char *lds) {
+    constexpr simk_u32 kTeams = SIMK_WAVE / L;
+    const SimkDevTables *D = A.tables;
+    const simk_u32 team = threadIdx.x / L;
+    unsigned char *mine = lds + team * simk_team_lds_bytes<T>(D->n, D->R, D->nnz);
     simk_u32 pos = blockIdx.x * kTeams + team;   // the queue head starts at gridDim.x * kTeams
     bool first_cell = !A.queue_first;
     for (;;) {
-        if (!first_cell) {
-            simk_u32 got = 0;
-            if (lane == 0) got = atomicAdd(A.queue, 1u);
-            if constexpr (L == SIMK_WAVE)
-                pos = (simk_u32)__builtin_amdgcn_readfirstlane((int)got);
-            else   // the team's first lane took the position: broadcast it to the team
-                pos = (simk_u32)__builtin_amdgcn_ds_bpermute((int)((team * L) << 2), (int)got);
-        }
+        if (!first_cell) pos = simk_team_take<L>(A);
         first_cell = false;
         if (pos >= A.lanes) break;
-        const simk_u64 cell = A.order ? A.order[pos] : pos;
-        // dbd_advance_group's load (dbd_lanes.h:31-54)
-        for (simk_u32 s = lane; s < n; s += L) {
-            const double v = A.y[s * p + cell];
-            if (DELTA) {
-                y64[s] = v;
-                S.base[s] = (T)v;
-                S.y[s] = T(0);
-            } else {
-                S.y[s] = (T)v;
-            }
-            S.src[s] = (T)A.src[s * p + cell];
-            S.pulse[s] = (T)A.pulse[s * p + cell];
+        simk_wave_cell<T, DELTA, L>(A, C, W, mine, A.order ? A.order[pos] : pos);
+    }
+}
+
+// Several mechanisms in one launch (ragged batches, FP64): per-mechanism
+// arguments, constants and contribution lists; a team visits the
+// mechanisms' queues in `seq` order (every queue position comes from its
+// queue's counter, starting at 0) and moves on when one is empty.
+struct SimkWaveMultiArgs {
+    SimkKinArgs a[4];
+    SimkTConsts<double> c[4];
+    const SimkWaveTables *w[4];
+    simk_u32 count;
+    simk_u32 team_bytes;     // LDS bytes per team: the largest mechanism's
+    simk_u32 seq[4];
+};
+
+template <simk_u32 L>
+SIMK_DEV void simk_wave_multi_body(const SimkWaveMultiArgs &M, unsigned char *lds) {
+    const simk_u32 team = threadIdx.x / L;
+    unsigned char *mine = lds + team * M.team_bytes;
+    for (simk_u32 k = 0; k < M.count;) {
+        const simk_u32 i = M.seq[k];
+        const SimkKinArgs &A = M.a[i];
+        const simk_u32 pos = simk_team_take<L>(A);
+        if (pos >= A.lanes) {
+            ++k;
+            continue;
         }
-        for (simk_u32 r = lane; r < R; r += L) S.k[r] = (T)A.k[r * p + cell];
-        const T period = (T)A.period[cell], first = (T)A.pulse_first[cell], count = (T)A.pulse_count[cell];
-        T h = (T)A.h[cell];
-        T failed = (T)A.failed[cell];
-        double acc_steps = 0.0, acc_rejects = 0.0, acc_pulses = 0.0;
-        simk_team_sync<L>();
-        for (simk_u32 rep = 0; rep < A.repeats; ++rep) {
-            if (DELTA && rep > 0) {   // the next call restarts from the f64 storage
-                for (simk_u32 s = lane; s < n; s += L) {
-                    S.base[s] = (T)y64[s];
-                    S.y[s] = T(0);
-                }
-                simk_team_sync<L>();
-            }
-            T steps = T(0), rejects = T(0), pulses = T(0), t = T(0), applied = T(0);
-            // kin_advance (kin_solver.h:322-347)
-            for (;;) {
-                const bool pending = applied < count;
-                const T next_pulse = simk_t_fma(applied, period, first);
-                const T seg_end = pending ? simk_t_min(next_pulse, duration) : duration;
-                // kin_integrate (kin_solver.h:189-313)
-                h = h > T(0) ? h : C.h_start;
-                bool reject_last = false, reject_more = false;
-                bool active = (t < seg_end) && !(failed > T(0));
-                simk_u32 it = 0;
-                while (active) {
-                    if (++it > C.max_steps) {
-                        failed = T(1);
-                        break;
-                    }
-                    simk_wave_attempt<T, DELTA, L>(D, W, S, C, t, seg_end, h, steps, rejects, reject_last, reject_more);
-                    active = t < seg_end;
-                }
-                const bool alive = !(failed > T(0));
-                const bool hit = pending && alive && (t >= seg_end);
-                if (hit) {
-                    for (simk_u32 s = lane; s < n; s += L) S.y[s] = S.y[s] + S.pulse[s];
-                    simk_team_sync<L>();
-                    applied = applied + T(1);
-                    pulses = pulses + T(1);
-                    h = C.h_start;
-                }
-                // "!pending" ends the loop for a NaN count as well; for every
-                // finite count it is the reference's exit test
-                if (((t >= duration) && (applied >= count || !pending)) || !alive) break;
-            }
-            if (DELTA) {   // dbd_lanes.h:58-67
-                for (simk_u32 s = lane; s < n; s += L) {
-                    double v = y64[s] + (double)S.y[s];
-                    if (C.clamp_negative) v = v > 0.0 ? v : 0.0;
-                    y64[s] = v;
-                }
-                simk_team_sync<L>();
-            }
-            // per-call counters added to the f64 totals (integers: exact in any grouping)
-            acc_steps = acc_steps + (double)steps;
-            acc_rejects = acc_rejects + (double)rejects;
-            acc_pulses = acc_pulses + (double)pulses;
-        }
-        // dbd_advance_group's store (dbd_lanes.h:56-77)
-        for (simk_u32 s = lane; s < n; s += L) A.y[s * p + cell] = DELTA ? y64[s] : (double)S.y[s];
-        if (lane == 0) {
-            A.h[cell] = (double)h;
-            A.failed[cell] = (double)failed;
-            A.steps[cell] = A.steps[cell] + acc_steps;
-            A.rejects[cell] = A.rejects[cell] + acc_rejects;
-            A.pulses[cell] = A.pulses[cell] + acc_pulses;
-        }
-        simk_team_sync<L>();
+        simk_wave_cell<double, false, L>(A, M.c[i], M.w[i], mine, A.order ? A.order[pos] : pos);
     }
 }

@@ -164,6 +164,14 @@ __global__ void __launch_bounds__(SIMK_WAVE) simk_wave_kernel(SimkKinArgs A, Sim
     simk_wave_body<T, DELTA, L>(A, C, W, simk_wave_lds);
 }
 
+// The cooperative form of a ragged batch (FP64): teams of L lanes per cell
+// over several mechanisms' queues.
+template <simk_u32 L>
+__global__ void __launch_bounds__(SIMK_WAVE) simk_wave_multi_kernel(SimkWaveMultiArgs M) {
+    extern __shared__ __align__(16) unsigned char simk_wave_lds[];
+    simk_wave_multi_body<L>(M, simk_wave_lds);
+}
+
 // The reference's M1 point demo (a spring/swirl particle step), kept so every
 // reference export exists.
 __global__ void simk_step_points_kernel(f32 *x, f32 *y, f32 *z, f32 *vx, f32 *vy, f32 *vz, u32 count, f32 dt) {
@@ -279,6 +287,10 @@ struct State {
     std::vector<SimkDevCtl> multi_ctl;
     JitKernel multi_jit;
     bool multi_ok = false;
+    // tables of each mechanism for the cooperative (team) form of the
+    // multi-mechanism launch, and its team size (0: one lane per cell)
+    void *multi_dtables[4] = {}, *multi_dwave[4] = {};
+    u32 multi_team = 0, user_multi_team = 0;
     // captured step sequences
     // Captured multi-mechanism steps. Each graph owns its workspace and queue
     // counters, so later launches that grow the shared workspace never leave a
@@ -434,7 +446,7 @@ void put(std::vector<u8> &blob, u64 &off, const T *data, u64 count) {
 }
 
 // Flattens the tables for the table-walk kernel into one device allocation.
-hipError_t upload_tables(const Mech &m) {
+hipError_t upload_tables(const Mech &m, void **out) {
     std::vector<u8> rc(m.R), nc(m.R), reac(m.R * 3), ns(m.R * 6);
     std::vector<f64> coef(m.R * 6, 0.0);
     std::vector<u16> jac(m.R * 18, 0), div(m.lu_div.size() * 4), sub(m.lu_sub.size() * 2), lo(m.lower.size() * 2),
@@ -515,14 +527,14 @@ hipError_t upload_tables(const Mech &m) {
         (void)hipFree(d);
         return e;
     }
-    g.d_tables = d;
+    *out = d;
     return hipSuccess;
 }
 
 // Contribution lists of the cooperative kernel, in the reference's
 // accumulation order (fun: reaction order per species; -J: (r, p, e) order per
 // LU entry; solves: by column).
-hipError_t upload_wave_tables(const Mech &m) {
+hipError_t upload_wave_tables(const Mech &m, void **out) {
     std::vector<u16> fb(m.n + 1), fr, jb(m.nnz + 1), jrp, lb(m.n + 1), lr, li, ub(m.n + 1), ur, ui;
     std::vector<f64> fc, jc;
     for (u32 sp = 0; sp < m.n; ++sp) {
@@ -612,7 +624,7 @@ hipError_t upload_wave_tables(const Mech &m) {
         (void)hipFree(d);
         return e;
     }
-    g.d_wave = d;
+    *out = d;
     return hipSuccess;
 }
 
@@ -750,6 +762,7 @@ void untune() {
     g.gen = g.user_gen;
     g.mode = g.user_mode;
     g.team_lanes = g.user_team;
+    g.multi_team = g.user_multi_team;
     g.grid_override = g.user_grid;
     g.tuned = false;
 }
@@ -778,8 +791,8 @@ s32 set_mech_impl(const Mech &m, const KinMethod *method, const KinControl *cont
     g.ctl.max_steps = control->max_steps;
     g.ctl.fixed_step = control->fixed_step ? 1 : 0;
     g.ctl.clamp_negative = control->clamp_negative ? 1 : 0;
-    e = upload_tables(m);
-    if (e == hipSuccess) e = upload_wave_tables(m);
+    e = upload_tables(m, &g.d_tables);
+    if (e == hipSuccess) e = upload_wave_tables(m, &g.d_wave);
     if (e != hipSuccess) return err_code(e);
     if (!g.blocks_per_cu_table) {
         int blocks = 0;
@@ -868,7 +881,11 @@ u64 multi_wave_bytes(const f64 *weight, u32 count) {
 
 // Method and control in the lane type T (lane_splat's rounding for f32, base_lane.h:98).
 template <class T>
-SimkTConsts<T> t_consts() {
+SimkTConsts<T> t_consts_ctl(const SimkDevCtl &ctl);
+template <class T>
+SimkTConsts<T> t_consts() { return t_consts_ctl<T>(g.ctl); }
+template <class T>
+SimkTConsts<T> t_consts_ctl(const SimkDevCtl &ctl) {
     SimkTConsts<T> C;
     memset(&C, 0, sizeof(C));
     const KinMethod &me = g.method;
@@ -888,16 +905,16 @@ SimkTConsts<T> t_consts() {
     C.gamma0 = (T)me.gamma0;
     C.stages = me.stages;
     C.order = me.order;
-    C.rtol = (T)g.ctl.rtol;
-    C.atol = (T)g.ctl.atol;
-    C.h_min = (T)g.ctl.h_min;
-    C.h_max = (T)g.ctl.h_max;
-    C.h_start = (T)g.ctl.h_start;
-    C.inv_n = (T)g.ctl.inv_n;   // (T)(1.0 / (f64)n), kin_solver.h:197
-    C.h_max_pos = g.ctl.h_max > 0.0 ? 1u : 0u;
-    C.max_steps = g.ctl.max_steps;
-    C.fixed_step = g.ctl.fixed_step;
-    C.clamp_negative = g.ctl.clamp_negative;
+    C.rtol = (T)ctl.rtol;
+    C.atol = (T)ctl.atol;
+    C.h_min = (T)ctl.h_min;
+    C.h_max = (T)ctl.h_max;
+    C.h_start = (T)ctl.h_start;
+    C.inv_n = (T)ctl.inv_n;   // (T)(1.0 / (f64)n), kin_solver.h:197
+    C.h_max_pos = ctl.h_max > 0.0 ? 1u : 0u;
+    C.max_steps = ctl.max_steps;
+    C.fixed_step = ctl.fixed_step;
+    C.clamp_negative = ctl.clamp_negative;
     return C;
 }
 
@@ -1253,7 +1270,67 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
     return hipSuccess;
 }
 
-hipError_t enqueue_multi(const MultiPlan &P) {
+// The cooperative form of a planned multi-mechanism launch, teams of TL lanes.
+template <u32 TL>
+hipError_t enqueue_multi_team(const MultiPlan &P) {
+    constexpr u32 teams = SIMK_WAVE / TL;
+    SimkWaveMultiArgs M;
+    memset(&M, 0, sizeof(M));
+    M.count = P.MA.count;
+    u64 team_bytes = 0, cells = 0;
+    for (u32 i = 0; i < M.count; ++i) {
+        M.a[i] = P.MA.a[i];
+        M.a[i].tables = (const SimkDevTables *)g.multi_dtables[i];
+        M.w[i] = (const SimkWaveTables *)g.multi_dwave[i];
+        M.c[i] = t_consts_ctl<f64>(g.multi_ctl[i]);
+        M.seq[i] = P.MA.seq[i];
+        if (!M.a[i].tables || !M.w[i] || M.a[i].slice) return hipErrorInvalidValue;
+        const u64 b = team_cell_bytes<f64>(g.multi_mechs[i]);
+        if (b > team_bytes) team_bytes = b;
+        cells += M.a[i].lanes;
+    }
+    M.team_bytes = (u32)team_bytes;
+    const u64 lds = teams * team_bytes;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, g.device);
+    if (e != hipSuccess) return e;
+    if (lds > prop.sharedMemPerBlock) return hipErrorInvalidValue;
+    const auto kernel = simk_wave_multi_kernel<TL>;
+    if (lds > 65536) {
+        e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    int blocks = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, SIMK_WAVE, lds) != hipSuccess || blocks < 1)
+        blocks = 1;
+    const u64 resident = (u64)blocks * (g.cus ? g.cus : 1);
+    const u64 needed = (cells + teams - 1) / teams;
+    const u32 waves = (u32)(needed < resident ? (needed ? needed : 1) : resident);
+    hipLaunchKernelGGL(simk_queues_zero_kernel, dim3(1), dim3(64), 0, g.stream, P.queue, 8u);
+    g.last_kernel = 6;
+    char name[64];
+    snprintf(name, sizeof(name), "simk_wave_multi_kernel<%u lanes per cell>", TL);
+    g.last_name = name;
+    g.last_slots = 0;
+    g.last_waves = waves;
+    hipLaunchKernelGGL(kernel, dim3(waves), dim3(SIMK_WAVE), (u32)lds, g.stream, M);
+    return hipGetLastError();
+}
+
+// Team size of ragged launches: the multi-mechanism tuner's choice, else the
+// caller's cooperative mode (gpu_kin_set_mode(1), simk_kin_set_team), else
+// one lane per cell (0).
+u32 multi_team_size() { return g.multi_team ? g.multi_team : (g.mode == 1 ? g.team_lanes : 0u); }
+
+hipError_t enqueue_multi(const MultiPlan &P, bool allow_team = true) {
+    if (allow_team && multi_team_size()) {
+        switch (multi_team_size()) {
+        case 8: return enqueue_multi_team<8>(P);
+        case 16: return enqueue_multi_team<16>(P);
+        case 32: return enqueue_multi_team<32>(P);
+        default: return enqueue_multi_team<SIMK_WAVE>(P);
+        }
+    }
     hipLaunchKernelGGL(simk_queues_zero_kernel, dim3(1), dim3(64), 0, g.stream, P.queue, 8u);
     SimkMultiArgs MA = P.MA;
     size_t size = sizeof(MA);
@@ -1318,7 +1395,7 @@ hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *
         }
     }
     multi_schedule(P.MA, count);
-    return enqueue_multi(P);
+    return enqueue_multi(P, false);   // time slices and carry lists exist in the one-lane-per-cell kernel only
 }
 
 // ---------------------------------------------------------------------------
@@ -2069,6 +2146,11 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
     if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
     g.multi_jit = JitKernel();
     g.multi_ok = false;
+    for (u32 i = 0; i < 4; ++i) {
+        if (g.multi_dtables[i]) (void)hipFree(g.multi_dtables[i]);
+        if (g.multi_dwave[i]) (void)hipFree(g.multi_dwave[i]);
+        g.multi_dtables[i] = g.multi_dwave[i] = nullptr;
+    }
     g.multi_mechs = ms;
     g.method = *method;
     g.method.name = nullptr;
@@ -2086,6 +2168,11 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
         c.fixed_step = control->fixed_step ? 1 : 0;
         c.clamp_negative = control->clamp_negative ? 1 : 0;
     }
+    for (u32 i = 0; i < count && e == hipSuccess; ++i) {
+        e = upload_tables(ms[i], &g.multi_dtables[i]);
+        if (e == hipSuccess) e = upload_wave_tables(ms[i], &g.multi_dwave[i]);
+    }
+    if (e != hipSuccess) return err_code(e);
     e = load_multi(g.gen, g.multi_jit);
     if (e != hipSuccess) return err_code(e);
     g.multi_ok = true;
@@ -2199,6 +2286,42 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     knob([](GenOptions &o) { o.lds_slots = 0; });
     knob([](GenOptions &o) { o.lds_slots = 80; });
     knob([](GenOptions &o) { o.lds_slots = 128; });
+    // teams of 64, 32, 16 and 8 lanes per cell (the cooperative form)
+    const u32 saved_multi_team = g.multi_team;
+    u32 best_team = 0;
+    {
+        const GenOptions keep = g.gen;
+        // timed on 1/16 of every mechanism's sample (at least 64 cells) and
+        // scaled: a team takes 15-80x longer per cell than a lane
+        const std::vector<GpuKinLaunch> full = work;
+        u64 all = 0, part = 0;
+        for (GpuKinLaunch &w : work) {
+            const u32 l = w.lane_count ? w.lane_count : w.padded;
+            u32 sub = l / 16;
+            if (sub < 64) sub = l < 64 ? l : 64;
+            w.lane_count = sub;
+            all += l;
+            part += sub;
+        }
+        const u32 team_sizes[] = {SIMK_WAVE, 32, 16, 8};
+        for (u32 tl : team_sizes) {
+            if (e != hipSuccess) break;
+            g.multi_team = tl;
+            hipError_t r = time_it(t);
+            if (r == hipErrorInvalidValue) continue;   // the team's cells do not fit the LDS
+            e = r;
+            if (e != hipSuccess) break;
+            t = t * (f32)((f64)all / (f64)(part ? part : 1));
+            out->candidates++;
+            if (t < best * kTuneMargin) {
+                best = t;
+                best_team = tl;
+            }
+        }
+        g.multi_team = saved_multi_team;
+        g.gen = keep;
+        work = full;
+    }
     for (TuneSample &x : S)
         if (x.mem) (void)hipFree(x.mem);
     if (e == hipSuccess && !same_gen(best_gen, g.gen)) {
@@ -2227,11 +2350,14 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
         g.tuned = true;
         g.user_gen = saved_gen;
         g.user_mode = g.mode;
+        g.user_team = g.team_lanes;
+        g.user_multi_team = saved_multi_team;
         g.user_grid = saved_grid;
     }
     g.grid_override = best_grid;
-    out->mode = 0;
-    out->lanes_per_cell = 1;
+    g.multi_team = best_team;
+    out->mode = best_team ? 1 : 0;
+    out->lanes_per_cell = best_team ? best_team : 1;
     out->waves = best_grid;
     out->batch = resolved_multi(best_gen).batch;
     out->reg_window = resolved_multi(best_gen).reg_window;

@@ -162,7 +162,7 @@ def test_multi_autotune_keeps_bits_and_restores(gm):
     for d in dzs:
         d.upload()
     res = gm.autotune_multi(dzs, duration=0.05, sample_cells=128)
-    assert res.candidates >= 6 and res.lanes_per_cell == 1 and res.best_ms > 0.0
+    assert res.candidates >= 6 and res.lanes_per_cell in (1, 8, 16, 32, 64) and res.best_ms > 0.0
     assert res.best_ms <= res.default_ms
     for d, (_, z) in zip(dzs, blocks):
         d.download(mutable_only=False)
@@ -175,3 +175,33 @@ def test_multi_autotune_keeps_bits_and_restores(gm):
         oracle_run(mech, meth, ctl, want, [0.25])
         assert compare_bits(got, want) == ZERO
     _set(gm, blocks)   # restores the default settings for the tests after this one
+
+
+@pytest.mark.parametrize("team", [64, 32, 16, 8])
+def test_multi_teams_bit_exact(gm, team):
+    """The cooperative form of a ragged launch (gpu_kin_set_mode(1) with
+    simk_kin_set_team): teams of `team` lanes walk the three mechanisms'
+    queues in one launch; every cell bit-identical to the oracle, with a
+    fused pair of repeats."""
+    print(f"[case] ragged launch in teams of {team} lanes", flush=True)
+    blocks = _blocks((70, 150, 90), first=3000)
+    meth, ctl = _set(gm, blocks)
+    work = [z.copy() for _, z in blocks]
+    dzs = [gpu.DeviceZones(gm, z) for z in work]
+    for d in dzs:
+        d.upload()
+    gm._check(gm.lib.gpu_kin_set_mode(1), "gpu_kin_set_mode")
+    gm._check(gm.lib.simk_kin_set_team(team), "simk_kin_set_team")
+    try:
+        gm.advance_multi(dzs, 0.05, repeats=2)
+        name = gm.kin_info().kernel_name
+    finally:
+        gm.lib.gpu_kin_set_mode(0)
+        gm.lib.simk_kin_set_team(64)
+    for d in dzs:
+        d.download(mutable_only=False)
+    assert f"{team} lanes".encode() in name
+    for (mech, z), got in zip(blocks, work):
+        want = z.copy()
+        oracle_run(mech, meth, ctl, want, [0.05, 0.05])
+        assert compare_bits(got, want) == ZERO

@@ -70,6 +70,22 @@ SIMK_HOST_API void simk_dbd_commit_pulses(SimkDbdZones *z);
 /* dbd_default_control (dbd.cpp:486-496). */
 SIMK_HOST_API void simk_dbd_default_control(KinControl *out);
 
+/* A DBD reactor as a request of the GPU module's serving loop
+ * (SimkServeRequest.kind = SimkServe_Dbd): `cells` zones advanced per master
+ * step as the reference's scene does (dbd_scene_step, dbd_scene.cpp:226-298):
+ * the discharge source and pulse schedule over (t, t + dt] prepared on the
+ * device from each zone's composition, the chemistry advance, the pulses
+ * committed, then (residence_s > 0) plug flow along the chain of zones toward
+ * the request's inflow (its feed gas), a = dt * cells / residence_s. The
+ * request's clock starts at 0. */
+typedef struct SimkServeDbd {
+    SimkAirSpecies species;       /* the mechanism's species indices (simk_air_model) */
+    u32 mode;                     /* SimkDbd_Pulsed or SimkDbd_Averaged */
+    SimkAirSource source;         /* boltzmann must be 0 */
+    f64 residence_s;              /* 0: independent zones */
+    const f64 *T_K, *power_W_cm3, *freq_Hz, *pulse_index;   /* [cells]: operating point, pulses fired so far */
+} SimkServeDbd;
+
 #ifdef __cplusplus
 }
 #endif

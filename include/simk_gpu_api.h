@@ -101,6 +101,11 @@ GPU_API s32 simk_kin_advance_static(const GpuKinLaunch *launch, u32 repeats, f32
  * with its own LDS region and control flow). Results do not depend on it. */
 GPU_API s32 simk_kin_set_team(u32 lanes_per_cell);
 
+/* Control of mechanism `index` of the simk_kin_set_mechs list (ragged
+ * batches and serving; simk_kin_set_mechs gives every mechanism the same
+ * one): for a DBD mechanism among synthetic ones, say. */
+GPU_API s32 simk_kin_set_mech_control(u32 index, const KinControl *control);
+
 /* Persistent grid size in waves (0: as many as are resident on the device). */
 GPU_API s32 simk_kin_set_grid(u32 waves);
 
@@ -277,16 +282,26 @@ typedef struct SimkServeConfig {
     f64 dt;              /* chemistry seconds per step (0: 1e-3) */
 } SimkServeConfig;
 
+struct SimkServeDbd;     /* simk_dbd.h */
+
+enum SimkServeKind {
+    SimkServe_Transport = 0,   /* per step: upwind advection along the channels, then chemistry */
+    SimkServe_Dbd = 1,         /* per step: discharge source, chemistry, pulses, plug flow (simk_dbd.h) */
+};
+
 typedef struct SimkServeRequest {
     u32 mech;            /* index into the simk_kin_set_mechs list */
-    u32 cells;           /* channels * nodes */
-    u32 channels, nodes; /* nodes <= 8192 */
+    u32 cells;           /* transport: channels * nodes; DBD: zones (<= 8192 with plug flow) */
+    u32 channels, nodes; /* transport: nodes <= 8192; DBD: unused */
     u32 padded;          /* row stride of block (>= cells) */
-    u32 reserved;
+    u32 kind;            /* SimkServeKind */
     const f64 *block;    /* host zone block (all rows, reference order) holding the request's inputs */
-    const f64 *courant;  /* [channels * nodes], each in [0, 1] */
-    const f64 *inflow;   /* [species] concentrations entering node 0, or null (zero) */
+    const f64 *courant;  /* transport: [channels * nodes], each in [0, 1]; DBD: unused */
+    const f64 *inflow;   /* [species] concentrations entering node 0 (DBD: the feed gas of the chain), or null */
     f64 *result;         /* host [species + 5][cells]: y, h, steps, rejects, pulses, failed after the last step */
+    u32 steps;           /* operator-split steps of this request (0: the server's) */
+    u32 pad_;
+    const struct SimkServeDbd *dbd;   /* DBD requests: the zones' discharge description (read by simk_serve_submit only) */
 } SimkServeRequest;
 
 typedef struct SimkServeStats {

@@ -411,11 +411,16 @@ GPU_API s32 simk_upwind_f64(f64 *y, u64 padded, u32 fields, u32 channels, u32 no
 }
 
 GPU_API s32 simk_plugflow_f64(f64 *y, u64 padded, u32 species, u32 zones, f64 a, const f64 *feed) {
-    if (!y || !feed || species == 0 || zones == 0 || zones > padded) return (s32)hipErrorInvalidValue;
-    if (!(a >= 0.0 && a <= 1.0)) return (s32)hipErrorInvalidValue;   // the reference asserts a <= 1 (dbd.cpp:473)
+    return (s32)simk_enqueue_plugflow(simk_module_stream(), y, padded, species, zones, a, feed);
+}
+
+}  // extern "C"
+
+hipError_t simk_enqueue_plugflow(hipStream_t st, f64 *y, u64 padded, u32 species, u32 zones, f64 a, const f64 *feed) {
+    if (!y || !feed || species == 0 || zones == 0 || zones > padded) return hipErrorInvalidValue;
+    if (!(a >= 0.0 && a <= 1.0)) return hipErrorInvalidValue;   // the reference asserts a <= 1 (dbd.cpp:473)
     const u32 chunk_needed = (zones + kThreads - 1) / kThreads;
     const dim3 grid(species), block(kThreads);
-    hipStream_t st = simk_module_stream();
     if (chunk_needed <= 4)
         hipLaunchKernelGGL((simk_plugflow_kernel<4>), grid, block, 0, st, y, (unsigned long long)padded, zones, a, feed);
     else if (chunk_needed <= 16)
@@ -425,9 +430,11 @@ GPU_API s32 simk_plugflow_f64(f64 *y, u64 padded, u32 species, u32 zones, f64 a,
         hipLaunchKernelGGL((simk_plugflow_kernel<kMaxChunk>), grid, block, 0, st, y, (unsigned long long)padded, zones,
                            a, feed);
     else
-        return (s32)hipErrorInvalidValue;
-    return (s32)hipGetLastError();
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
+
+extern "C" {
 
 GPU_API s32 simk_upwind_f32(f32 *x, u64 padded, u32 fields, u32 channels, u32 nodes, const f32 *courant,
                             u32 substeps, f32 *outlet_sum) {

@@ -2179,6 +2179,24 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
     return 0;
 }
 
+GPU_API s32 simk_kin_set_mech_control(u32 index, const KinControl *control) {
+    if (!g.multi_ok || !control || index >= g.multi_ctl.size()) return err_code(hipErrorInvalidValue);
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return err_code(e);
+    destroy_graphs();   // they hold the previous controls in their launch arguments
+    SimkDevCtl &c = g.multi_ctl[index];
+    c.rtol = control->rtol;
+    c.atol = control->atol;
+    c.h_min = control->h_min;
+    c.h_max = control->h_max;
+    c.h_start = control->h_start;
+    c.inv_n = 1.0 / (f64)g.multi_mechs[index].n;
+    c.max_steps = control->max_steps;
+    c.fixed_step = control->fixed_step ? 1 : 0;
+    c.clamp_negative = control->clamp_negative ? 1 : 0;
+    return 0;
+}
+
 GPU_API s32 simk_kin_advance_multi(u32 count, const GpuKinLaunch *launches, u32 repeats, f32 *kernel_ms) {
     MultiPlan P;
     hipError_t e = prepare_multi(count, launches, repeats, P);

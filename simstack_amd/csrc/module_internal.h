@@ -11,6 +11,10 @@
 // the library is ordered on it.
 hipStream_t simk_module_stream();
 
+// simk_plugflow_f64 on a given stream (the serving loop's DBD requests).
+hipError_t simk_enqueue_plugflow(hipStream_t stream, f64 *y, u64 padded, u32 species, u32 zones, f64 a,
+                                 const f64 *feed);
+
 // One request's advection rows for the batched upwind kernel: fields x
 // channels rows of `nodes` values at x + field * padded + channel * nodes.
 struct SimkUpwindSeg {

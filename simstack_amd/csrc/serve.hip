@@ -3,12 +3,20 @@
 //
 // Memory: one slab per mechanism, a zone block of `cap` lanes in the
 // reference's SoA row order (y, h, steps, rejects, pulses, failed, k, src,
-// pulse, period, pulse_first, pulse_count) plus one courant row. A request
-// owns a contiguous lane range of its mechanism's slab from admission to
-// completion, so its state never leaves HBM between steps. A per-slab order
+// pulse, period, pulse_first, pulse_count), one courant row, the resume rows
+// of time slices and the DBD rows of reactor requests. A request owns a
+// contiguous lane range of its mechanism's slab from admission to completion,
+// so its state never leaves HBM between steps. A per-slab order
 // array lists the lanes of the admitted requests; the persistent kernel takes
 // cells from it, so every request of every mechanism advances in one launch
 // and a request joins or leaves at any iteration boundary.
+//
+// Two kinds of request share the launches. A transport request runs upwind
+// advection along its channels before each chemistry step. A DBD reactor
+// request runs the chained reactor's master step (dbd_scene_step,
+// src/sim/dbd_scene.cpp:226-298): the discharge source and pulse schedule of
+// (t, t + dt] prepared on the device from each zone's composition, the
+// chemistry step, the pulses committed, then plug flow along the chain.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -20,7 +28,9 @@
 #include <unordered_map>
 #include <vector>
 
+#include "air_source.h"
 #include "module_internal.h"
+#include "simk_dbd.h"
 #include "simk_gpu_api.h"
 
 namespace {
@@ -28,9 +38,14 @@ namespace {
 constexpr u32 kMaxSpecies = 64;
 constexpr u32 kMaxNodes = 8192;
 constexpr u32 kResumeRows = 5;
+// DBD requests' per-zone rows: T, power, frequency, pulses fired so far, then
+// the discharge channels' rate coefficients at T (computed on the host).
+constexpr u32 kDbdT = 0, kDbdPower = 1, kDbdFreq = 2, kDbdPulseIndex = 3, kDbdRates = 4;
+constexpr u32 kDbdRows = kDbdRates + simk_air::kQuenchRates;
+constexpr u32 kDbdMaxSpecies = 32;
 
 struct Slab {
-    double *d = nullptr;     // zone block rows, courant row, resume rows; row stride cap
+    double *d = nullptr;     // zone block rows, courant row, resume rows, DBD rows; row stride cap
     u32 cap = 0;
     u32 n = 0, R = 0;
     u64 zone_rows = 0;       // 3n + R + 8
@@ -43,6 +58,7 @@ struct Slab {
     u32 *h_carry = nullptr;
     double *courant() const { return d + zone_rows * cap; }
     double *resume() const { return d + (zone_rows + 1) * cap; }
+    double *dbd() const { return d + (zone_rows + 1 + kResumeRows) * cap; }
 };
 
 enum { kAdvect = 0, kChem = 1 };
@@ -52,11 +68,74 @@ struct Request {
     u32 ticket = 0;
     u32 off = 0;
     u32 steps_done = 0;
+    u32 steps = 0;               // operator-split steps of this request
     u32 phase = kAdvect;
     u64 bytes = 0;
     u32 slot = 0;                // inflow slot
     std::vector<u32> remaining;  // lanes whose chemistry step is pending: stopped cells first
+    // DBD requests
+    f64 t = 0.0;                 // the request's clock (accumulated step by step, as the scene's)
+    u32 mode = 0;
+    f64 residence = 0.0;
+    u32 species[17] = {};
+    f64 g[4] = {}, f_O1D = 0.0, filament = 0.0;
+    std::vector<f64> dbd_rows;   // [kDbdRows][cells], uploaded at admission
 };
+
+// A DBD request's step-preparation work (dbd_zones_prepare_source, dbd.cpp:123-154).
+struct DbdPrepSeg {
+    f64 *y, *src, *pulse, *period, *first, *count;   // the request's zone-block rows in its slab
+    const f64 *dbd;                                  // its DBD rows
+    u64 stride;                                      // lanes between rows
+    u32 n, zones, mode, pad_;
+    u32 species[17], pad2_;
+    f64 t0, t1, g[4], f_O1D, filament;
+};
+
+// Each zone of each segment: the discharge source (pulse increments or a
+// steady rate) from its composition, and the pulses due in (t0, t1] on its
+// absolute clock (bursts numbered from 1; pulse_index of them fired so far).
+__global__ void __launch_bounds__(256) simk_dbd_prep_kernel(const DbdPrepSeg *segs) {
+    const DbdPrepSeg &S = segs[blockIdx.y];
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.zones) return;
+    const u64 p = S.stride;
+    f64 gas[kDbdMaxSpecies], change[kDbdMaxSpecies], rates[simk_air::kQuenchRates];
+    for (u32 s = 0; s < S.n; ++s) gas[s] = S.y[s * p + i];
+    for (u32 c = 0; c < simk_air::kQuenchRates; ++c) rates[c] = S.dbd[(kDbdRates + c) * p + i];
+    const f64 power = S.dbd[kDbdPower * p + i], freq = S.dbd[kDbdFreq * p + i];
+    const bool on = power > 0.0 && freq > 0.0;
+    const f64 bursts_per_s = (on && S.filament > 0.0) ? power / S.filament : 2.0 * freq;
+    const bool pulsed = S.mode == SimkDbd_Pulsed && on;
+    const f64 energy = pulsed ? power / bursts_per_s : (on ? power : 0.0);
+    simk_air::air_source_delta_rates(S.species, S.g, S.f_O1D, gas, S.n, energy, rates, change);
+    f64 *to = pulsed ? S.pulse : S.src, *zeroed = pulsed ? S.src : S.pulse;
+    for (u32 s = 0; s < S.n; ++s) {
+        to[s * p + i] = change[s];
+        zeroed[s * p + i] = 0.0;
+    }
+    if (!pulsed) {
+        S.period[i] = S.first[i] = S.count[i] = 0.0;
+        return;
+    }
+    const f64 period = 1.0 / bursts_per_s;
+    const f64 first_burst = S.dbd[kDbdPulseIndex * p + i] + 1.0;
+    f64 due = 0.0;
+    for (;;) {
+        if (!((first_burst + due) * period <= S.t1)) break;
+        due += 1.0;
+    }
+    S.period[i] = period;
+    S.first[i] = first_burst * period - S.t0;
+    S.count[i] = due;
+}
+
+// After a DBD step's chemistry: the scheduled pulses are fired
+// (dbd_zones_commit_pulses, dbd.cpp:156-158).
+__global__ void __launch_bounds__(256) simk_dbd_commit_kernel(f64 *pulse_index, const f64 *count, u32 zones) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < zones) pulse_index[i] = pulse_index[i] + count[i];
+}
 
 struct Server {
     bool open = false;
@@ -73,6 +152,7 @@ struct Server {
     u32 *d_queue = nullptr, *h_queue = nullptr;
     double *d_inflow = nullptr;      // [max_requests][kMaxSpecies]
     SimkUpwindSeg *d_segs = nullptr, *h_segs = nullptr;
+    DbdPrepSeg *d_prep = nullptr, *h_prep = nullptr;   // [max_requests]
     hipEvent_t ev[5] = {};
     // Admission uploads run on a stream of their own (default priority), so
     // a request admitted while a chemistry launch runs is copied in during it;
@@ -141,6 +221,8 @@ void release_all() {
     if (S.d_inflow) (void)hipFree(S.d_inflow);
     if (S.d_segs) (void)hipFree(S.d_segs);
     if (S.h_segs) (void)hipHostFree(S.h_segs);
+    if (S.d_prep) (void)hipFree(S.d_prep);
+    if (S.h_prep) (void)hipHostFree(S.h_prep);
     for (hipEvent_t &e : S.ev)
         if (e) (void)hipEventDestroy(e);
     if (S.admitted) (void)hipEventDestroy(S.admitted);
@@ -171,11 +253,14 @@ hipError_t admit(hipStream_t st) {
         const u64 cap = sl.cap;
         hipError_t e = hipMemcpy2DAsync(sl.d + off, cap * 8, q.r.block, (u64)q.r.padded * 8, (u64)q.r.cells * 8,
                                         sl.zone_rows, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
+        if (e == hipSuccess && q.r.kind == SimkServe_Transport)
             e = hipMemcpyAsync(sl.courant() + off, q.r.courant, (u64)q.r.cells * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess && q.r.inflow)
             e = hipMemcpyAsync(S.d_inflow + (u64)q.slot * kMaxSpecies, q.r.inflow, (u64)sl.n * 8,
                                hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && q.r.kind == SimkServe_Dbd)
+            e = hipMemcpy2DAsync(sl.dbd() + off, cap * 8, q.dbd_rows.data(), (u64)q.r.cells * 8, (u64)q.r.cells * 8,
+                                 kDbdRows, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
         for (u32 c = 0; c < q.r.cells; ++c) sl.owner[off + c] = q.ticket;
         q.phase = kAdvect;
@@ -191,11 +276,39 @@ hipError_t admit(hipStream_t st) {
 // Advects every request whose next step starts now; they enter the chemistry
 // phase with all their lanes pending.
 hipError_t advect(hipStream_t st) {
-    u32 rows = 0, max_nodes = 0, count = 0;
+    u32 rows = 0, max_nodes = 0, count = 0, preps = 0, max_zones = 0;
     for (u32 t : S.active) {
         Request &q = S.reqs[t];
         if (q.phase != kAdvect) continue;
         Slab &sl = S.slabs[q.r.mech];
+        if (q.r.kind == SimkServe_Dbd) {
+            // the step's source and pulse schedule over (t, t + dt]
+            DbdPrepSeg &g = S.h_prep[preps++];
+            u64 off[12];
+            zone_offsets(sl.n, sl.R, sl.cap, off);
+            g.y = sl.d + off[0] + q.off;
+            g.src = sl.d + off[7] + q.off;
+            g.pulse = sl.d + off[8] + q.off;
+            g.period = sl.d + off[9] + q.off;
+            g.first = sl.d + off[10] + q.off;
+            g.count = sl.d + off[11] + q.off;
+            g.dbd = sl.dbd() + q.off;
+            g.stride = sl.cap;
+            g.n = sl.n;
+            g.zones = q.r.cells;
+            g.mode = q.mode;
+            memcpy(g.species, q.species, sizeof(g.species));
+            g.t0 = q.t;
+            g.t1 = q.t + S.cfg.dt;
+            memcpy(g.g, q.g, sizeof(g.g));
+            g.f_O1D = q.f_O1D;
+            g.filament = q.filament;
+            if (q.r.cells > max_zones) max_zones = q.r.cells;
+            q.phase = kChem;
+            q.remaining.resize(q.r.cells);
+            for (u32 c = 0; c < q.r.cells; ++c) q.remaining[c] = q.off + c;
+            continue;
+        }
         SimkUpwindSeg &g = S.h_segs[count++];
         g.x = sl.d + q.off;
         g.padded = sl.cap;
@@ -211,8 +324,17 @@ hipError_t advect(hipStream_t st) {
         q.remaining.resize(q.r.cells);
         for (u32 c = 0; c < q.r.cells; ++c) q.remaining[c] = q.off + c;
     }
-    if (!count) return hipSuccess;
-    hipError_t e = hipMemcpyAsync(S.d_segs, S.h_segs, count * sizeof(SimkUpwindSeg), hipMemcpyHostToDevice, st);
+    hipError_t e = hipSuccess;
+    if (preps) {
+        e = hipMemcpyAsync(S.d_prep, S.h_prep, preps * sizeof(DbdPrepSeg), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(simk_dbd_prep_kernel, dim3((max_zones + 255) / 256, preps), dim3(256), 0, st,
+                               (const DbdPrepSeg *)S.d_prep);
+            e = hipGetLastError();
+        }
+    }
+    if (!count || e != hipSuccess) return e;
+    e = hipMemcpyAsync(S.d_segs, S.h_segs, count * sizeof(SimkUpwindSeg), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = simk_enqueue_upwind_batch(st, S.d_segs, count, rows, max_nodes, S.cfg.substeps);
     return e;
 }
@@ -305,7 +427,7 @@ GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
         sl.n = shape.n[i];
         sl.R = shape.R[i];
         sl.zone_rows = zone_rows(sl.n, sl.R);
-        const u64 rows = sl.zone_rows + 1 + kResumeRows;
+        const u64 rows = sl.zone_rows + 1 + kResumeRows + kDbdRows;
         const u64 widest = sl.R > sl.n ? sl.R : sl.n;   // row group whose byte offsets must stay below 2^31
         // Rounded up, so any request within the budget fits an empty slab.
         u64 cap = (c.hbm_budget / (rows * 8) + 63) & ~63ull;
@@ -333,6 +455,8 @@ GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
     if (e == hipSuccess) e = hipMalloc(&S.d_inflow, (u64)c.max_requests * kMaxSpecies * 8);
     if (e == hipSuccess) e = hipMalloc(&S.d_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg));
     if (e == hipSuccess) e = hipHostMalloc(&S.h_segs, (u64)c.max_requests * sizeof(SimkUpwindSeg), 0);
+    if (e == hipSuccess) e = hipMalloc(&S.d_prep, (u64)c.max_requests * sizeof(DbdPrepSeg));
+    if (e == hipSuccess) e = hipHostMalloc(&S.h_prep, (u64)c.max_requests * sizeof(DbdPrepSeg), 0);
     for (hipEvent_t &ev : S.ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&S.admitted, hipEventDisableTiming);
@@ -352,15 +476,53 @@ GPU_API s32 simk_serve_submit(const SimkServeRequest *request, u32 *ticket) {
     if (!S.open) return code(hipErrorNotInitialized);
     if (!request || !ticket) return code(hipErrorInvalidValue);
     const SimkServeRequest &r = *request;
-    if (r.mech >= S.shape.count || r.cells == 0 || r.channels == 0 || r.nodes == 0 || r.nodes > kMaxNodes ||
-        (u64)r.channels * r.nodes != r.cells || r.padded < r.cells || !r.block || !r.courant || !r.result)
+    if (r.mech >= S.shape.count || r.cells == 0 || r.padded < r.cells || !r.block || !r.result ||
+        r.kind > SimkServe_Dbd)
+        return code(hipErrorInvalidValue);
+    if (r.kind == SimkServe_Transport && (r.channels == 0 || r.nodes == 0 || r.nodes > kMaxNodes ||
+                                          (u64)r.channels * r.nodes != r.cells || !r.courant))
         return code(hipErrorInvalidValue);
     const Slab &sl = S.slabs[r.mech];
-    const u64 bytes = (u64)r.cells * (sl.zone_rows + 1 + kResumeRows) * 8;
+    Request q;
+    if (r.kind == SimkServe_Dbd) {
+        const SimkServeDbd *d = r.dbd;
+        if (!d || d->source.boltzmann || sl.n > kDbdMaxSpecies || !d->T_K || !d->power_W_cm3 || !d->freq_Hz ||
+            !d->pulse_index || d->mode > SimkDbd_Averaged)
+            return code(hipErrorInvalidValue);
+        memcpy(q.species, &d->species, sizeof(q.species));
+        for (u32 k = 0; k < 17; ++k)
+            if (q.species[k] >= sl.n) return code(hipErrorInvalidValue);
+        if (d->residence_s > 0.0) {
+            // plug flow toward the feed: a = dt * zones / residence <= 1 (dbd.cpp:473)
+            if (!r.inflow || r.cells > kMaxNodes || !(S.cfg.dt * (f64)r.cells / d->residence_s <= 1.0))
+                return code(hipErrorInvalidValue);
+        }
+        q.mode = d->mode;
+        q.residence = d->residence_s > 0.0 ? d->residence_s : 0.0;
+        q.g[0] = d->source.g_O2_diss;
+        q.g[1] = d->source.g_N2A;
+        q.g[2] = d->source.g_N2_diss;
+        q.g[3] = d->source.g_OH_per_100eV;
+        q.f_O1D = d->source.f_O1D;
+        q.filament = d->source.filament_J_cm3;
+        q.dbd_rows.assign((u64)kDbdRows * r.cells, 0.0);
+        f64 *rows = q.dbd_rows.data();
+        for (u32 i = 0; i < r.cells; ++i) {
+            rows[kDbdT * r.cells + i] = d->T_K[i];
+            rows[kDbdPower * r.cells + i] = d->power_W_cm3[i];
+            rows[kDbdFreq * r.cells + i] = d->freq_Hz[i];
+            rows[kDbdPulseIndex * r.cells + i] = d->pulse_index[i];
+            f64 rates[simk_air::kQuenchRates];
+            simk_air::air_quench_rates(d->T_K[i], rates);
+            for (u32 c = 0; c < simk_air::kQuenchRates; ++c) rows[(kDbdRates + c) * r.cells + i] = rates[c];
+        }
+    }
+    const u64 bytes = (u64)r.cells * (sl.zone_rows + 1 + kResumeRows + kDbdRows) * 8;
     if (bytes > S.cfg.hbm_budget || r.cells > sl.cap) return code(hipErrorInvalidValue);
     if (S.free_slots.empty()) return code(hipErrorOutOfMemory);
-    Request q;
     q.r = r;
+    q.r.dbd = nullptr;   // the caller's description is staged; it need not outlive this call
+    q.steps = r.steps ? r.steps : S.cfg.steps;
     q.ticket = S.next_ticket++;
     if (S.next_ticket == 0) S.next_ticket = 1;
     q.bytes = bytes;
@@ -444,7 +606,23 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
         // Requests whose last step just ended: copy their solver rows back.
         for (u32 t : step_done) {
             Request &q = S.reqs[t];
-            if (++q.steps_done < S.cfg.steps) {
+            if (q.r.kind == SimkServe_Dbd) {
+                // the step's end: pulses committed, then plug flow along the chain (dbd_scene_step)
+                Slab &sl = S.slabs[q.r.mech];
+                u64 off[12];
+                zone_offsets(sl.n, sl.R, sl.cap, off);
+                hipLaunchKernelGGL(simk_dbd_commit_kernel, dim3((q.r.cells + 255) / 256), dim3(256), 0, st,
+                                   sl.dbd() + (u64)kDbdPulseIndex * sl.cap + q.off,
+                                   (const f64 *)(sl.d + off[11] + q.off), q.r.cells);
+                e = hipGetLastError();
+                if (e == hipSuccess && q.residence > 0.0)
+                    e = simk_enqueue_plugflow(st, sl.d + off[0] + q.off, sl.cap, sl.n, q.r.cells,
+                                              S.cfg.dt * (f64)q.r.cells / q.residence,
+                                              S.d_inflow + (u64)q.slot * kMaxSpecies);
+                if (e != hipSuccess) return code(e);
+                q.t = q.t + S.cfg.dt;
+            }
+            if (++q.steps_done < q.steps) {
                 q.phase = kAdvect;
                 continue;
             }

@@ -46,8 +46,15 @@ class SimkServeConfig(C.Structure):
 
 class SimkServeRequest(C.Structure):
     _fields_ = [("mech", C.c_uint32), ("cells", C.c_uint32), ("channels", C.c_uint32), ("nodes", C.c_uint32),
-                ("padded", C.c_uint32), ("reserved", C.c_uint32), ("block", C.c_void_p), ("courant", C.c_void_p),
-                ("inflow", C.c_void_p), ("result", C.c_void_p)]
+                ("padded", C.c_uint32), ("kind", C.c_uint32), ("block", C.c_void_p), ("courant", C.c_void_p),
+                ("inflow", C.c_void_p), ("result", C.c_void_p), ("steps", C.c_uint32), ("pad_", C.c_uint32),
+                ("dbd", C.c_void_p)]
+
+
+class SimkServeDbd(C.Structure):
+    _fields_ = [("species", _ffi.SimkAirSpecies), ("mode", C.c_uint32), ("source", _ffi.SimkAirSource),
+                ("residence_s", C.c_double), ("T_K", C.c_void_p), ("power_W_cm3", C.c_void_p),
+                ("freq_Hz", C.c_void_p), ("pulse_index", C.c_void_p)]
 
 
 class SimkServeStats(C.Structure):
@@ -69,7 +76,8 @@ EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_a
                      "simk_kin_set_mechs", "simk_kin_advance_multi", "simk_graph_capture_multi", "simk_graph_replay",
                      "simk_graph_destroy", "simk_jit_precompile_multi", "simk_serve_open", "simk_serve_submit",
                      "simk_serve_step", "simk_serve_stats", "simk_serve_close", "simk_kin_set_cache_budget",
-                     "simk_kin_autotune", "simk_kin_autotune_multi", "simk_plugflow_f64")
+                     "simk_kin_autotune", "simk_kin_autotune_multi", "simk_plugflow_f64",
+                     "simk_kin_set_mech_control")
 
 
 def _sig(lib, name, res, *args):
@@ -129,6 +137,7 @@ def load_library(path=_ffi.GPU_LIB):
          P(SimkTuneResult))
     _sig(lib, "simk_serve_open", C.c_int32, P(SimkServeConfig))
     _sig(lib, "simk_serve_submit", C.c_int32, P(SimkServeRequest), P(C.c_uint32))
+    _sig(lib, "simk_kin_set_mech_control", C.c_int32, C.c_uint32, P(_ffi.KinControl))
     _sig(lib, "simk_serve_step", C.c_int32, P(C.c_uint32), C.c_uint32, P(C.c_uint32))
     _sig(lib, "simk_serve_stats", C.c_int32, P(SimkServeStats))
     _sig(lib, "simk_serve_close", C.c_int32)
@@ -239,11 +248,35 @@ class GpuModule:
         inflow = None if inflow is None else np.ascontiguousarray(inflow, dtype=np.float64)
         assert result.dtype == np.float64 and result.flags.c_contiguous and result.shape[1] == cells
         req = SimkServeRequest(mech_index, cells, channels, nodes, padded, 0, block_ptr, courant.ctypes.data,
-                               None if inflow is None else inflow.ctypes.data, result.ctypes.data)
+                               None if inflow is None else inflow.ctypes.data, result.ctypes.data, 0, 0, None)
         t = C.c_uint32()
         self._check(self.lib.simk_serve_submit(C.byref(req), C.byref(t)), "simk_serve_submit")
         self._serve_keep[t.value] = (courant, inflow, result, keep)
         return t.value
+
+    def serve_submit_dbd(self, mech_index, zones, steps, residence=2.0, mode=0, source=None, feed=None, species=None):
+        """Queues a DBD reactor request: a dbd.DbdZones block (its operating
+        point and pulse clock), `steps` master steps, plug flow over
+        `residence` seconds toward `feed` (0: independent zones). `species`:
+        the air species map (default: the block's mechanism's). Returns
+        (ticket, result array [species + 5, zones])."""
+        from . import dbd as _dbd
+        result = np.zeros((zones.n + 5, zones.count))
+        arrays = [np.ascontiguousarray(getattr(zones, a)[:zones.count], dtype=np.float64)
+                  for a in ("T_K", "power_W_cm3", "freq_Hz", "pulse_index")]
+        d = SimkServeDbd(species if species is not None else zones.mech.sp, mode, source if source is not None else _dbd.source_defaults(), residence,
+                         *[a.ctypes.data for a in arrays])
+        feed = None if feed is None else np.ascontiguousarray(feed, dtype=np.float64)
+        req = SimkServeRequest(mech_index, zones.count, 1, zones.count, zones.padded, 1, zones.block.ctypes.data, None,
+                               None if feed is None else feed.ctypes.data, result.ctypes.data, steps, 0,
+                               C.addressof(d))
+        t = C.c_uint32()
+        self._check(self.lib.simk_serve_submit(C.byref(req), C.byref(t)), "simk_serve_submit")
+        self._serve_keep[t.value] = (feed, result, zones)
+        return t.value, result
+
+    def set_mech_control(self, index, ctl):
+        self._check(self.lib.simk_kin_set_mech_control(index, C.byref(ctl)), "simk_kin_set_mech_control")
 
     def serve_submit_zones(self, mech_index, zones, channels, nodes, courant, inflow=None):
         """serve_submit of a whole problem.Zones block; returns (ticket, result array)."""

@@ -122,8 +122,9 @@ def test_time_slices_keep_bits(gm, pools):
 def test_budget_admission_serialises_and_keeps_results(gm, pools):
     reqs = [(1, 0, 2, 500), (1, 1000, 2, 500), (0, 3000, 1, 1000)]
     free, st_free = _serve(gm, pools, reqs)
-    rows1 = 3 * pools.mechs[1].n + pools.mechs[1].R + 14     # zone rows + courant + 5 resume rows
-    rows0 = 3 * pools.mechs[0].n + pools.mechs[0].R + 14
+    # zone rows (3n + R + 8) + courant + 5 resume rows + 24 DBD-request rows
+    rows1 = 3 * pools.mechs[1].n + pools.mechs[1].R + 38
+    rows0 = 3 * pools.mechs[0].n + pools.mechs[0].R + 38
     budget = max(1000 * rows1 * 8, 1000 * rows0 * 8)   # one request at a time
     tight, st = _serve(gm, pools, reqs, budget=budget)
     assert st.peak_admitted_bytes <= budget

@@ -86,6 +86,35 @@ typedef struct SimkServeDbd {
     const f64 *T_K, *power_W_cm3, *freq_Hz, *pulse_index;   /* [cells]: operating point, pulses fired so far */
 } SimkServeDbd;
 
+/* Split of a zone block between the GPU module and the caller's CPU path, for
+ * a caller that advances both at once as dbd_advance_hybrid does
+ * (src/sim/dbd.cpp:296-463): lanes [0, split) through gpu_kin_advance_async
+ * (GpuKinLaunch.lane_count = split), [split, padded) on its CPU path, then
+ * gpu_kin_wait. Every lane gives the same bits on either side, so the split
+ * changes only the wall time. The controller fits t_cpu = c (padded - n) and
+ * t_gpu = o + g n to the reported timings and picks the n that ends both
+ * together (multiples of 8 lanes; 0 when the device saves under 3 %),
+ * dithering the split while the fit needs a second point and every 16th
+ * advance after. Its own design; the reference chooses by bucketed timings
+ * and local probes. */
+typedef struct SimkHybrid {
+    u32 padded;          /* lanes in the block */
+    u32 split;           /* lanes for the device at the next advance */
+    u32 best;            /* the model's optimum (split differs from it on dither steps) */
+    u32 steps, cpu_samples, gpu_samples, dither, pad_;
+    f64 cpu_lane_ms;     /* c: CPU milliseconds per lane */
+    f64 gpu_fixed_ms;    /* o: device milliseconds of an advance independent of its lanes (launch, copies) */
+    f64 gpu_lane_ms;     /* g: device milliseconds per lane */
+    f64 w, sn, snn, st, snt;   /* decaying least-squares sums of the device samples */
+} SimkHybrid;
+
+/* A new controller; its first split is about half the block. */
+SIMK_HOST_API void simk_hybrid_init(SimkHybrid *h, u32 padded);
+/* Reports one advance: the split used, the device's time for its share
+ * (gpu_kin_wait's total_ms, copies included) and the CPU path's time for the
+ * rest; sets h->split for the next advance. */
+SIMK_HOST_API void simk_hybrid_record(SimkHybrid *h, u32 split, f64 gpu_ms, f64 cpu_ms);
+
 #ifdef __cplusplus
 }
 #endif

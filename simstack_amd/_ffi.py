@@ -99,6 +99,13 @@ class SimkDbdZones(C.Structure):
                 ("power_W_cm3", C.c_void_p), ("freq_Hz", C.c_void_p), ("pulse_index", C.c_void_p)]
 
 
+class SimkHybrid(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("padded", "split", "best", "steps", "cpu_samples", "gpu_samples", "dither",
+                                          "pad_")] + \
+               [(f, C.c_double) for f in ("cpu_lane_ms", "gpu_fixed_ms", "gpu_lane_ms", "w", "sn", "snn", "st",
+                                          "snt")]
+
+
 def _sig(lib, name, res, *args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -149,5 +156,7 @@ def host():
          C.c_uint32, C.c_double, C.c_double)
     _sig(lib, "simk_dbd_commit_pulses", None, P(SimkDbdZones))
     _sig(lib, "simk_dbd_default_control", None, P(KinControl))
+    _sig(lib, "simk_hybrid_init", None, P(SimkHybrid), C.c_uint32)
+    _sig(lib, "simk_hybrid_record", None, P(SimkHybrid), C.c_uint32, C.c_double, C.c_double)
     _host = lib
     return lib

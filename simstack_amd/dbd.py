@@ -189,3 +189,50 @@ class Scene:
         """FNV-1a 64 of every zone's species row (dbd_scene_hash,
         src/sim/dbd_scene.cpp:388-391: n rows of the zones, in row order)."""
         return fnv1a64(np.ascontiguousarray(self.z.y[:, :self.count]).tobytes())
+
+
+class Hybrid:
+    """One zone block advanced by the GPU module and a CPU path at once, as the
+    reference caller dbd_advance_hybrid does (src/sim/dbd.cpp:296-463): lanes
+    [0, split) go to the device through gpu_kin_advance_async (the host block
+    is copied in and the lanes' mutable rows back, `split` lanes wide), while
+    `cpu_advance(zones, duration, lanes)` integrates [split, padded) on the
+    host; gpu_kin_wait joins them. Each lane gets the same bits on either
+    side, so the split (`force_split`, or the simk_hybrid_* controller fed
+    with each advance's timings) changes only the wall time.
+
+    `dz` is a gpu.DeviceZones of the block; `cpu_advance` is the caller's CPU
+    path (the tests pass the oracle)."""
+
+    def __init__(self, gm, dz, cpu_advance, force_split=None):
+        self.gm, self.dz, self.cpu_advance = gm, dz, cpu_advance
+        self.force_split = force_split
+        self.ctl = _ffi.SimkHybrid()
+        _ffi.host().simk_hybrid_init(C.byref(self.ctl), dz.z.padded)
+        self.history = []
+
+    @property
+    def split(self):
+        s = self.ctl.split if self.force_split is None else self.force_split
+        return min(s & ~7, self.dz.z.padded)
+
+    def advance(self, duration, precision=0):
+        import time
+        z, gm = self.dz.z, self.gm
+        split = self.split
+        kms, tms = C.c_float(0.0), C.c_float(0.0)
+        if split:
+            L = self.dz.launch_desc(duration, lane_count=split, precision=precision)
+            gm._check(gm.lib.gpu_kin_advance_async(C.byref(L), z.block.ctypes.data, self.dz.d_block,
+                                                   z.total // z.padded, z.mutable // z.padded),
+                      "gpu_kin_advance_async")
+        t0 = time.perf_counter()
+        if split < z.padded:
+            self.cpu_advance(z, duration, np.arange(split, z.padded, dtype=np.uint32))
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        if split:
+            gm._check(gm.lib.gpu_kin_wait(C.byref(kms), C.byref(tms)), "gpu_kin_wait")
+        self.history.append((split, tms.value, cpu_ms))
+        if self.force_split is None:
+            _ffi.host().simk_hybrid_record(C.byref(self.ctl), split, tms.value, cpu_ms)
+        return split

@@ -137,6 +137,9 @@ typedef struct SimkGenOptions {
                             160 KiB) */
     u32 fun_in_jac;      /* 0: default (on); 1: the -J pass also evaluates f(y) for stage 0 (one rate-constant
                             pass fewer per attempt); ~0u: stage 0 evaluates f(y) itself */
+    u32 pair_slots;      /* FP64 kernels: 1: workspace slots in adjacent pairs per lane ([pair][64 lanes][2]), the
+                            LU entries in the triangular solves' read order, so that two slots move in one
+                            16-byte access; ~0u: one 8-byte slot per access; 0: default (paired) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 

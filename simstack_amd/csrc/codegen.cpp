@@ -18,6 +18,7 @@
 // ever touches its own column, so no barrier is needed).
 #include "codegen.hpp"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <map>
@@ -53,7 +54,34 @@ struct Op {
     std::string stmt;      // "%s" stands for the loaded value
     bool barrier = false;  // loads never move across it (fences, stores they depend on)
     bool immutable = false;   // the load reads a slot no statement writes (a rate constant)
+    int slot = -1;         // paired workspace: the slot this op's load reads (W(slot)), else -1
+    int from = -1;         // the value is half of op `from`'s pair load (this op loads nothing)
+    std::string part;      // ".x" / ".y": the half of a pair load this op uses
+    std::string type;      // the loaded value's type when not the segment's default
 };
+
+// Paired workspace: an op loading slot s and one of the next `lookahead` ops
+// loading its partner s ^ 1 (no barrier between) share one 16-byte load.
+void pair_loads(std::vector<Op> &ops, size_t lookahead = 4) {
+    for (size_t i = 0; i < ops.size(); ++i) {
+        Op &a = ops[i];
+        if (a.slot < 0 || a.from >= 0 || !a.type.empty() || a.barrier) continue;
+        for (size_t j = i + 1; j < ops.size() && j <= i + lookahead; ++j) {
+            Op &b = ops[j];
+            if (b.barrier) break;
+            if (b.slot != (a.slot ^ 1) || b.from >= 0 || !b.type.empty()) continue;
+            char buf[32];
+            snprintf(buf, sizeof(buf), "W2(%d)", a.slot >> 1);
+            a.load = buf;
+            a.type = "SimkD2";
+            a.part = (a.slot & 1) ? ".y" : ".x";
+            b.load.clear();
+            b.from = (int)i;
+            b.part = (b.slot & 1) ? ".y" : ".x";
+            break;
+        }
+    }
+}
 
 // Emits ops with their loads software-pipelined `batch` ops ahead: the loads
 // of ops [b + batch, b + 2 batch) are issued before the statements of
@@ -86,7 +114,7 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
                 char buf[32];
                 snprintf(buf, sizeof(buf), "q%u", uid++);
                 name[issued - seg] = buf;
-                w("%sconst %s %s = %s;\n", indent, type, buf, op.load.c_str());
+                w("%sconst %s %s = %s;\n", indent, op.type.empty() ? type : op.type.c_str(), buf, op.load.c_str());
                 if (reuse && op.immutable) recent[op.load] = {buf, issued};
             }
         };
@@ -95,8 +123,9 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
             if ((i - seg) % batch == 0) issue_until(i + 2 * batch);
             const Op &o = ops[i];
             std::string st = o.stmt;
+            if (st.empty()) continue;
             size_t at = st.find("%s");
-            if (at != std::string::npos) st.replace(at, 2, name[i - seg]);
+            if (at != std::string::npos) st.replace(at, 2, (o.from >= 0 ? name[o.from - seg] : name[i - seg]) + o.part);
             w("%s%s\n", indent, st.c_str());
         }
         if (end < ops.size()) w("%s%s\n", indent, ops[end].stmt.c_str());   // the barrier itself
@@ -123,6 +152,14 @@ u32 lds_entries(const Mech &m, const GenOptions &o) {
 struct Layout {
     u32 lds, koff, foff, rhsoff, lu32off, k32off, kcp, scp, slots;
     bool k32;
+    // Paired FP64 workspace: [paired region: LU entries lds.. in the solves'
+    // read order (forward: each row's lower entries; backward, rows n-1..0:
+    // upper entries then the pivot), K stages and f at stride np (n rounded
+    // up to even) | unpaired region: k copy, src copy]; kcp and scp are
+    // offsets in the unpaired region. Otherwise np = n and paired = 0.
+    bool pairs = false;
+    u32 np = 0, paired = 0;
+    std::vector<u32> lu_slot;   // workspace slot of LU entry idx >= lds
 };
 
 static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
@@ -155,6 +192,31 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
     L.kcp = end;
     L.scp = L.kcp + m.R;
     L.slots = L.scp + m.n;
+    L.np = m.n;
+    L.lu_slot.assign(m.nnz, 0);
+    for (u32 idx = L.lds; idx < m.nnz; ++idx) L.lu_slot[idx] = idx - L.lds;
+    if (!mixed && o.pair_slots) {
+        L.pairs = true;
+        u32 t = 0;
+        for (u32 i = 0; i < m.n; ++i)
+            for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e)
+                if (m.lower[e].index >= L.lds) L.lu_slot[m.lower[e].index] = t++;
+        t += t & 1u;   // the backward sweep starts a pair
+        for (u32 ii = m.n; ii-- > 0;) {
+            for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e)
+                if (m.upper[e].index >= L.lds) L.lu_slot[m.upper[e].index] = t++;
+            if (m.diag[ii] >= L.lds) L.lu_slot[m.diag[ii]] = t++;
+        }
+        t += t & 1u;
+        L.np = m.n + (m.n & 1u);
+        L.koff = t;
+        L.foff = L.koff + stored_stages * L.np;
+        L.paired = L.foff + L.np;
+        L.rhsoff = L.lu32off = L.paired;
+        L.kcp = 0;
+        L.scp = m.R;
+        L.slots = L.paired + m.R + m.n;
+    }
     return L;
 }
 
@@ -187,6 +249,14 @@ u32 auto_reuse(const GenOptions &o, bool multi) {
 }
 
 u32 auto_fun_in_jac(const GenOptions &) { return 1u; }
+
+// FP64 kernels, paired against single workspace slots (MI355X, bit-identical
+// results): 48/150 at 65,536 cells 529 -> 512 ms (+3 %), at 262,144 cells
+// unchanged (1,674 / 1,671 ms); 60/200 at 65,536 cells +4.6 %; the DBD air
+// mechanism at 65,536 zones 16.0 -> 14.2 ms (+13 %); the three-mechanism
+// kernel at 221,000 cells 318 -> 286 ms (+11 %). Mixed-precision and f32
+// delta kernels keep their own layouts.
+u32 auto_pair(const GenOptions &o) { return o.precision == 0 ? 1u : 0u; }
 
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
@@ -225,15 +295,18 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const Layout Lo = layout(m, me, o);
     const u32 lds = Lo.lds, koff = Lo.koff, foff = Lo.foff, rhsoff = Lo.rhsoff, lu32off = Lo.lu32off;
     const u32 kcp = Lo.kcp, scp = Lo.scp, ws_slots = Lo.slots;
+    const bool pairs = Lo.pairs;
     w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
     w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u, K32OFF = %u;\n", koff, foff,
       rhsoff, me.stages, Lo.k32off);
+    w("    static constexpr simk_u32 NP = %u, PAIRED = %u;\n", Lo.np, Lo.paired);
     // jac_factor also evaluates f(y) and leaves it in the f slots (fij).
     const bool fij = o.fun_in_jac != 0;
     w("    static constexpr bool kFunInJac = %s;\n", fij ? "true" : "false");
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
+    if (pairs) w("    SimkRowPairs wp;\n");
     // srcnz: the lane's cell has a source row with a bit set. A wave whose
     // lanes all have all-zero (+0.0) rows starts f from the literal +0.0,
     // which is what loading those rows would give.
@@ -251,17 +324,22 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     w("        kr.init(A.k, 0, stride, R * stride);\n");
     w("        sr.init(A.src, 0, stride, N * stride);\n");
     w("        pr.init(A.pulse, 0, stride, N * stride);\n");
-    w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    if (pairs) {
+        w("        wp.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane, PAIRED * SIMK_WAVE * 8u);\n");
+        w("        ws.init(A.ws + ((simk_u64)wave * A.ws_slots + PAIRED) * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, (R + N) * SIMK_WAVE * 8u);\n");
+    } else {
+        w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    }
     if (mixed)
         w("        w32.init(A.ws + ((simk_u64)wave * A.ws_slots + %u) * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, %u * SIMK_WAVE * 8u);\n",
           lu32off, ws_slots - lu32off);
     w("    }\n");
     w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
     w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
-    w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) WS(%u + r, kr.ld(r));\n", kcp);
+    w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) ws.st(%u + r, kr.ld(r));\n", kcp);
     w("        srcnz = 0;\n");
     w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) { const double v = sr.ld(s); srcnz |= "
-      "(simk_u32)(__builtin_bit_cast(simk_u64, v) != 0); WS(%u + s, v); }\n", scp);
+      "(simk_u32)(__builtin_bit_cast(simk_u64, v) != 0); ws.st(%u + s, v); }\n", scp);
     w("    }\n");
     w("    SIMK_DEV bool any_src() const { return __builtin_amdgcn_ballot_w64(srcnz != 0) != 0; }\n");
     w("    static SIMK_DEV constexpr simk_u32 n() { return N; }\n");
@@ -269,8 +347,15 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     w("    SIMK_DEV double K(simk_u32 r) const { return ws.ld(%u + r); }\n", kcp);
     w("    SIMK_DEV double S(simk_u32 s) const { return ws.ld(%u + s); }\n", scp);
     w("    SIMK_DEV double pulse(simk_u32 s) const { return pr.ld(s); }\n");
-    w("    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }\n");
-    w("    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }\n");
+    if (pairs) {
+        w("    SIMK_DEV double W(simk_u32 i) const { return wp.ld(i); }\n");
+        w("    SIMK_DEV void WS(simk_u32 i, double v) const { wp.st(i, v); }\n");
+        w("    SIMK_DEV SimkD2 W2(simk_u32 pair) const { return wp.ld2(pair); }\n");
+        w("    SIMK_DEV void WS2(simk_u32 pair, double a, double b) const { wp.st2(pair, a, b); }\n");
+    } else {
+        w("    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }\n");
+        w("    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }\n");
+    }
     if (mixed) {
         w("    SIMK_DEV float W32(simk_u32 i) const { return w32.ld(i); }\n");
         w("    SIMK_DEV void W32S(simk_u32 i, float v) const { w32.st(i, v); }\n");
@@ -280,6 +365,13 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
         w("        if (st + 1 >= STAGES) return;\n");
         w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) W32S(K32OFF + st * N + s, (float)x.v[s]);\n    }\n");
+    } else if (pairs) {
+        // stage vectors at even stride NP: slots s, s + 1 of a vector are a pair
+        w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(KOFF + st * NP + s); }\n");
+        w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
+        w("        if (st + 1 >= STAGES) return;\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s + 1 < N; s += 2) WS2((KOFF + st * NP + s) / 2, x.v[s], x.v[s + 1]);\n");
+        w("        if (N & 1u) WS(KOFF + st * NP + N - 1, x.v[N - 1]);\n    }\n");
     } else {
         w("    SIMK_DEV double kget(simk_u32 st, simk_u32 s) const { return W(KOFF + st * N + s); }\n");
         w("    SIMK_DEV void kstore(simk_u32 st, const Vec &x) const {\n");
@@ -292,8 +384,19 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         char buf[96];
         for (u32 s2 = 0; s2 < n; ++s2) {
             Op op;
-            snprintf(buf, sizeof(buf), Lo.k32 ? "(double)W32(K32OFF + j * N + %u)" : "W(KOFF + j * N + %u)", s2);
-            op.load = buf;
+            if (pairs && s2 % 2 == 1 && s2 < n) {
+                op.from = (int)ops.size() - 1;   // the odd slot of the pair loaded for s2 - 1
+                op.part = ".y";
+            } else if (pairs && s2 + 1 < n) {
+                snprintf(buf, sizeof(buf), "W2((KOFF + j * NP + %u) / 2)", s2);
+                op.load = buf;
+                op.type = "SimkD2";
+                op.part = ".x";
+            } else {
+                snprintf(buf, sizeof(buf), Lo.k32 ? "(double)W32(K32OFF + j * N + %u)"
+                                                  : (pairs ? "W(KOFF + j * NP + %u)" : "W(KOFF + j * N + %u)"), s2);
+                op.load = buf;
+            }
             snprintf(buf, sizeof(buf), "v.v[%u] = simk_fma(c, %%s, v.v[%u]);", s2, s2);
             op.stmt = buf;
             ops.push_back(op);
@@ -306,12 +409,22 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // fjac: f(y) as jac_factor left it in the f slots. The saved f stays in
     // workspace slots: keeping it in registers was measured slower (the
     // register pressure costs more than the slot traffic it saves).
-    w("    SIMK_DEV void fjac(Vec &x) const {\n");
-    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
-    w("    SIMK_DEV void fsave(const Vec &x) const {\n");
-    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(FOFF + s, x.v[s]);\n    }\n");
-    w("    SIMK_DEV void fload(Vec &x) const {\n");
-    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
+    if (pairs) {
+        const char *ld = "#pragma unroll\n        for (simk_u32 s = 0; s + 1 < N; s += 2) { const SimkD2 t = W2((FOFF + s) / 2); "
+                         "x.v[s] = t.x; x.v[s + 1] = t.y; }\n        if (N & 1u) x.v[N - 1] = W(FOFF + N - 1);\n";
+        w("    SIMK_DEV void fjac(Vec &x) const {\n%s    }\n", ld);
+        w("    SIMK_DEV void fsave(const Vec &x) const {\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s + 1 < N; s += 2) WS2((FOFF + s) / 2, x.v[s], x.v[s + 1]);\n");
+        w("        if (N & 1u) WS(FOFF + N - 1, x.v[N - 1]);\n    }\n");
+        w("    SIMK_DEV void fload(Vec &x) const {\n%s    }\n", ld);
+    } else {
+        w("    SIMK_DEV void fjac(Vec &x) const {\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
+        w("    SIMK_DEV void fsave(const Vec &x) const {\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) WS(FOFF + s, x.v[s]);\n    }\n");
+        w("    SIMK_DEV void fload(Vec &x) const {\n");
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) x.v[s] = W(FOFF + s);\n    }\n");
+    }
 
     // fun: one op per reaction, rate = k * y_a [* y_b], then its net entries
     // in order (kin_codegen.cpp:58-72 order)
@@ -372,13 +485,13 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // slots; W32: the FP32 LU region of mixed kernels).
     auto lu_ld = [&](u32 idx) {
         char buf[24];
-        snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : (mixed ? "W32(%u)" : "W(%u)"), idx < lds ? idx : idx - lds);
+        snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : (mixed ? "W32(%u)" : "W(%u)"), idx < lds ? idx : Lo.lu_slot[idx]);
         return std::string(buf);
     };
     auto lu_st = [&](u32 idx, const char *v) {
         char buf[40];
         snprintf(buf, sizeof(buf), idx < lds ? "LS(%u, %s%u);" : (mixed ? "W32S(%u, %s%u);" : "WS(%u, %s%u);"),
-                 idx < lds ? idx : idx - lds, v, idx);
+                 idx < lds ? idx : Lo.lu_slot[idx], v, idx);
         return std::string(buf);
     };
     {
@@ -485,6 +598,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 Op op;
                 if (!kk_reg) {
                     op.load = lu_ld(dv.kk);
+                    if (pairs && dv.kk >= lds) op.slot = (int)Lo.lu_slot[dv.kk];
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %%s; %s%u = l%u;", T, d, V, dv.ik, V, dv.ik, d);
                 } else {
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %s%u; %s%u = l%u;", T, d, V, dv.ik, V, dv.kk, V,
@@ -500,6 +614,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %s%u, %s%u);", V, sb.ij, fn, d, V, sb.kj, V, sb.ij);
                     } else {
                         o2.load = lu_ld(sb.kj);
+                        if (pairs && sb.kj >= lds) o2.slot = (int)Lo.lu_slot[sb.kj];
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %%s, %s%u);", V, sb.ij, fn, d, V, sb.ij);
                     }
                     o2.stmt = buf;
@@ -513,14 +628,42 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             } else {
                 snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
                 stmt(buf);
-                for (u32 idx : row) stmt(lu_st(idx, "a"));
+                if (!pairs) {
+                    for (u32 idx : row) stmt(lu_st(idx, "a"));
+                } else {
+                    // the row's workspace entries by slot, partners in one 16-byte store
+                    std::vector<std::pair<u32, u32>> ws_row;   // (slot, compact index)
+                    for (u32 idx : row) {
+                        if (idx < lds) stmt(lu_st(idx, "a"));
+                        else ws_row.push_back({Lo.lu_slot[idx], idx});
+                    }
+                    std::sort(ws_row.begin(), ws_row.end());
+                    for (size_t q = 0; q < ws_row.size(); ++q) {
+                        const u32 sl = ws_row[q].first;
+                        if (sl % 2 == 0 && q + 1 < ws_row.size() && ws_row[q + 1].first == sl + 1) {
+                            snprintf(buf, sizeof(buf), "WS2(%u, a%u, a%u);", sl / 2, ws_row[q].second,
+                                     ws_row[q + 1].second);
+                            ++q;
+                        } else {
+                            snprintf(buf, sizeof(buf), "WS(%u, a%u);", sl, ws_row[q].second);
+                        }
+                        stmt(buf);
+                    }
+                }
             }
-            if (fij) {
+            if (fij && pairs) {
+                // f in pairs: rows 2k and 2k + 1 stored together
+                if (i % 2 == 1) snprintf(buf, sizeof(buf), "WS2((FOFF + %u) / 2, f%u, f%u);", i - 1, i - 1, i);
+                else if (i + 1 == n) snprintf(buf, sizeof(buf), "WS(FOFF + %u, f%u);", i, i);
+                else buf[0] = 0;
+                if (buf[0]) stmt(buf);
+            } else if (fij) {
                 snprintf(buf, sizeof(buf), "WS(FOFF + %u, f%u);", i, i);
                 stmt(buf);
             }
         }
         barrier();
+        if (pairs) pair_loads(ops);
         w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
         if (fij) w("        const bool anysrc = any_src();\n");
         if (mixed) w("        const float gf = (float)g;\n");
@@ -538,6 +681,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
                 Op op;
                 op.load = lu_ld(m.lower[e].index);
+                if (pairs && m.lower[e].index >= lds) op.slot = (int)Lo.lu_slot[m.lower[e].index];
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, i);
                 std::string st = buf + f + ", ";
@@ -549,6 +693,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
                 Op op;
                 op.load = lu_ld(m.upper[e].index);
+                if (pairs && m.upper[e].index >= lds) op.slot = (int)Lo.lu_slot[m.upper[e].index];
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, ii);
                 std::string st = buf + f + ", ";
@@ -558,10 +703,12 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             }
             Op op;
             op.load = lu_ld(m.diag[ii]);
+            if (pairs && m.diag[ii] >= lds) op.slot = (int)Lo.lu_slot[m.diag[ii]];
             snprintf(buf, sizeof(buf), "%s[%u] = %s[%u] * %%s;", vec, ii, vec, ii);
             op.stmt = buf;
             ops.push_back(op);
         }
+        if (pairs) pair_loads(ops);
         return ops;
     };
     if (!mixed) {

@@ -28,11 +28,15 @@ struct GenOptions {
                                 // (kAutoLds: resolved by auto_lds before generation)
     u32 fun_in_jac = kAutoFunInJac;   // 1: the -J pass also evaluates f(y) (one rate-constant pass fewer);
                                       // 0: stage 0 evaluates it (kAutoFunInJac: auto_fun_in_jac)
+    u32 pair_slots = kAutoPair;   // FP64 kernels: 1: workspace slots in adjacent pairs per lane, LU entries
+                                  // placed in the solves' read order, two slots per 16-byte access; 0: one
+                                  // 8-byte slot per access (kAutoPair: auto_pair)
     static constexpr u32 kDefaultWindow = 0xFFFF;
     static constexpr u32 kAutoLds = 0xFFFF;
     static constexpr u32 kAutoBatch = 0xFFFF;
     static constexpr u32 kAutoReuse = 0xFFFF;
     static constexpr u32 kAutoFunInJac = 0xFFFF;
+    static constexpr u32 kAutoPair = 0xFFFF;
 };
 
 struct GenResult {
@@ -63,6 +67,7 @@ u32 auto_reuse(const GenOptions &o, bool multi = false);
 
 // The default for f(y) in the -J pass: on.
 u32 auto_fun_in_jac(const GenOptions &o);
+u32 auto_pair(const GenOptions &o);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

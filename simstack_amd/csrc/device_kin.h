@@ -23,6 +23,7 @@
 typedef unsigned int simk_u32;
 typedef unsigned long long simk_u64;
 typedef unsigned int simk_v2u __attribute__((ext_vector_type(2)));
+typedef unsigned int simk_v4u __attribute__((ext_vector_type(4)));
 
 #define SIMK_WAVE 64
 #define SIMK_DEV __device__ __forceinline__
@@ -156,6 +157,43 @@ struct SimkRows {
     }
     SIMK_DEV void st(simk_u32 row, double x) const {
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(simk_v2u, x), rs, (int)voff, (int)(row * stride), 0);
+    }
+};
+
+// Workspace slots held in adjacent pairs per lane, [pair][64 lanes][2] f64,
+// so that two slots a lane reads or writes together move in one 16-byte
+// access (a wave instruction then carries 1 KiB instead of 512 B). A single
+// slot is still one 8-byte access.
+struct SimkD2 {
+    double x, y;
+};
+
+struct SimkRowPairs {
+    __amdgpu_buffer_rsrc_t rs;
+    simk_u32 voff;           // lane * 16
+    static constexpr simk_u32 kPairBytes = SIMK_WAVE * 16u;
+    SIMK_DEV void init(const double *base, simk_u32 lane, simk_u32 bytes) {
+        rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)bytes, SIMK_RSRC_FLAGS);
+        voff = lane * 16u;
+    }
+    SIMK_DEV double ld(simk_u32 slot) const {
+        simk_v2u v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)((slot >> 1) * kPairBytes + (slot & 1u) * 8u), 0);
+        return __builtin_bit_cast(double, v);
+    }
+    SIMK_DEV void st(simk_u32 slot, double x) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(simk_v2u, x), rs, (int)voff,
+                                              (int)((slot >> 1) * kPairBytes + (slot & 1u) * 8u), 0);
+    }
+    SIMK_DEV SimkD2 ld2(simk_u32 pair) const {
+        const simk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)(pair * kPairBytes), 0);
+        SimkD2 r;
+        r.x = __builtin_bit_cast(double, simk_v2u{v.x, v.y});
+        r.y = __builtin_bit_cast(double, simk_v2u{v.z, v.w});
+        return r;
+    }
+    SIMK_DEV void st2(simk_u32 pair, double a, double b) const {
+        const simk_v2u lo = __builtin_bit_cast(simk_v2u, a), hi = __builtin_bit_cast(simk_v2u, b);
+        __builtin_amdgcn_raw_buffer_store_b128(simk_v4u{lo.x, lo.y, hi.x, hi.y}, rs, (int)voff, (int)(pair * kPairBytes), 0);
     }
 };
 

@@ -715,6 +715,7 @@ GenOptions resolved(GenOptions o) {
     if (o.batch == GenOptions::kAutoBatch) o.batch = auto_batch(o);
     if (o.reuse == GenOptions::kAutoReuse) o.reuse = auto_reuse(o);
     if (o.fun_in_jac == GenOptions::kAutoFunInJac) o.fun_in_jac = auto_fun_in_jac(o);
+    if (o.pair_slots == GenOptions::kAutoPair) o.pair_slots = auto_pair(o);
     return o;
 }
 
@@ -1191,6 +1192,7 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.lds_slots = options->lds_slots == 0 ? GenOptions().lds_slots
                                           : (options->lds_slots == 0xFFFFFFFFu ? 0u : options->lds_slots);
     o.fun_in_jac = options->fun_in_jac == 0 ? GenOptions::kAutoFunInJac : (options->fun_in_jac == 0xFFFFFFFFu ? 0u : 1u);
+    o.pair_slots = options->pair_slots == 0 ? GenOptions::kAutoPair : (options->pair_slots == 0xFFFFFFFFu ? 0u : 1u);
     return o;
 }
 
@@ -1634,6 +1636,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.reuse = n.reuse;
     g.gen.lds_slots = n.lds_slots;
     g.gen.fun_in_jac = n.fun_in_jac;
+    g.gen.pair_slots = n.pair_slots;
     g.gen.batch = n.batch;
     // explicit options are what a later untune() restores
     if (g.tuned) g.user_gen = g.gen;
@@ -1719,7 +1722,7 @@ bool same_gen(const GenOptions &a, const GenOptions &b) {
     return a.fence_rows == b.fence_rows && a.waves_per_eu == b.waves_per_eu && a.unroll_stages == b.unroll_stages &&
            a.batch == b.batch && a.precision == b.precision && a.sweeps == b.sweeps &&
            a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots &&
-           a.fun_in_jac == b.fun_in_jac;
+           a.fun_in_jac == b.fun_in_jac && a.pair_slots == b.pair_slots;
 }
 
 // Mixed precision (precision 2): the same protocol over the mixed kernel's
@@ -1933,6 +1936,12 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
     if (e == hipSuccess) {
         GenOptions o = best_gen;
         o.waves_per_eu = 2;
+        e = consider(0, best_grid, o);
+    }
+    // workspace slots one per 8-byte access instead of paired (or paired, if off)
+    if (e == hipSuccess) {
+        GenOptions o = best_gen;
+        o.pair_slots = resolved(best_gen).pair_slots ? 0u : 1u;
         e = consider(0, best_grid, o);
     }
     // LU entries in LDS: none, and shares for 4, 3 and 2 waves per CU
@@ -2304,6 +2313,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     knob([](GenOptions &o) { o.lds_slots = 0; });
     knob([](GenOptions &o) { o.lds_slots = 80; });
     knob([](GenOptions &o) { o.lds_slots = 128; });
+    knob([](GenOptions &o) { o.pair_slots = resolved(o).pair_slots ? 0u : 1u; });
     // teams of 64, 32, 16 and 8 lanes per cell (the cooperative form)
     const u32 saved_multi_team = g.multi_team;
     u32 best_team = 0;

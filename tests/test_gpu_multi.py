@@ -31,24 +31,30 @@ def _set(gm, blocks):
     return meth, ctl
 
 
-def test_three_mechanisms_one_launch(gm):
-    """Ragged cell counts (not multiples of 64), two operator-split calls."""
-    blocks = _blocks((200, 333, 130))
-    meth, ctl = _set(gm, blocks)
-    work = [z.copy() for _, z in blocks]
-    dzs = [gpu.DeviceZones(gm, z) for z in work]
-    for d in dzs:
-        d.upload()
-    gm.advance_multi(dzs, 0.25)
-    gm.advance_multi(dzs, 0.25)
-    for d in dzs:
-        d.download(mutable_only=False)
-    assert gm.kin_info().kernel == 4
-    for (mech, z), got in zip(blocks, work):
-        want = z.copy()
-        oracle_run(mech, meth, ctl, want, [0.25, 0.25])
-        assert compare_bits(got, want) == ZERO
-        assert want.steps[:want.count].max() >= 5 * want.steps[:want.count].min()
+@pytest.mark.parametrize("pair", [0, 0xFFFFFFFF], ids=["paired-slots", "single-slots"])
+def test_three_mechanisms_one_launch(gm, pair):
+    """Ragged cell counts (not multiples of 64), two operator-split calls;
+    workspace slots paired (the default) or single."""
+    try:
+        gm.set_gen_options(pair_slots=pair)
+        blocks = _blocks((200, 333, 130))
+        meth, ctl = _set(gm, blocks)
+        work = [z.copy() for _, z in blocks]
+        dzs = [gpu.DeviceZones(gm, z) for z in work]
+        for d in dzs:
+            d.upload()
+        gm.advance_multi(dzs, 0.25)
+        gm.advance_multi(dzs, 0.25)
+        for d in dzs:
+            d.download(mutable_only=False)
+        assert gm.kin_info().kernel == 4
+        for (mech, z), got in zip(blocks, work):
+            want = z.copy()
+            oracle_run(mech, meth, ctl, want, [0.25, 0.25])
+            assert compare_bits(got, want) == ZERO
+            assert want.steps[:want.count].max() >= 5 * want.steps[:want.count].min()
+    finally:
+        gm.set_gen_options()
 
 
 def test_multi_with_lds_lu_entries(gm):

@@ -64,9 +64,9 @@ def test_lds_lu_entries_keep_bits(gm, lds):
     lds_slots, at most 320 entries = 160 KiB per wave) change where values
     live, not one operation: the advance stays bit-identical to the oracle,
     with pulses, for the 24/70 mechanism (lds_max: all 216 entries in LDS) and
-    a 48/150 sample."""
+    a 48/150 sample. Single workspace slots, so the slot count below is exact."""
     try:
-        gm.set_gen_options(lds_slots=lds)
+        gm.set_gen_options(lds_slots=lds, pair_slots=0xFFFFFFFF)
         sp, mech, z = make_problem("c3c", 256)
         add_pulses(z, seed=7)
         meth, ctl = problem.method(2), problem.control(rtol=1e-5, atol=1e-10)
@@ -85,6 +85,29 @@ def test_lds_lu_entries_keep_bits(gm, lds):
         b = z.subset(lanes)
         oracle_run(mech, meth, ctl, b, [0.5])
         assert compare_bits(a.subset(lanes), b) == ZERO
+    finally:
+        gm.set_gen_options()
+
+
+@pytest.mark.parametrize("pair", [1, 0xFFFFFFFF], ids=["paired", "single"])
+def test_workspace_slot_pairing_keeps_bits(gm, pair):
+    """FP64 workspace slots in adjacent pairs per lane (the default: LU
+    entries in the triangular solves' read order, two slots per 16-byte
+    access) or one slot per 8-byte access change only the width of the
+    memory accesses: the advance is bit-identical to the oracle, with pulses,
+    for 24/70 (part of its LU in LDS) and 60/200 (rows of odd length, so
+    pairs straddle rows)."""
+    try:
+        gm.set_gen_options(pair_slots=pair)
+        for name in ("c3c", "c3a"):
+            sp, mech, z = make_problem(name, 256)
+            add_pulses(z, seed=3)
+            meth, ctl = problem.method(2), problem.control(rtol=1e-5, atol=1e-10)
+            a, b = z.copy(), z.copy()
+            gpu_run(gm, mech, meth, ctl, a, [0.1, 0.1], kernel=1)
+            oracle_run(mech, meth, ctl, b, [0.1, 0.1])
+            assert gm.kin_info().kernel == 1
+            assert compare_bits(a, b) == ZERO, name
     finally:
         gm.set_gen_options()
 

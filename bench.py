@@ -390,10 +390,12 @@ def cell_errors(got, ref, atol, rtol, n):
 
 def accuracy(got, ref, ctl, n):
     e = cell_errors(got, ref, ctl.atol, ctl.rtol, n)
+    same = same_counts(got, ref, n)
     return {"max_rel_err": float(e.max()), "cell_p99_9": float(np.percentile(e, 99.9)),
             "cell_p99": float(np.percentile(e, 99)), "cells_above_1e-6": float(np.mean(e > 1e-6)),
-            "cells_same_step_counts": float(same_counts(got, ref, n).mean()),
-            "meets_criterion": bool(np.percentile(e, 99.9) <= CRITERION_P999)}
+            "cells_same_step_counts": float(same.mean()),
+            "meets_criterion": bool(np.percentile(e, 99.9) <= CRITERION_P999),
+            "max_rel_err_by_species_group": rel_err_groups(got.y[:, :n], ref.y[:, :n], ctl.atol, ctl.rtol, same)}
 
 
 def host_cpu():

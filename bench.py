@@ -67,10 +67,13 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration, threads=0):
 
 def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
                  kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0, lds_slots=0, perturb=False,
-                 reuse=0):
+                 reuse=0, order=0):
     """Advances `cells` cells of the workload `steps` x 1 s from the seeded initial
     state (after `warmup` untimed seconds on a copy). Returns timings and the block.
-    perturb: every initial concentration moved up by one ulp."""
+    perturb: every initial concentration moved up by one ulp. order: the
+    module's queue policy; 0 (natural order) by default, because the timed
+    run repeats the warm-up's cells from the same state, so costs recorded
+    in the warm-up would be exact foreknowledge no real caller has."""
     z0 = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
     if perturb:
         z0.y[:] = np.nextafter(z0.y, np.inf)
@@ -86,6 +89,7 @@ def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precis
     gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps,
                        lds_slots=lds_slots, reuse=reuse)
     gm.set_grid(grid)
+    gm.set_queue_policy(order)
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z0.copy())
     dz.upload()
@@ -304,7 +308,10 @@ def dbd_block(gpu, problem, zones=65536, steps=5, dt=1e-3):
     pulsed discharge, `steps` master steps of dt with the host preparing each
     step's pulses (not timed). Kernel time per step for FP64 lanes and for the
     reference's default f32 delta lanes (generated kernel, and the table walk
-    for comparison)."""
+    for comparison). Steps after the first take the zones longest first by
+    the previous step's attempts (the module's default queue policy; a
+    continuing simulation, so the previous step is a real estimate); the
+    first step runs in natural order and is reported beside the mean."""
     from simstack_amd import _ffi, dbd
     air = dbd.AirMechanism(caps=_ffi.DEFAULT_CAPS)
     gm = run_workload.gm
@@ -316,18 +323,23 @@ def dbd_block(gpu, problem, zones=65536, steps=5, dt=1e-3):
         z = dbd.reference_zones(air, zones)
         gm.arena_reset()
         gm.set_kernel(kind)
+        gm.set_queue_policy(1)
         gm._check(gm.lib.gpu_kin_set_mech(air.bytes.ctypes.data, C.byref(meth), C.byref(ctl)), "gpu_kin_set_mech")
         dz = gpu.DeviceZones(gm, z)
-        kms = []
+        kms, ordered = [], []
         for step in range(steps):
             z.prepare_source(src, dbd.PULSED, step * dt, (step + 1) * dt)
             dz.upload()
             kms.append(dz.advance(dt, precision=precision))
+            ordered.append(int(gm.kin_info().ordered))
             dz.download(mutable_only=True)
             z.commit_pulses()
         gm.set_kernel(0)
         acc = float(z.steps[:zones].sum())
         out[name] = {"kernel_ms_per_step": float(np.mean(kms)), "cell_steps_per_s": acc / (sum(kms) / 1e3),
+                     "kernel_ms_first_step": float(kms[0]),
+                     "kernel_ms_later_steps": float(np.mean(kms[1:])) if len(kms) > 1 else None,
+                     "steps_cost_ordered": int(sum(ordered)),
                      "failed_zones": int(z.failed[:zones].sum()), "kernel": gm.kin_info().kernel_name.decode()}
     return out
 

@@ -114,6 +114,19 @@ GPU_API s32 simk_kin_set_grid(u32 waves);
  * do not depend on it. The array must stay valid while advances run. */
 GPU_API s32 simk_kin_set_order(const u32 *order_device);
 
+/* Queue order of single-mechanism advances when no order is set:
+ * 1 (default): an advance of the same block as the previous advance (the
+ * same rows, row stride and cell count) with more cells than the device's
+ * resident lanes takes its cells longest first, by the step attempts each
+ * took in that previous advance; lanes take cells in queue order, so the
+ * costliest cells start early instead of stretching the last round.
+ * Consecutive operator-split advances of a cell cost about the same. 0:
+ * natural order. Results per cell are the same either way; the first
+ * advance of a block, the allocation-free advance, cooperative mode and the
+ * multi-mechanism kernel keep the natural order. Setting the policy forgets
+ * the recorded costs. */
+GPU_API s32 simk_kin_set_queue_policy(u32 policy);
+
 /* Kernel used by the FP64 and f32 delta advances: 0 auto (generated when it
  * compiled, else table), 1 generated straight-line, 2 table walk. */
 GPU_API s32 simk_kin_set_kernel(u32 kind);
@@ -194,7 +207,7 @@ typedef struct SimkKinInfo {
                             5 f32 delta (table walk), 6 / 7 one wave per cell (FP64 / f32 delta), 8 generated f32 delta */
     u32 ws_slots;        /* workspace f64 slots per lane */
     u32 waves;           /* persistent waves of the last launch */
-    u32 pad_;
+    u32 ordered;         /* 1: the last advance took its cells longest first (simk_kin_set_queue_policy) */
     u32 species, reactions, nnz, lu_div, lu_sub;
     u32 jit_cached;      /* 1: the generated kernel came from the disk cache */
     f64 jit_ms;          /* generate + compile (or cache load) time */

@@ -314,6 +314,11 @@ struct State {
     u32 grid_override = 0;     // persistent waves per launch (0: resident waves)
     int blocks_per_cu_table = 0, blocks_per_cu_d32 = 0;
     const u32 *order = nullptr;
+    // Queue order by the previous advance's per-cell attempts (policy 1, the
+    // default; 0: natural order), and whether the last advance used it.
+    u32 queue_policy = 1;
+    SimkCostOrder cost_order;
+    bool last_ordered = false;
     // last call
     u32 last_kernel = 0;
     u32 last_slots = 0;
@@ -422,6 +427,7 @@ bool jit_code(const GenResult &gr, std::vector<char> &code, bool &cached, std::s
 }
 
 void free_mech_resources() {
+    simk_cost_order_free(g.cost_order);   // recorded costs belong to the old mechanism's cells
     if (g.d_tables) (void)hipFree(g.d_tables);
     g.d_tables = nullptr;
     if (g.d_wave) (void)hipFree(g.d_wave);
@@ -999,6 +1005,41 @@ hipError_t launch_wave(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     return hipErrorNotSupported;   // mixed precision has no cooperative form
 }
 
+// Queue order of a single-mechanism advance: the caller's (simk_kin_set_order),
+// else, under queue policy 1, the cells by descending attempts in the
+// previous advance of the same block (rows, row stride and cell count). It
+// matters only when there are more cells than resident lanes: with fewer,
+// every cell starts at once. The allocation-free entry point keeps the
+// natural order (ordering allocates its buffers on first use).
+const u32 *order_for(const GpuKinLaunch *L, u32 lanes, u32 waves, bool &by_cost) {
+    by_cost = !g.order && g.queue_policy == 1 && !g.use_scratch && (u64)lanes > (u64)waves * SIMK_WAVE;
+    if (!by_cost) return g.order;
+    SimkCostOrder &c = g.cost_order;
+    const bool same = c.valid && c.y == L->y && c.padded == L->padded && c.lanes == lanes;
+    if (simk_cost_order_begin(c, g.stream, L->steps, L->rejects, lanes) != hipSuccess) {
+        (void)hipGetLastError();
+        c.valid = false;
+        by_cost = false;
+        return nullptr;
+    }
+    g.last_ordered = same;
+    return same ? c.order : nullptr;
+}
+
+// After the advance is enqueued: this advance's per-cell attempts become the
+// order of the next advance of the block.
+hipError_t order_after(const GpuKinLaunch *L, u32 lanes, bool by_cost, hipError_t launched) {
+    if (!by_cost) return launched;
+    SimkCostOrder &c = g.cost_order;
+    hipError_t e = launched;
+    if (e == hipSuccess) e = simk_cost_order_end(c, g.stream, L->steps, L->rejects, lanes);
+    c.valid = e == hipSuccess;
+    c.y = L->y;
+    c.padded = L->padded;
+    c.lanes = lanes;
+    return e;
+}
+
 // f32 delta lanes: the generated kernel (kernel kind 0 when it builds, or 1),
 // else the table walk (kind 2, or 0 when generation fails).
 hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
@@ -1047,7 +1088,8 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
     A.n_reactions = m.R;
     A.repeats = repeats;
     A.queue = queue;
-    A.order = g.order;
+    bool by_cost = false;
+    A.order = order_for(L, lanes, waves, by_cost);
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
     A.method = g.dev_method;
@@ -1062,16 +1104,18 @@ hipError_t launch_delta32(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
         DA.C = C;
         size_t size = sizeof(DA);
         void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &DA, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
-        return hipModuleLaunchKernel(g.jit_d32.fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+        return order_after(L, lanes, by_cost,
+                           hipModuleLaunchKernel(g.jit_d32.fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg));
     }
     hipLaunchKernelGGL(simk_delta32_kernel, dim3(waves), dim3(SIMK_WAVE), 0, g.stream, A, C);
-    return hipGetLastError();
+    return order_after(L, lanes, by_cost, hipGetLastError());
 }
 
 // Launches `repeats` consecutive advances of queue positions [0, lanes) on the
 // module stream with the persistent kernel: grid = resident waves (or fewer
 // when there are fewer cells), lanes refill from the queue.
 hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
+    g.last_ordered = false;
     const u32 lanes = L->lane_count ? L->lane_count : L->padded;
     if (lanes == 0 || repeats == 0) return hipSuccess;
     if (lanes > L->padded) return hipErrorInvalidValue;
@@ -1127,7 +1171,8 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     A.n_reactions = g.mech.R;
     A.repeats = repeats;
     A.queue = queue;
-    A.order = g.order;
+    bool by_cost = false;
+    A.order = order_for(L, lanes, waves, by_cost);
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
     A.method = g.dev_method;
@@ -1138,10 +1183,11 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     if (use_gen) {
         size_t size = sizeof(A);
         void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
-        return hipModuleLaunchKernel(jk->fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+        return order_after(L, lanes, by_cost,
+                           hipModuleLaunchKernel(jk->fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg));
     }
     hipLaunchKernelGGL(simk_table_kernel, dim3(waves), dim3(SIMK_WAVE), 0, g.stream, A);
-    return hipGetLastError();
+    return order_after(L, lanes, by_cost, hipGetLastError());
 }
 
 // Kernel arguments of one advance of a launch's block (shared by the single
@@ -1473,6 +1519,7 @@ GPU_API u64 gpu_arena_used(void) { return g.arena.used; }
 GPU_API void gpu_arena_reset(void) {
     if (g.ready) (void)hipDeviceSynchronize();
     g.arena.used = 0;
+    g.cost_order.valid = false;   // blocks in the arena are gone
 }
 
 GPU_API s32 gpu_copy_to_device(void *dst_device, const void *src_host, u64 size) {
@@ -1612,6 +1659,13 @@ GPU_API s32 simk_kin_advance_static(const GpuKinLaunch *L, u32 repeats, f32 *ker
 GPU_API s32 simk_kin_set_grid(u32 waves) {
     g.grid_override = waves;
     g.user_grid = waves;
+    return 0;
+}
+
+GPU_API s32 simk_kin_set_queue_policy(u32 policy) {
+    if (policy > 1) return err_code(hipErrorInvalidValue);
+    g.queue_policy = policy;
+    g.cost_order.valid = false;
     return 0;
 }
 
@@ -1843,7 +1897,21 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
     return 0;
 }
 
+static s32 autotune_impl(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out);
+
+// The tuner times every candidate on the same copy of the same cells: a
+// queue ordered by an earlier candidate's costs would favour the later ones,
+// so candidates run in natural order.
 GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
+    const u32 saved = g.queue_policy;
+    g.queue_policy = 0;
+    const s32 r = autotune_impl(L, sample_cells, duration, out);
+    g.queue_policy = saved;
+    g.cost_order.valid = false;
+    return r;
+}
+
+static s32 autotune_impl(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
     if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision == 1 || L->precision > 2)
         return err_code(hipErrorInvalidValue);
     if (L->precision == 2) {
@@ -2024,6 +2092,7 @@ GPU_API s32 simk_kin_info(SimkKinInfo *out) {
     out->kernel = g.last_kernel;
     out->ws_slots = g.last_slots;
     out->waves = g.last_waves;
+    out->ordered = g.last_ordered ? 1u : 0u;
     out->species = g.mech.n;
     out->reactions = g.mech.R;
     out->nnz = g.mech.nnz;

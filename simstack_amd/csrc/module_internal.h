@@ -17,6 +17,24 @@ hipError_t simk_enqueue_plugflow(hipStream_t stream, f64 *y, u64 padded, u32 spe
 
 // One request's advection rows for the batched upwind kernel: fields x
 // channels rows of `nodes` values at x + field * padded + channel * nodes.
+// Queue order by the previous advance's per-cell attempts (order.hip): the
+// key (block rows, lane count) it was measured on, and device buffers.
+struct SimkCostOrder {
+    const f64 *y = nullptr;   // block the order belongs to
+    u32 padded = 0, lanes = 0;
+    bool valid = false;
+    u32 cap = 0;              // cells the buffers hold
+    f64 *before = nullptr;    // steps + rejects before the advance
+    u32 *keys = nullptr, *keys_out = nullptr, *cells = nullptr, *order = nullptr;
+    void *temp = nullptr;     // radix-sort scratch
+    size_t temp_bytes = 0;
+};
+// Before an advance of `lanes` cells: sizes the buffers, records steps + rejects.
+hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t stream, const f64 *steps, const f64 *rejects, u32 lanes);
+// After it (same stream): c.order = the cells by descending attempts in that advance.
+hipError_t simk_cost_order_end(SimkCostOrder &c, hipStream_t stream, const f64 *steps, const f64 *rejects, u32 lanes);
+void simk_cost_order_free(SimkCostOrder &c);
+
 struct SimkUpwindSeg {
     double *x;
     unsigned long long padded;

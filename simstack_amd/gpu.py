@@ -22,7 +22,7 @@ class SimkGenOptions(C.Structure):
 
 
 class SimkKinInfo(C.Structure):
-    _fields_ = [("kernel", C.c_uint32), ("ws_slots", C.c_uint32), ("waves", C.c_uint32), ("pad_", C.c_uint32),
+    _fields_ = [("kernel", C.c_uint32), ("ws_slots", C.c_uint32), ("waves", C.c_uint32), ("ordered", C.c_uint32),
                 ("species", C.c_uint32),
                 ("reactions", C.c_uint32), ("nnz", C.c_uint32), ("lu_div", C.c_uint32), ("lu_sub", C.c_uint32),
                 ("jit_cached", C.c_uint32), ("jit_ms", C.c_double), ("last_kernel_ms", C.c_double),
@@ -70,6 +70,7 @@ REFERENCE_EXPORTS = ("gpu_init", "gpu_error_string", "gpu_alloc", "gpu_arena_use
                      "gpu_kin_advance", "gpu_host_register", "gpu_host_unregister", "gpu_kin_advance_async",
                      "gpu_kin_wait", "gpu_kin_set_mode")
 EXTENSION_EXPORTS = ("simk_kin_set_mech_caps", "simk_kin_advance_n", "simk_kin_advance_static", "simk_kin_set_team", "simk_kin_set_grid", "simk_kin_set_order",
+                     "simk_kin_set_queue_policy",
                      "simk_kin_set_kernel", "simk_kin_set_gen_options", "simk_kin_info",
                      "simk_jit_precompile", "simk_copy_device", "simk_copy_device_b64", "simk_upwind_f64",
                      "simk_upwind_f32",
@@ -121,6 +122,7 @@ def load_library(path=_ffi.GPU_LIB):
     _sig(lib, "simk_kin_set_grid", C.c_int32, C.c_uint32)
     _sig(lib, "simk_kin_set_team", C.c_int32, C.c_uint32)
     _sig(lib, "simk_kin_set_order", C.c_int32, V)
+    _sig(lib, "simk_kin_set_queue_policy", C.c_int32, C.c_uint32)
     _sig(lib, "simk_kin_set_mechs", C.c_int32, C.c_uint32, V, V, P(_ffi.KinMethod), P(_ffi.KinControl))
     _sig(lib, "simk_jit_precompile_multi", C.c_int32, C.c_uint32, V, V, P(_ffi.KinMethod), P(SimkGenOptions),
          C.c_char_p, C.c_uint32)
@@ -307,6 +309,11 @@ class GpuModule:
 
     def set_grid(self, waves):
         self._check(self.lib.simk_kin_set_grid(waves), "simk_kin_set_grid")
+
+    def set_queue_policy(self, policy):
+        """1 (default): advances of the same block take cells longest first by
+        the previous advance's attempts; 0: natural order."""
+        self._check(self.lib.simk_kin_set_queue_policy(policy), "simk_kin_set_queue_policy")
 
     def kin_info(self):
         info = SimkKinInfo()

@@ -112,3 +112,38 @@ def test_dbd_many_zones(gm, air):
         rel = np.where(before > 0, np.abs(after - before) / np.maximum(np.abs(after), np.abs(before)), 0.0)
     assert rel[:, :2].max() < 1e-10 and rel[:, 2].max() < 1e-9
     assert z.failed[:2048].sum() == 0
+
+
+def test_f32_delta_lanes_accuracy_bounds(gm, air):
+    """The reference's f32-lane accuracy bounds (test_kinetics.cpp:278-341)
+    on the device: 16 zones at the reference's operating points, 100 ms
+    pulsed and 1,000 ms averaged in 1-ms master steps, f32 delta lanes
+    against FP64 lanes. The worst relative error of the watched products
+    (O3, NO, NO2, N2O5, HNO2, HNO3, H2O2, N2O, where the FP64 run holds at
+    least 1e-3 ppm) stays below 1e-2, the N/O/H atom drift below 3e-7 per
+    simulated second (each master step rounds the increments to f32), and no
+    zone fails."""
+    sp = air.sp
+    watch = [sp.O3, sp.NO, sp.NO2, sp.N2O5, sp.HNO2, sp.HNO3, sp.H2O2, sp.N2O]
+    for mode, ms in ((dbd.PULSED, 100), (dbd.AVERAGED, 1000)):
+        print(f"[case] f32 delta vs FP64 lanes, 16 zones x {ms} ms, mode {mode}", flush=True)
+        z32, z64 = dbd.reference_zones(air, 16), dbd.reference_zones(air, 16)
+        before = np.array([air.count_elements(z32.y[:, i]) for i in range(16)])
+        _gpu_run(gm, air, z32, mode, ms, precision=1)
+        _gpu_run(gm, air, z64, mode, ms, precision=0)
+        worst = 0.0
+        for i in range(16):
+            total = z64.y[:, i].sum()
+            for s in watch:
+                if z64.y[s, i] / total * 1e6 < 1e-3:
+                    continue
+                a, b = z32.y[s, i], z64.y[s, i]
+                worst = max(worst, abs(a - b) / max(abs(a), abs(b)))
+        after = np.array([air.count_elements(z32.y[:, i]) for i in range(16)])
+        with np.errstate(invalid="ignore", divide="ignore"):
+            drift = np.where(before > 0, np.abs(after - before) / np.maximum(np.abs(after), np.abs(before)), 0.0).max()
+        print(f"[f32] worst product error {worst:.2e}, atom drift {drift / (ms * 1e-3):.2e} per s, "
+              f"steps {z32.steps[:16].sum():.0f} vs {z64.steps[:16].sum():.0f}", flush=True)
+        assert worst < 1e-2
+        assert drift / (ms * 1e-3) < 3e-7
+        assert z32.failed[:16].sum() == 0 and z64.failed[:16].sum() == 0

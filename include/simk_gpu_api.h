@@ -316,7 +316,9 @@ typedef struct SimkServeRequest {
     const f64 *inflow;   /* [species] concentrations entering node 0 (DBD: the feed gas of the chain), or null */
     f64 *result;         /* host [species + 5][cells]: y, h, steps, rejects, pulses, failed after the last step */
     u32 steps;           /* operator-split steps of this request (0: the server's) */
-    u32 pad_;
+    u32 precision;       /* 0: FP64 lanes (time-sliced with every other request); 1: f32 delta lanes (the
+                            reference's default chemistry precision), each step run whole in a launch of the
+                            mechanism's f32 delta kernel beside the FP64 one */
     const struct SimkServeDbd *dbd;   /* DBD requests: the zones' discharge description (read by simk_serve_submit only) */
 } SimkServeRequest;
 

@@ -290,6 +290,10 @@ struct State {
     // tables of each mechanism for the cooperative (team) form of the
     // multi-mechanism launch, and its team size (0: one lane per cell)
     void *multi_dtables[4] = {}, *multi_dwave[4] = {};
+    // generated f32 delta kernels of the ragged list's mechanisms (serving
+    // requests in f32 delta lanes), compiled on first use
+    JitKernel multi_d32[4];
+    bool multi_d32_tried[4] = {};
     u32 multi_team = 0, user_multi_team = 0;
     // captured step sequences
     // Captured multi-mechanism steps. Each graph owns its workspace and queue
@@ -725,9 +729,9 @@ GenOptions resolved(GenOptions o) {
     return o;
 }
 
-bool load_generated(const GenOptions &opt, JitKernel &out) {
+bool load_generated_mech(const Mech &m, const GenOptions &opt, JitKernel &out) {
     auto t0 = std::chrono::steady_clock::now();
-    GenResult gr = generate(g.mech, g.method, resolved(opt), kDeviceKinSource);
+    GenResult gr = generate(m, g.method, resolved(opt), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false, ok = false;
     if (jit_code(gr, code, cached, g.jit_log)) {
@@ -753,6 +757,8 @@ bool load_generated(const GenOptions &opt, JitKernel &out) {
     out.ms = std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return ok;
 }
+
+bool load_generated(const GenOptions &opt, JitKernel &out) { return load_generated_mech(g.mech, opt, out); }
 
 void unload_mixed() {
     if (g.jit_mixed.mod) (void)hipModuleUnload(g.jit_mixed.mod);
@@ -1444,6 +1450,40 @@ hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *
     }
     multi_schedule(P.MA, count);
     return enqueue_multi(P, false);   // time slices and carry lists exist in the one-lane-per-cell kernel only
+}
+
+hipError_t simk_enqueue_d32_ext(u32 index, const GpuKinLaunch *L, u32 lanes, const u32 *order) {
+    if (!g.multi_ok || index >= g.multi_mechs.size() || !launch_ok(L) || lanes == 0 || lanes > L->padded || !order)
+        return hipErrorInvalidValue;
+    const Mech &m = g.multi_mechs[index];
+    if (!g.multi_d32_tried[index]) {
+        g.multi_d32_tried[index] = true;
+        GenOptions o = g.gen;
+        o.precision = 1;
+        if (!load_generated_mech(m, o, g.multi_d32[index])) g.multi_d32[index] = JitKernel();
+    }
+    const JitKernel &jk = g.multi_d32[index];
+    if (!jk.mod) return hipErrorInvalidImage;
+    const u64 rows = m.R > m.n ? m.R : m.n;
+    const u64 wave_bytes = (u64)jk.ws_slots * SIMK_WAVE * 8;
+    if (rows * L->padded * 8ull >= (1ull << 31) || wave_bytes >= (1ull << 31)) return hipErrorInvalidValue;
+    const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
+    const u32 resident = resident_waves(jk.blocks_per_cu, wave_bytes, 0);
+    u32 waves = needed < resident ? needed : resident;
+    double *ws = nullptr;
+    u32 *queue = nullptr;
+    hipError_t e = launch_buffers(wave_bytes, waves, ws, queue);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(simk_queue_reset_kernel, dim3(1), dim3(64), 0, g.stream, queue, waves * SIMK_WAVE);
+    SimkD32Args DA;
+    DA.A = make_args(L, lanes, m, g.multi_ctl[index], jk.ws_slots, 1, queue, 0);
+    DA.A.ws = ws;
+    DA.A.order = order;
+    DA.A.tables = (const SimkDevTables *)g.multi_dtables[index];
+    DA.C = t_consts_ctl<f32>(g.multi_ctl[index]);
+    size_t size = sizeof(DA);
+    void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &DA, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+    return hipModuleLaunchKernel(jk.fn, waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
 }
 
 // ---------------------------------------------------------------------------
@@ -2228,6 +2268,9 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
         if (g.multi_dtables[i]) (void)hipFree(g.multi_dtables[i]);
         if (g.multi_dwave[i]) (void)hipFree(g.multi_dwave[i]);
         g.multi_dtables[i] = g.multi_dwave[i] = nullptr;
+        if (g.multi_d32[i].mod) (void)hipModuleUnload(g.multi_d32[i].mod);
+        g.multi_d32[i] = JitKernel();
+        g.multi_d32_tried[i] = false;
     }
     g.multi_mechs = ms;
     g.method = *method;

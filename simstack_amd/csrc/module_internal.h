@@ -72,6 +72,12 @@ struct SimkSlice {
 // 8 u32 (queue heads [0, 4), carry counts [4, 8)), both owned by the caller.
 // After the launch queue[i] (clamped to lanes[i]) is the number of queue
 // positions taken and queue[4 + i] the number of carried cells.
+// Serving: one f32 delta-lane advance of L->duration for queue positions
+// [0, lanes) of `order` (lanes of mechanism `index` of the simk_kin_set_mechs
+// list), run to completion with that mechanism's generated f32 delta kernel
+// (compiled on first use), on the module stream.
+hipError_t simk_enqueue_d32_ext(u32 index, const GpuKinLaunch *L, u32 lanes, const u32 *order);
+
 hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *lanes, const u32 *const *orders,
                                   u32 repeats, double *ws, u32 *queue, u32 waves, const SimkSlice *slice);
 

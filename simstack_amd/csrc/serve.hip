@@ -54,6 +54,8 @@ struct Slab {
     u32 *d_order = nullptr;  // queue position -> lane
     u32 *h_order = nullptr;  // pinned staging
     u32 lanes = 0;           // current order length
+    u32 *d_order32 = nullptr, *h_order32 = nullptr;   // the lanes of f32 delta requests
+    u32 lanes32 = 0;
     u32 *d_carry = nullptr;  // cells stopped by the slice end
     u32 *h_carry = nullptr;
     double *courant() const { return d + zone_rows * cap; }
@@ -211,6 +213,8 @@ void release_all() {
         if (sl.d) (void)hipFree(sl.d);
         if (sl.d_order) (void)hipFree(sl.d_order);
         if (sl.h_order) (void)hipHostFree(sl.h_order);
+        if (sl.d_order32) (void)hipFree(sl.d_order32);
+        if (sl.h_order32) (void)hipHostFree(sl.h_order32);
         if (sl.d_carry) (void)hipFree(sl.d_carry);
         if (sl.h_carry) (void)hipHostFree(sl.h_carry);
     }
@@ -350,11 +354,13 @@ hipError_t build_orders(hipStream_t st, u64 &waves) {
     std::sort(order.begin(), order.end(), [](const Request *a, const Request *b) {
         return a->r.cells != b->r.cells ? a->r.cells < b->r.cells : a->ticket < b->ticket;
     });
-    for (Slab &sl : S.slabs) sl.lanes = 0;
+    for (Slab &sl : S.slabs) sl.lanes = sl.lanes32 = 0;
     for (const Request *q : order) {
         Slab &sl = S.slabs[q->r.mech];
-        memcpy(sl.h_order + sl.lanes, q->remaining.data(), q->remaining.size() * 4);
-        sl.lanes += (u32)q->remaining.size();
+        // f32 delta requests run in a launch of their own (whole steps, no slices)
+        u32 *to = q->r.precision == 1 ? sl.h_order32 + sl.lanes32 : sl.h_order + sl.lanes;
+        memcpy(to, q->remaining.data(), q->remaining.size() * 4);
+        (q->r.precision == 1 ? sl.lanes32 : sl.lanes) += (u32)q->remaining.size();
     }
     waves = 0;
     hipError_t e = hipSuccess;
@@ -362,6 +368,8 @@ hipError_t build_orders(hipStream_t st, u64 &waves) {
         waves += (sl.lanes + 63) / 64;
         if (e == hipSuccess && sl.lanes)
             e = hipMemcpyAsync(sl.d_order, sl.h_order, (u64)sl.lanes * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && sl.lanes32)
+            e = hipMemcpyAsync(sl.d_order32, sl.h_order32, (u64)sl.lanes32 * 4, hipMemcpyHostToDevice, st);
     }
     return e;
 }
@@ -446,6 +454,8 @@ GPU_API s32 simk_serve_open(const SimkServeConfig *config) {
         if (e == hipSuccess) e = hipMemsetAsync(sl.resume(), 0, kResumeRows * cap * 8, st);
         if (e == hipSuccess) e = hipMalloc(&sl.d_order, cap * 4);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_order, cap * 4, 0);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_order32, cap * 4);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.h_order32, cap * 4, 0);
         if (e == hipSuccess) e = hipMalloc(&sl.d_carry, carry_cap * 4);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_carry, carry_cap * 4, 0);
     }
@@ -477,7 +487,7 @@ GPU_API s32 simk_serve_submit(const SimkServeRequest *request, u32 *ticket) {
     if (!request || !ticket) return code(hipErrorInvalidValue);
     const SimkServeRequest &r = *request;
     if (r.mech >= S.shape.count || r.cells == 0 || r.padded < r.cells || !r.block || !r.result ||
-        r.kind > SimkServe_Dbd)
+        r.kind > SimkServe_Dbd || r.precision > 1)
         return code(hipErrorInvalidValue);
     if (r.kind == SimkServe_Transport && (r.channels == 0 || r.nodes == 0 || r.nodes > kMaxNodes ||
                                           (u64)r.channels * r.nodes != r.cells || !r.courant))
@@ -553,7 +563,9 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
     u64 waves = 0;
     e = build_orders(st, waves);
     if (e != hipSuccess) return code(e);
-    const bool ran = waves > 0;
+    bool any32 = false;
+    for (const Slab &sl : S.slabs) any32 = any32 || sl.lanes32 > 0;
+    const bool ran = waves > 0 || any32;
     std::vector<u32> step_done, finished;
     if (ran) {
         GpuKinLaunch Ls[4];
@@ -587,7 +599,11 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
             slice.carry[i] = sl.d_carry;
         }
         if (waves > S.shape.resident_waves) waves = S.shape.resident_waves;
-        e = simk_enqueue_multi_ext(S.shape.count, Ls, lanes, orders, 1, S.d_ws, S.d_queue, (u32)waves, &slice);
+        if (waves)   // without FP64 lanes nothing is taken or carried (collect reads no position)
+            e = simk_enqueue_multi_ext(S.shape.count, Ls, lanes, orders, 1, S.d_ws, S.d_queue, (u32)waves, &slice);
+        // f32 delta requests: each slab's lanes advance one whole step
+        for (u32 i = 0; i < S.shape.count && e == hipSuccess; ++i)
+            if (S.slabs[i].lanes32) e = simk_enqueue_d32_ext(i, &Ls[i], S.slabs[i].lanes32, S.slabs[i].d_order32);
         if (e == hipSuccess) e = hipEventRecord(S.ev[3], st);
         // Uploads of requests that fit now overlap the launch: their lanes
         // are outside every order array of this launch, and the next
@@ -600,7 +616,7 @@ GPU_API s32 simk_serve_step(u32 *done, u32 max_done, u32 *n_done) {
                                    hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return code(e);
-        if (!slice.ticks)
+        if (!slice.ticks || !waves)
             for (u32 i = 0; i < S.shape.count; ++i) S.h_queue[4 + i] = 0;
         collect(step_done);
         // Requests whose last step just ended: copy their solver rows back.

@@ -47,7 +47,7 @@ class SimkServeConfig(C.Structure):
 class SimkServeRequest(C.Structure):
     _fields_ = [("mech", C.c_uint32), ("cells", C.c_uint32), ("channels", C.c_uint32), ("nodes", C.c_uint32),
                 ("padded", C.c_uint32), ("kind", C.c_uint32), ("block", C.c_void_p), ("courant", C.c_void_p),
-                ("inflow", C.c_void_p), ("result", C.c_void_p), ("steps", C.c_uint32), ("pad_", C.c_uint32),
+                ("inflow", C.c_void_p), ("result", C.c_void_p), ("steps", C.c_uint32), ("precision", C.c_uint32),
                 ("dbd", C.c_void_p)]
 
 
@@ -243,25 +243,28 @@ class GpuModule:
         self._serve_keep = {}
 
     def serve_submit(self, mech_index, block_ptr, padded, cells, channels, nodes, courant, result, inflow=None,
-                     keep=None):
+                     keep=None, precision=0):
         """Queues a request whose input zone block starts at host address
-        block_ptr (row stride padded); result: float64 [species + 5, cells]."""
+        block_ptr (row stride padded); result: float64 [species + 5, cells].
+        precision 1: f32 delta lanes."""
         courant = np.ascontiguousarray(courant, dtype=np.float64)
         inflow = None if inflow is None else np.ascontiguousarray(inflow, dtype=np.float64)
         assert result.dtype == np.float64 and result.flags.c_contiguous and result.shape[1] == cells
         req = SimkServeRequest(mech_index, cells, channels, nodes, padded, 0, block_ptr, courant.ctypes.data,
-                               None if inflow is None else inflow.ctypes.data, result.ctypes.data, 0, 0, None)
+                               None if inflow is None else inflow.ctypes.data, result.ctypes.data, 0, precision, None)
         t = C.c_uint32()
         self._check(self.lib.simk_serve_submit(C.byref(req), C.byref(t)), "simk_serve_submit")
         self._serve_keep[t.value] = (courant, inflow, result, keep)
         return t.value
 
-    def serve_submit_dbd(self, mech_index, zones, steps, residence=2.0, mode=0, source=None, feed=None, species=None):
+    def serve_submit_dbd(self, mech_index, zones, steps, residence=2.0, mode=0, source=None, feed=None, species=None,
+                         precision=0):
         """Queues a DBD reactor request: a dbd.DbdZones block (its operating
         point and pulse clock), `steps` master steps, plug flow over
         `residence` seconds toward `feed` (0: independent zones). `species`:
-        the air species map (default: the block's mechanism's). Returns
-        (ticket, result array [species + 5, zones])."""
+        the air species map (default: the block's mechanism's); precision 1:
+        f32 delta lanes (the reference scene's default). Returns (ticket,
+        result array [species + 5, zones])."""
         from . import dbd as _dbd
         result = np.zeros((zones.n + 5, zones.count))
         arrays = [np.ascontiguousarray(getattr(zones, a)[:zones.count], dtype=np.float64)
@@ -270,7 +273,7 @@ class GpuModule:
                          *[a.ctypes.data for a in arrays])
         feed = None if feed is None else np.ascontiguousarray(feed, dtype=np.float64)
         req = SimkServeRequest(mech_index, zones.count, 1, zones.count, zones.padded, 1, zones.block.ctypes.data, None,
-                               None if feed is None else feed.ctypes.data, result.ctypes.data, steps, 0,
+                               None if feed is None else feed.ctypes.data, result.ctypes.data, steps, precision,
                                C.addressof(d))
         t = C.c_uint32()
         self._check(self.lib.simk_serve_submit(C.byref(req), C.byref(t)), "simk_serve_submit")
@@ -280,11 +283,11 @@ class GpuModule:
     def set_mech_control(self, index, ctl):
         self._check(self.lib.simk_kin_set_mech_control(index, C.byref(ctl)), "simk_kin_set_mech_control")
 
-    def serve_submit_zones(self, mech_index, zones, channels, nodes, courant, inflow=None):
+    def serve_submit_zones(self, mech_index, zones, channels, nodes, courant, inflow=None, precision=0):
         """serve_submit of a whole problem.Zones block; returns (ticket, result array)."""
         result = np.zeros((zones.n + 5, zones.count))
         t = self.serve_submit(mech_index, zones.block.ctypes.data, zones.padded, zones.count, channels, nodes,
-                              courant, result, inflow, keep=zones)
+                              courant, result, inflow, keep=zones, precision=precision)
         return t, result
 
     def serve_step(self, max_done=4096):

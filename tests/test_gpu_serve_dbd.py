@@ -15,6 +15,7 @@ from helpers import make_problem
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 
 HASH_F64 = 0x9BEB634425FAF2C2
+HASH_F32 = 0x32256818F6F35BC9
 
 
 def _open(gm, air, c3c, slice_us):
@@ -147,3 +148,68 @@ def test_served_dbd_rejects_bad_requests(gm):
         assert gm.serve_stats().submitted == 0
     finally:
         gm.serve_close()
+
+
+def test_served_dbd_reactor_f32_lanes_reproduces_the_f32_hash(gm):
+    """The same reactor as a request in f32 delta lanes (each step run whole
+    by the air mechanism's f32 delta kernel beside the FP64 launch): the
+    reference's f32 hash, alone and beside time-sliced FP64 transport
+    requests."""
+    print("[case] served DBD reactor in f32 delta lanes, 64 zones x 500 steps", flush=True)
+    air = dbd.AirMechanism(caps=_ffi.DEFAULT_CAPS)
+    sp, c3c, _ = make_problem("c3c", 64)
+    results = []
+    for mix, slice_us in ((False, 0), (True, 300)):
+        sc = dbd.Scene(air)
+        gm.arena_reset()
+        _open(gm, air, c3c, slice_us)
+        try:
+            t, res = gm.serve_submit_dbd(1, sc.z, steps=500, residence=sc.residence, mode=dbd.PULSED,
+                                         feed=sc.feed(), precision=1)
+            tickets = [t]
+            if mix:
+                for k, cells in enumerate((300, 4096, 1000)):
+                    _, _, z = make_problem("c3c", cells, first_cell=100 * k)
+                    tk, _ = gm.serve_submit_zones(0, z, 1, cells, serving.courant_profile(1, cells),
+                                                  inflow=z.y[:, 0].copy())
+                    tickets.append(tk)
+            _drain(gm, tickets)
+            carried = gm.serve_stats().carried
+        finally:
+            gm.serve_close()
+        h = dbd.fnv1a64(np.ascontiguousarray(res[:air.n, :64]).tobytes())
+        print(f"[serve f32] mix={mix} slice_us={slice_us}: state hash {h:016x}, cells carried {carried}", flush=True)
+        assert h == HASH_F32
+        assert res[air.n + 4].sum() == 0
+        results.append(res.copy())
+    assert results[0].tobytes() == results[1].tobytes()
+
+
+def test_served_f32_transport_request_matches_the_oracle(gm):
+    """A transport request in f32 delta lanes equals the oracle's operator
+    split with its f32 delta advance: 20 x (upwind advection, 4 substeps,
+    toward the inflow; then the chemistry of 1 ms), on all six solver rows."""
+    print("[case] served transport request in f32 delta lanes vs the oracle", flush=True)
+    import oracle_ffi
+    sp, c3c, z = make_problem("c3c", 2048)
+    air = dbd.AirMechanism(caps=_ffi.DEFAULT_CAPS)
+    courant = serving.courant_profile(1, 2048)
+    inflow = z.y[:, 0].copy()
+    gm.arena_reset()
+    _open(gm, air, c3c, 0)
+    try:
+        t, res = gm.serve_submit_zones(0, z.copy(), 1, 2048, courant, inflow=inflow, precision=1)
+        _drain(gm, [t])
+    finally:
+        gm.serve_close()
+    w = z.copy()
+    lib = oracle_ffi.oracle()
+    for _ in range(20):
+        sub = np.ascontiguousarray(w.y[:, :2048])
+        lib.oracle_upwind_f64(sub.ctypes.data, courant.ctypes.data, 2048, c3c.n, 2048, 4, inflow.ctypes.data)
+        w.y[:, :2048] = sub
+        oracle_ffi.oracle_advance_delta32(c3c, problem.method(2), problem.control(), w, 1e-3,
+                                          lanes=np.arange(2048), threads=8)
+    want = np.vstack([w.y[:, :2048], w.h[None, :2048], w.steps[None, :2048], w.rejects[None, :2048],
+                      w.pulses[None, :2048], w.failed[None, :2048]])
+    assert res.tobytes() == np.ascontiguousarray(want).tobytes()

@@ -116,15 +116,15 @@ GPU_API s32 simk_kin_set_order(const u32 *order_device);
 
 /* Queue order of single-mechanism advances when no order is set:
  * 1 (default): an advance of the same block as the previous advance (the
- * same rows, row stride and cell count) with more cells than the device's
- * resident lanes takes its cells longest first, by the step attempts each
- * took in that previous advance; lanes take cells in queue order, so the
- * costliest cells start early instead of stretching the last round.
- * Consecutive operator-split advances of a cell cost about the same. 0:
- * natural order. Results per cell are the same either way; the first
- * advance of a block, the allocation-free advance, cooperative mode and the
- * multi-mechanism kernel keep the natural order. Setting the policy forgets
- * the recorded costs. */
+ * same rows, row stride and cell count) of at least 512 cells takes its
+ * cells longest first, by the step attempts each took in that previous
+ * advance; lanes take cells in queue order, so the costliest cells start
+ * early instead of stretching the last round, and cells of like cost share
+ * a wave. Consecutive operator-split advances of a cell cost about the
+ * same. 0: natural order. Results per cell are the same either way; the
+ * first advance of a block, the allocation-free advance, cooperative mode
+ * and the multi-mechanism kernel keep the natural order. Setting the policy
+ * forgets the recorded costs. */
 GPU_API s32 simk_kin_set_queue_policy(u32 policy);
 
 /* Kernel used by the FP64 and f32 delta advances: 0 auto (generated when it

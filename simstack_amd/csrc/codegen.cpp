@@ -62,7 +62,7 @@ struct Op {
 
 // Paired workspace: an op loading slot s and one of the next `lookahead` ops
 // loading its partner s ^ 1 (no barrier between) share one 16-byte load.
-void pair_loads(std::vector<Op> &ops, size_t lookahead = 4) {
+void pair_loads(std::vector<Op> &ops, size_t lookahead = 4, const char *pair_type = "SimkD2") {
     for (size_t i = 0; i < ops.size(); ++i) {
         Op &a = ops[i];
         if (a.slot < 0 || a.from >= 0 || !a.type.empty() || a.barrier) continue;
@@ -73,8 +73,9 @@ void pair_loads(std::vector<Op> &ops, size_t lookahead = 4) {
             char buf[32];
             snprintf(buf, sizeof(buf), "W2(%d)", a.slot >> 1);
             a.load = buf;
-            a.type = "SimkD2";
+            a.type = pair_type;
             a.part = (a.slot & 1) ? ".y" : ".x";
+            a.immutable = false;   // a pair load is never shared by the reuse window
             b.load.clear();
             b.from = (int)i;
             b.part = (b.slot & 1) ? ".y" : ".x";
@@ -783,45 +784,102 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const u32 n = m.n;
     const u32 lds = lds_entries(m, o);
     const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;
-    const u32 koff = m.nnz - lds, kcp = koff + stored_stages * n, scp = kcp + m.R, pcp = scp + n;
-    const u32 slots32 = pcp + n;
+    // Paired f32 slots ([pair][64 lanes][2], two per 8-byte access): LU
+    // entries in the solves' read order as in the FP64 layout, then the stage
+    // vectors at an even stride np, the k, src and pulse copies, each starting
+    // on a pair.
+    const bool pairs = o.pair_slots != 0;
+    std::vector<u32> lu_slot(m.nnz, 0);
+    for (u32 idx = lds; idx < m.nnz; ++idx) lu_slot[idx] = idx - lds;
+    u32 np = n, koff = m.nnz - lds;
+    if (pairs) {
+        u32 t = 0;
+        for (u32 i = 0; i < n; ++i)
+            for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e)
+                if (m.lower[e].index >= lds) lu_slot[m.lower[e].index] = t++;
+        t += t & 1u;
+        for (u32 ii = n; ii-- > 0;) {
+            for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e)
+                if (m.upper[e].index >= lds) lu_slot[m.upper[e].index] = t++;
+            if (m.diag[ii] >= lds) lu_slot[m.diag[ii]] = t++;
+        }
+        koff = t + (t & 1u);
+        np = n + (n & 1u);
+    }
+    const u32 kcp = koff + stored_stages * np, scp = kcp + (pairs ? m.R + (m.R & 1u) : m.R), pcp = scp + np;
+    const u32 slots32 = pcp + np;
     w("struct %s {\n", sname);
-    w("    static constexpr simk_u32 N = %u, R = %u, STAGES = %u, KOFF = %u;\n", n, m.R, me.stages, koff);
-    w("    SimkRows kr, sr, pr;\n    SimkRows32 ws;\n");
+    w("    static constexpr simk_u32 N = %u, R = %u, STAGES = %u, KOFF = %u, NP = %u;\n", n, m.R, me.stages, koff, np);
+    w(pairs ? "    SimkRows kr, sr, pr;\n    SimkRowPairs32 wp;\n" : "    SimkRows kr, sr, pr;\n    SimkRows32 ws;\n");
     if (lds) {
         w("    simk_u32 ln;\n");
         w("    SIMK_DEV float L(simk_u32 i) const { return simk_lds32[i * SIMK_WAVE + ln]; }\n");
         w("    SIMK_DEV void LS(simk_u32 i, float v) const { simk_lds32[i * SIMK_WAVE + ln] = v; }\n");
     }
-    w("    SIMK_DEV float W32(simk_u32 i) const { return ws.ld(i); }\n");
-    w("    SIMK_DEV void W32S(simk_u32 i, float v) const { ws.st(i, v); }\n");
+    if (pairs) {
+        w("    SIMK_DEV float W32(simk_u32 i) const { return wp.ld(i); }\n");
+        w("    SIMK_DEV void W32S(simk_u32 i, float v) const { wp.st(i, v); }\n");
+        w("    SIMK_DEV SimkF2 W2(simk_u32 pair) const { return wp.ld2(pair); }\n");
+        w("    SIMK_DEV void WS2(simk_u32 pair, float a, float b) const { wp.st2(pair, a, b); }\n");
+    } else {
+        w("    SIMK_DEV float W32(simk_u32 i) const { return ws.ld(i); }\n");
+        w("    SIMK_DEV void W32S(simk_u32 i, float v) const { ws.st(i, v); }\n");
+    }
     w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
     w("        const simk_u32 stride = A.padded * 8u;\n");
     w("        kr.init(A.k, 0, stride, R * stride);\n");
     w("        sr.init(A.src, 0, stride, N * stride);\n");
     w("        pr.init(A.pulse, 0, stride, N * stride);\n");
-    w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, A.ws_slots * SIMK_WAVE * 8u);\n");
+    if (pairs)
+        w("        wp.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane, A.ws_slots * SIMK_WAVE * 8u);\n");
+    else
+        w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, A.ws_slots * SIMK_WAVE * 8u);\n");
     if (lds) w("        ln = lane;\n");
     w("    }\n");
     // lane_load_f64 (base_lane.h:101): inputs rounded to f32 once, when the lane takes the cell
     w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
     w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
-    w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) W32S(%u + r, (float)kr.ld(r));\n", kcp);
-    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) W32S(%u + s, (float)sr.ld(s));\n", scp);
-    w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) W32S(%u + s, (float)pr.ld(s));\n", pcp);
+    if (pairs) {
+        // copies in pairs (every copy starts on a pair)
+        w("#pragma unroll 8\n        for (simk_u32 r = 0; r + 1 < R; r += 2) WS2((%u + r) / 2, (float)kr.ld(r), (float)kr.ld(r + 1));\n", kcp);
+        w("        if (R & 1u) W32S(%u + R - 1, (float)kr.ld(R - 1));\n", kcp);
+        w("#pragma unroll 8\n        for (simk_u32 s = 0; s + 1 < N; s += 2) WS2((%u + s) / 2, (float)sr.ld(s), (float)sr.ld(s + 1));\n", scp);
+        w("        if (N & 1u) W32S(%u + N - 1, (float)sr.ld(N - 1));\n", scp);
+        w("#pragma unroll 8\n        for (simk_u32 s = 0; s + 1 < N; s += 2) WS2((%u + s) / 2, (float)pr.ld(s), (float)pr.ld(s + 1));\n", pcp);
+        w("        if (N & 1u) W32S(%u + N - 1, (float)pr.ld(N - 1));\n", pcp);
+    } else {
+        w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) W32S(%u + r, (float)kr.ld(r));\n", kcp);
+        w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) W32S(%u + s, (float)sr.ld(s));\n", scp);
+        w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) W32S(%u + s, (float)pr.ld(s));\n", pcp);
+    }
     w("    }\n");
     w("    SIMK_DEV float K(simk_u32 r) const { return W32(%u + r); }\n", kcp);
     w("    SIMK_DEV float S(simk_u32 s) const { return W32(%u + s); }\n", scp);
     w("    SIMK_DEV float pulse(simk_u32 s) const { return W32(%u + s); }\n", pcp);
     w("    SIMK_DEV void kstore(simk_u32 st, const float (&x)[N]) const {\n");
-    w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) W32S(KOFF + st * N + s, x[s]);\n    }\n");
+    if (pairs) {
+        w("#pragma unroll\n        for (simk_u32 s = 0; s + 1 < N; s += 2) WS2((KOFF + st * NP + s) / 2, x[s], x[s + 1]);\n");
+        w("        if (N & 1u) W32S(KOFF + st * NP + N - 1, x[N - 1]);\n    }\n");
+    } else {
+        w("#pragma unroll\n        for (simk_u32 s = 0; s < N; ++s) W32S(KOFF + st * N + s, x[s]);\n    }\n");
+    }
     {
         std::vector<Op> ops;
         char buf[96];
         for (u32 s2 = 0; s2 < n; ++s2) {
             Op op;
-            snprintf(buf, sizeof(buf), "W32(KOFF + j * N + %u)", s2);
-            op.load = buf;
+            if (pairs && s2 % 2 == 1) {
+                op.from = (int)ops.size() - 1;   // the odd slot of the pair loaded for s2 - 1
+                op.part = ".y";
+            } else if (pairs && s2 + 1 < n) {
+                snprintf(buf, sizeof(buf), "W2((KOFF + j * NP + %u) / 2)", s2);
+                op.load = buf;
+                op.type = "SimkF2";
+                op.part = ".x";
+            } else {
+                snprintf(buf, sizeof(buf), pairs ? "W32(KOFF + j * NP + %u)" : "W32(KOFF + j * N + %u)", s2);
+                op.load = buf;
+            }
             snprintf(buf, sizeof(buf), "v[%u] = __builtin_fmaf(c, %%s, v[%u]);", s2, s2);
             op.stmt = buf;
             ops.push_back(op);
@@ -839,6 +897,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             Op op;
             snprintf(buf, sizeof(buf), "S(%u)", s2);
             op.load = buf;
+            if (pairs) op.slot = (int)(scp + s2);
             snprintf(buf, sizeof(buf), "f[%u] = %%s;", s2);
             op.stmt = buf;
             ops.push_back(op);
@@ -849,6 +908,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             snprintf(buf, sizeof(buf), "K(%u)", r);
             op.load = buf;
             op.immutable = true;
+            if (pairs) op.slot = (int)(kcp + r);   // fun reads the rate constants in order: pairs
             std::string st = "{ const float rate = %s";
             for (u32 p = 0; p < x.reactant_count; ++p) {
                 snprintf(buf, sizeof(buf), " * pt[%u]", x.reactant[p]);
@@ -864,6 +924,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             op.stmt = st;
             ops.push_back(op);
         }
+        if (pairs) pair_loads(ops, 4, "SimkF2");
         w("    SIMK_DEV void fun(const float (&pt)[N], float (&f)[N]) const {\n");
         u32 uid = 0;
         emit_ops(w, ops, o.batch, "        ", uid, o.reuse, "float");
@@ -882,7 +943,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     auto near = [&](u32 idx, u32 i) { return window == 0 || i - row_of[idx] <= window; };
     auto lu_ld = [&](u32 idx) {
         char buf[24];
-        snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : "W32(%u)", idx < lds ? idx : idx - lds);
+        snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : "W32(%u)", idx < lds ? idx : lu_slot[idx]);
         return std::string(buf);
     };
     {
@@ -954,6 +1015,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 Op op;
                 if (!near(dv.kk, i)) {
                     op.load = lu_ld(dv.kk);
+                    if (pairs && dv.kk >= lds) op.slot = (int)lu_slot[dv.kk];
                     snprintf(buf, sizeof(buf), "const float l%u = a%u * %%s; a%u = l%u;", d, dv.ik, dv.ik, d);
                 } else {
                     snprintf(buf, sizeof(buf), "const float l%u = a%u * a%u; a%u = l%u;", d, dv.ik, dv.kk, dv.ik, d);
@@ -967,6 +1029,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                         snprintf(buf, sizeof(buf), "a%u = __builtin_fmaf(-l%u, a%u, a%u);", sb.ij, d, sb.kj, sb.ij);
                     } else {
                         o2.load = lu_ld(sb.kj);
+                        if (pairs && sb.kj >= lds) o2.slot = (int)lu_slot[sb.kj];
                         snprintf(buf, sizeof(buf), "a%u = __builtin_fmaf(-l%u, %%s, a%u);", sb.ij, d, sb.ij);
                     }
                     o2.stmt = buf;
@@ -975,13 +1038,29 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             }
             snprintf(buf, sizeof(buf), "a%u = 1.0f / (a%u + ghinv);", m.diag[i], m.diag[i]);
             stmt(buf);
+            std::vector<std::pair<u32, u32>> ws_row;   // (slot, compact index)
             for (u32 idx : row) {
-                if (idx < lds) snprintf(buf, sizeof(buf), "LS(%u, a%u);", idx, idx);
-                else snprintf(buf, sizeof(buf), "W32S(%u, a%u);", idx - lds, idx);
+                if (idx < lds) {
+                    snprintf(buf, sizeof(buf), "LS(%u, a%u);", idx, idx);
+                    stmt(buf);
+                } else {
+                    ws_row.push_back({lu_slot[idx], idx});
+                }
+            }
+            if (pairs) std::sort(ws_row.begin(), ws_row.end());
+            for (size_t q = 0; q < ws_row.size(); ++q) {
+                const u32 sl = ws_row[q].first;
+                if (pairs && sl % 2 == 0 && q + 1 < ws_row.size() && ws_row[q + 1].first == sl + 1) {
+                    snprintf(buf, sizeof(buf), "WS2(%u, a%u, a%u);", sl / 2, ws_row[q].second, ws_row[q + 1].second);
+                    ++q;
+                } else {
+                    snprintf(buf, sizeof(buf), "W32S(%u, a%u);", sl, ws_row[q].second);
+                }
                 stmt(buf);
             }
         }
         barrier();
+        if (pairs) pair_loads(ops, 4, "SimkF2");
         w("    SIMK_DEV void jac_factor(const float (&pt)[N], const float ghinv) const {\n");
         u32 uid = 0;
         emit_ops(w, ops, o.batch, "        ", uid, o.reuse, "float");
@@ -995,6 +1074,7 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
                 Op op;
                 op.load = lu_ld(m.lower[e].index);
+                if (pairs && m.lower[e].index >= lds) op.slot = (int)lu_slot[m.lower[e].index];
                 snprintf(buf, sizeof(buf), "x[%u] = __builtin_fmaf(-%%s, x[%u], x[%u]);", i, m.lower[e].col, i);
                 op.stmt = buf;
                 ops.push_back(op);
@@ -1003,16 +1083,19 @@ static u32 emit_d32_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
                 Op op;
                 op.load = lu_ld(m.upper[e].index);
+                if (pairs && m.upper[e].index >= lds) op.slot = (int)lu_slot[m.upper[e].index];
                 snprintf(buf, sizeof(buf), "x[%u] = __builtin_fmaf(-%%s, x[%u], x[%u]);", ii, m.upper[e].col, ii);
                 op.stmt = buf;
                 ops.push_back(op);
             }
             Op op;
             op.load = lu_ld(m.diag[ii]);
+            if (pairs && m.diag[ii] >= lds) op.slot = (int)lu_slot[m.diag[ii]];
             snprintf(buf, sizeof(buf), "x[%u] = x[%u] * %%s;", ii, ii);
             op.stmt = buf;
             ops.push_back(op);
         }
+        if (pairs) pair_loads(ops, 4, "SimkF2");
         w("    SIMK_DEV void solve(float (&x)[N]) const {\n        SIMK_MEMORY_FENCE();\n");
         u32 uid = 0;
         emit_ops(w, ops, o.batch, "        ", uid, 0, "float");

@@ -197,6 +197,39 @@ struct SimkRowPairs {
     }
 };
 
+// f32 workspace slots in adjacent pairs per lane, [pair][64 lanes][2] f32:
+// two slots in one 8-byte access (the f32 delta kernels).
+struct SimkF2 {
+    float x, y;
+};
+
+struct SimkRowPairs32 {
+    __amdgpu_buffer_rsrc_t rs;
+    simk_u32 voff;           // lane * 8
+    static constexpr simk_u32 kPairBytes = SIMK_WAVE * 8u;
+    SIMK_DEV void init(const void *base, simk_u32 lane, simk_u32 bytes) {
+        rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)bytes, SIMK_RSRC_FLAGS);
+        voff = lane * 8u;
+    }
+    SIMK_DEV float ld(simk_u32 slot) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                             rs, (int)voff, (int)((slot >> 1) * kPairBytes + (slot & 1u) * 4u), 0));
+    }
+    SIMK_DEV void st(simk_u32 slot, float x) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(simk_u32, x), rs, (int)voff,
+                                              (int)((slot >> 1) * kPairBytes + (slot & 1u) * 4u), 0);
+    }
+    SIMK_DEV SimkF2 ld2(simk_u32 pair) const {
+        // the whole 8-byte load reinterpreted at once: a bit cast applied to
+        // one element of the vector (v.y) yielded element 0 with this compiler
+        return __builtin_bit_cast(SimkF2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)(pair * kPairBytes), 0));
+    }
+    SIMK_DEV void st2(simk_u32 pair, float a, float b) const {
+        __builtin_amdgcn_raw_buffer_store_b64(simk_v2u{__builtin_bit_cast(simk_u32, a), __builtin_bit_cast(simk_u32, b)},
+                                              rs, (int)voff, (int)(pair * kPairBytes), 0);
+    }
+};
+
 // The same for f32 arrays (lane stride 4 bytes): the mixed-precision LU.
 struct SimkRows32 {
     __amdgpu_buffer_rsrc_t rs;

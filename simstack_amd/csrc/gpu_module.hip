@@ -1013,12 +1013,15 @@ hipError_t launch_wave(const GpuKinLaunch *L, u32 lanes, u32 repeats) {
 
 // Queue order of a single-mechanism advance: the caller's (simk_kin_set_order),
 // else, under queue policy 1, the cells by descending attempts in the
-// previous advance of the same block (rows, row stride and cell count). It
-// matters only when there are more cells than resident lanes: with fewer,
-// every cell starts at once. The allocation-free entry point keeps the
-// natural order (ordering allocates its buffers on first use).
+// previous advance of the same block (rows, row stride and cell count). With
+// more cells than resident lanes the costliest start first; with as many or
+// fewer, cells of like cost share waves, so fewer lanes wait on a wave's
+// slowest cell. Blocks under 8 waves of cells keep the natural order (the
+// sort would cost more than it saves), as does the allocation-free entry
+// point (ordering allocates its buffers on first use).
 const u32 *order_for(const GpuKinLaunch *L, u32 lanes, u32 waves, bool &by_cost) {
-    by_cost = !g.order && g.queue_policy == 1 && !g.use_scratch && (u64)lanes > (u64)waves * SIMK_WAVE;
+    (void)waves;
+    by_cost = !g.order && g.queue_policy == 1 && !g.use_scratch && lanes >= 8u * SIMK_WAVE;
     if (!by_cost) return g.order;
     SimkCostOrder &c = g.cost_order;
     const bool same = c.valid && c.y == L->y && c.padded == L->padded && c.lanes == lanes;

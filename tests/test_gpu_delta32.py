@@ -153,3 +153,25 @@ def test_delta32_repeats_after_a_previous_call(gm):
     c = z.subset(ok)
     _oracle(mech, meth, ctl, c, [0.2, 0.2, 0.2])
     assert compare_bits(a.subset(ok), c) == ZERO
+
+
+@pytest.mark.parametrize("name", ["c1", "c3c"])
+def test_delta32_paired_slots_keep_bits(gm, name):
+    """The generated f32 delta kernel with its workspace in adjacent f32
+    pairs (pair_slots = 1: LU entries in the solves' read order, two slots
+    per 8-byte access; an option, off by default): bit-identical to the
+    oracle, with pulses, over two calls."""
+    try:
+        gm.set_kernel(1)
+        gm.set_gen_options(pair_slots=1)
+        sp, mech, z = make_problem(name, 256)
+        add_pulses(z, seed=9, duration=0.01)
+        meth, ctl = problem.method(2), problem.control(rtol=1e-3, atol=1e-10, max_steps=1000)
+        a, b = z.copy(), z.copy()
+        info = _gpu(gm, mech, meth, ctl, a, [0.01, 0.01])
+        _oracle(mech, meth, ctl, b, [0.01, 0.01])
+        assert info.kernel == KERNEL_CODE[1]
+        assert compare_bits(a, b) == ZERO
+    finally:
+        gm.set_gen_options()
+        gm.set_kernel(0)

@@ -2,9 +2,9 @@
 (simk_kin_set_queue_policy 1, the default): consecutive advances of one
 block take its cells longest first. The order decides only which lane runs a
 cell and when, so every bit equals the natural-order run (itself checked
-against the oracle elsewhere). The grid is cut to 8 waves (512 lanes) so
-that the blocks below have more cells than resident lanes, the case where
-the order applies."""
+against the oracle elsewhere). The grid is cut to 8 waves (512 lanes), so
+the blocks below have more cells than resident lanes; the order applies to
+blocks of at least 512 cells."""
 import ctypes as C
 
 import numpy as np

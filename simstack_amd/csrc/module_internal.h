@@ -15,8 +15,6 @@ hipStream_t simk_module_stream();
 hipError_t simk_enqueue_plugflow(hipStream_t stream, f64 *y, u64 padded, u32 species, u32 zones, f64 a,
                                  const f64 *feed);
 
-// One request's advection rows for the batched upwind kernel: fields x
-// channels rows of `nodes` values at x + field * padded + channel * nodes.
 // Queue order by the previous advance's per-cell attempts (order.hip): the
 // key (block rows, lane count) it was measured on, and device buffers.
 struct SimkCostOrder {
@@ -28,8 +26,11 @@ struct SimkCostOrder {
     u32 *keys = nullptr, *keys_out = nullptr, *cells = nullptr, *order = nullptr;
     void *temp = nullptr;     // radix-sort scratch
     size_t temp_bytes = 0;
+    bool warmed = false;      // the sort has run once on these buffers
 };
-// Sizes the buffers for `lanes` cells (allocates when they grow).
+// Sizes the buffers for `lanes` cells (allocates when they grow); the first
+// time, also runs the sort once, so that its one-time set-up (kernel loading)
+// is not counted in the first ordered advance.
 hipError_t simk_cost_order_reserve(SimkCostOrder &c, hipStream_t stream, u32 lanes);
 // Before an advance of `lanes` cells: sizes the buffers, records steps + rejects.
 hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t stream, const f64 *steps, const f64 *rejects, u32 lanes);
@@ -37,6 +38,8 @@ hipError_t simk_cost_order_begin(SimkCostOrder &c, hipStream_t stream, const f64
 hipError_t simk_cost_order_end(SimkCostOrder &c, hipStream_t stream, const f64 *steps, const f64 *rejects, u32 lanes);
 void simk_cost_order_free(SimkCostOrder &c);
 
+// One request's advection rows for the batched upwind kernel: fields x
+// channels rows of `nodes` values at x + field * padded + channel * nodes.
 struct SimkUpwindSeg {
     double *x;
     unsigned long long padded;
@@ -67,6 +70,12 @@ struct SimkSlice {
     u32 *carry[4];           // per mechanism: stopped cells, waves * 64 entries
 };
 
+// Serving: one f32 delta-lane advance of L->duration for queue positions
+// [0, lanes) of `order` (lanes of mechanism `index` of the simk_kin_set_mechs
+// list), run to completion with that mechanism's generated f32 delta kernel
+// (compiled on first use), on the module stream.
+hipError_t simk_enqueue_d32_ext(u32 index, const GpuKinLaunch *L, u32 lanes, const u32 *order);
+
 // Enqueues the queue reset and one multi-mechanism persistent launch on the
 // module stream. Ls[i]: mechanism i's block (duration, padded, pointers);
 // lanes[i]: queue positions of mechanism i (may be 0); orders[i]: device
@@ -74,12 +83,6 @@ struct SimkSlice {
 // 8 u32 (queue heads [0, 4), carry counts [4, 8)), both owned by the caller.
 // After the launch queue[i] (clamped to lanes[i]) is the number of queue
 // positions taken and queue[4 + i] the number of carried cells.
-// Serving: one f32 delta-lane advance of L->duration for queue positions
-// [0, lanes) of `order` (lanes of mechanism `index` of the simk_kin_set_mechs
-// list), run to completion with that mechanism's generated f32 delta kernel
-// (compiled on first use), on the module stream.
-hipError_t simk_enqueue_d32_ext(u32 index, const GpuKinLaunch *L, u32 lanes, const u32 *order);
-
 hipError_t simk_enqueue_multi_ext(u32 count, const GpuKinLaunch *Ls, const u32 *lanes, const u32 *const *orders,
                                   u32 repeats, double *ws, u32 *queue, u32 waves, const SimkSlice *slice);
 

@@ -29,6 +29,13 @@ __global__ void __launch_bounds__(256) simk_attempts_key_kernel(const f64 *steps
     cells[i] = i;
 }
 
+__global__ void __launch_bounds__(256) simk_order_fill_kernel(u32 *keys, u32 *cells, u32 n) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = 0u;
+    cells[i] = i;
+}
+
 }  // namespace
 
 void simk_cost_order_free(SimkCostOrder &c) {
@@ -66,6 +73,16 @@ hipError_t simk_cost_order_reserve(SimkCostOrder &c, hipStream_t st, u32 lanes) 
         e = hipMalloc(&c.temp, need);
         if (e != hipSuccess) return e;
         c.temp_bytes = need;
+    }
+    if (!c.warmed) {
+        hipLaunchKernelGGL(simk_order_fill_kernel, dim3((lanes + 255) / 256), dim3(256), 0, st, c.keys, c.cells, lanes);
+        e = hipGetLastError();
+        size_t bytes = c.temp_bytes;
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairsDescending(c.temp, bytes, c.keys, c.keys_out, c.cells, c.order,
+                                                             (int)lanes, 0, 32, st);
+        if (e != hipSuccess) return e;
+        c.warmed = true;
     }
     return hipSuccess;
 }

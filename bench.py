@@ -459,8 +459,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="generator load-pipeline depth (0: default)")
     ap.add_argument("--lds-slots", type=int, default=0,
                     help="LU entries per lane held in LDS (0: default, -1: none)")
-    ap.add_argument("--variants", default="norefine,1sweep,2sweeps",
-                    help="mixed-precision variants to run: norefine, 1sweep, 2sweeps")
+    ap.add_argument("--variants", default="norefine,1sweep,2sweeps,fp32store",
+                    help="mixed-precision variants to run: norefine, 1sweep, 2sweeps (FP32 LU and solves), "
+                         "fp32store (FP32 factor storage, FP64 pivots and arithmetic)")
     ap.add_argument("--tune", type=float, default=0.0,
                     help="autotune FP64 on the workload for this many simulated seconds before timing (0: defaults)")
     ap.add_argument("--no-ragged", action="store_true", help="skip the three-mechanism ragged batch")
@@ -497,7 +498,7 @@ def main():
         kms = r["kernel_ms"]
         avg_ms = float(np.mean(kms))
         att = r["attempts"] / len(kms)
-        if precision == 0:
+        if precision in (0, 3):
             f64, f32 = mech.flops_per_attempt(), 0
         else:
             f64, f32 = mech.flops_mixed(0 if sweeps == NONE else sweeps)
@@ -528,17 +529,18 @@ def main():
         floor = accuracy(pr["zones"], f64r["zones"], ctl, n)
         floor.pop("meets_criterion")
         del pr
-        sweeps_of = {"norefine": NONE, "1sweep": 1, "2sweeps": 2}
+        sweeps_of = {"norefine": NONE, "1sweep": 1, "2sweeps": 2, "fp32store": 0}
         for name in [v for v in args.variants.split(",") if v]:
             sw = sweeps_of[name]
-            mr = run(2, sw)
+            prec = 3 if name == "fp32store" else 2
+            mr = run(prec, sw)
             variants["mixed_" + name] = {
                 "value": rate(mr), "wall_s": mr["wall"], "vs_fp64": rate(mr) / rate(f64r),
-                "dtype": "f32 LU / f64 state",
+                "dtype": "f32 LU / f64 state" if prec == 2 else "f32 LU storage / f64 pivots and arithmetic",
                 "kernel": mr["kernel"], "waves": mr["waves"], "failed_cells": mr["failed"],
                 "steps_ratio": mr["accepted"] / f64r["accepted"],
                 "accuracy": accuracy(mr["zones"], f64r["zones"], ctl, n),
-                "roofline": roofline(mr, 2, sw, f"c4_mixed_{name}")}
+                "roofline": roofline(mr, prec, sw, f"c4_mixed_{name}")}
             del mr
     best_name = max((k for k, v in variants.items() if v["accuracy"]["meets_criterion"]),
                     key=lambda k: variants[k]["value"])

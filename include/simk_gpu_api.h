@@ -140,14 +140,17 @@ typedef struct SimkGenOptions {
                             mixed precision and the multi-mechanism kernel; ~0u: all) */
     u32 batch;           /* loads issued this many statements ahead (0: 24; 16 in f32 delta lanes and refined mixed
                             precision) */
-    u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel */
+    u32 precision;       /* simk_jit_precompile only: 0 FP64 kernel, 1 f32 delta kernel, 2 mixed-precision kernel,
+                            3 FP32 factor-storage kernel */
     u32 sweeps;          /* mixed precision: FP64-residual refinement sweeps per stage (0: 1; ~0u: none) */
     u32 reuse;           /* a rate constant loaded this many statements back is reused (0: 384 in one-mechanism FP64
                             kernels, 1000 in unrefined mixed precision, else 96; ~0u: never) */
     u32 lds_slots;       /* LU entries per lane held in LDS instead of the workspace (0: default: 160 f64 entries
                             = 80 KiB per wave; 200 f32 = 50 KiB in unrefined mixed precision; 160 f32 =
                             40 KiB in f32 delta lanes; none for refined mixed kernels; ~0u: none; at most
-                            160 KiB) */
+                            160 KiB). FP32 factor storage: the lane's LDS budget in f32 words, the FP64
+                            pivots (two words each, always in LDS) first, off-diagonal entries in the rest
+                            (0: 320 words = 80 KiB per wave) */
     u32 fun_in_jac;      /* 0: default (on); 1: the -J pass also evaluates f(y) for stage 0 (one rate-constant
                             pass fewer per attempt); ~0u: stage 0 evaluates f(y) itself */
     u32 pair_slots;      /* FP64 kernels: 1: workspace slots in adjacent pairs per lane ([pair][64 lanes][2]), the
@@ -204,7 +207,8 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
 /* Statistics of the current mechanism and the last advance. */
 typedef struct SimkKinInfo {
     u32 kernel;          /* last launch: 1 generated FP64, 2 table FP64, 3 mixed precision, 4 multi-mechanism,
-                            5 f32 delta (table walk), 6 / 7 one wave per cell (FP64 / f32 delta), 8 generated f32 delta */
+                            5 f32 delta (table walk), 6 / 7 one wave per cell (FP64 / f32 delta), 8 generated f32 delta,
+                            9 FP32 factor storage */
     u32 ws_slots;        /* workspace f64 slots per lane */
     u32 waves;           /* persistent waves of the last launch */
     u32 ordered;         /* 1: the last advance took its cells longest first (simk_kin_set_queue_policy) */

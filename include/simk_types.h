@@ -154,7 +154,9 @@ typedef struct GpuKinLaunch {
     u32 padded;
     u32 lane_count;
     u32 precision;   /* 0: FP64 (bit-identical to the CPU path); 1: the reference's f32 delta lanes;
-                        2: mixed precision (FP32 Jacobian/LU, FP64 residual refinement and control) */
+                        2: mixed precision (FP32 Jacobian/LU, FP64 residual refinement and control);
+                        3: FP32 factor storage (off-diagonal LU entries rounded to and stored as f32, FP64
+                        reciprocal pivots, FP64 arithmetic and control) */
     f64 duration;
 } GpuKinLaunch;
 

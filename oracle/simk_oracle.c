@@ -234,8 +234,15 @@ static void o_jac_signed(const OMech *m, const f64 *k, const f64 *y, f64 *a, f64
 }
 
 /* kin_factor, kin_solver.h:95-113: row i's eliminations, then the diagonal
- * holds the reciprocal pivot 1 / (a_ii + ghinv). */
-static void o_factor(const OMech *m, f64 ghinv, f64 *a) {
+ * holds the reciprocal pivot 1 / (a_ii + ghinv).
+ *
+ * store32 (not the reference: the FP32 factor-storage form, GpuKinLaunch
+ * precision 3): once row i is final, each of its off-diagonal entries (L and U)
+ * is rounded to the nearest f32 and kept as that value, so later rows eliminate
+ * against the rounded U entries and the solves read rounded L and U entries.
+ * The reciprocal pivots and every operation stay FP64. With store32 0 this is
+ * the reference's factor. */
+static void o_factor_rows(const OMech *m, f64 ghinv, f64 *a, int store32) {
     for (u32 i = 0; i < m->n; ++i) {
         for (u32 d = m->div_begin[i]; d < m->div_begin[i + 1]; ++d) {
             KinLuDiv div = m->lu_div[d];
@@ -248,8 +255,16 @@ static void o_factor(const OMech *m, f64 ghinv, f64 *a) {
         }
         u32 ii = m->diag[i];
         a[ii] = 1.0 / (a[ii] + ghinv);
+        if (store32) {
+            for (u32 e = m->lower_begin[i]; e < m->lower_begin[i + 1]; ++e)
+                a[m->lower[e].index] = (f64)(f32)a[m->lower[e].index];
+            for (u32 e = m->upper_begin[i]; e < m->upper_begin[i + 1]; ++e)
+                a[m->upper[e].index] = (f64)(f32)a[m->upper[e].index];
+        }
     }
 }
+
+static void o_factor(const OMech *m, f64 ghinv, f64 *a) { o_factor_rows(m, ghinv, a, 0); }
 
 /* kin_solve, kin_solver.h:122-139 */
 static void o_solve(const OMech *m, const f64 *a, f64 *x) {
@@ -279,6 +294,10 @@ ORACLE_API void oracle_neg_jac(const void *mech, const SimkCaps *caps, const f64
 ORACLE_API void oracle_factor(const void *mech, const SimkCaps *caps, f64 ghinv, f64 *a) {
     OMech m = o_view(mech, caps);
     o_factor(&m, ghinv, a);
+}
+ORACLE_API void oracle_factor_store32(const void *mech, const SimkCaps *caps, f64 ghinv, f64 *a) {
+    OMech m = o_view(mech, caps);
+    o_factor_rows(&m, ghinv, a, 1);
 }
 ORACLE_API void oracle_solve(const void *mech, const SimkCaps *caps, const f64 *a, f64 *x) {
     OMech m = o_view(mech, caps);
@@ -319,7 +338,7 @@ typedef struct OWork {
 /* kin_integrate<f64>, kin_solver.h:189-313, one lane. stats = steps, rejects,
  * pulses, failed. */
 static void o_integrate(const OMech *m, const KinMethod *method, const KinControl *ctl, const f64 *k, const f64 *src,
-                        f64 *y, f64 *t_io, f64 t_end, f64 *h_io, f64 *stats, OWork *w) {
+                        f64 *y, f64 *t_io, f64 t_end, f64 *h_io, f64 *stats, OWork *w, int store32) {
     const u32 n = m->n;
     const f64 h_min = ctl->h_min, rtol = ctl->rtol, atol = ctl->atol;
     const f64 inv_n = 1.0 / (f64)n;
@@ -348,7 +367,7 @@ static void o_integrate(const OMech *m, const KinMethod *method, const KinContro
         o_fun(m, k, src, y, fcn);
         o_jac_signed(m, k, y, lu, -1.0);
         f64 ghinv = 1.0 / (hs * method->gamma0);
-        o_factor(m, ghinv, lu);
+        o_factor_rows(m, ghinv, lu, store32);
         f64 hinv = 1.0 / hs;
 
         for (u32 st = 0; st < method->stages; ++st) {
@@ -435,7 +454,7 @@ ORACLE_API void oracle_integrate(const void *mech, const SimkCaps *caps, const K
     OMech m = o_view(mech, caps);
     OWork w;
     o_alloc(&w, &m);
-    o_integrate(&m, method, ctl, k, src, y, t, t_end, h, stats, &w);
+    o_integrate(&m, method, ctl, k, src, y, t, t_end, h, stats, &w, 0);
     free(w.lu);
 }
 
@@ -443,7 +462,7 @@ ORACLE_API void oracle_integrate(const void *mech, const SimkCaps *caps, const K
  * kin_advance's pulse segments (kin_solver.h:322-347): local time [0, duration],
  * restart at h_start after each pulse, counters accumulated into the block. */
 static void o_advance_lane(const OMech *m, const KinMethod *method, const KinControl *ctl, const OracleLaunch *L,
-                           u32 i, OWork *w, f64 *y, f64 *k, f64 *src, f64 *pulse) {
+                           u32 i, OWork *w, f64 *y, f64 *k, f64 *src, f64 *pulse, int store32) {
     const u32 n = m->n, p = L->padded;
     for (u32 s = 0; s < n; ++s) {
         y[s] = L->y[(size_t)s * p + i];
@@ -461,7 +480,7 @@ static void o_advance_lane(const OMech *m, const KinMethod *method, const KinCon
         int pending = applied < count;
         f64 next_pulse = o_fmadd(applied, period, first);
         f64 seg_end = pending ? o_min(next_pulse, duration) : duration;
-        o_integrate(m, method, ctl, k, src, y, &t, seg_end, &h, stats, w);
+        o_integrate(m, method, ctl, k, src, y, &t, seg_end, &h, stats, w, store32);
         int alive = !(stats[3] > 0.0);
         int hit = pending && alive && (t >= seg_end);
         if (hit) {
@@ -489,6 +508,7 @@ typedef struct OPool {
     const u32 *lanes;
     u32 count;
     u32 next;
+    int store32;
 } OPool;
 
 static void *o_worker(void *arg) {
@@ -501,10 +521,22 @@ static void *o_worker(void *arg) {
         u32 i = __atomic_fetch_add(&pool->next, 1u, __ATOMIC_RELAXED);
         if (i >= pool->count) break;
         u32 lane = pool->lanes ? pool->lanes[i] : i;
-        o_advance_lane(pool->m, pool->method, pool->ctl, pool->L, lane, &w, y, k, src, pulse);
+        o_advance_lane(pool->m, pool->method, pool->ctl, pool->L, lane, &w, y, k, src, pulse, pool->store32);
     }
     free(buf);
     free(w.lu);
+    return 0;
+}
+
+static s32 o_run_pool(OPool *pool, u32 threads) {
+    if (threads <= 1) {
+        o_worker(pool);
+        return 0;
+    }
+    pthread_t *tid = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    for (u32 t = 0; t < threads; ++t) pthread_create(&tid[t], 0, o_worker, pool);
+    for (u32 t = 0; t < threads; ++t) pthread_join(tid[t], 0);
+    free(tid);
     return 0;
 }
 
@@ -513,16 +545,19 @@ static void *o_worker(void *arg) {
 ORACLE_API s32 oracle_advance(const void *mech, const SimkCaps *caps, const KinMethod *method, const KinControl *ctl,
                               const OracleLaunch *L, const u32 *lanes, u32 lane_count, u32 threads) {
     OMech m = o_view(mech, caps);
-    OPool pool = {&m, method, ctl, L, lanes, lane_count, 0};
-    if (threads <= 1) {
-        o_worker(&pool);
-        return 0;
-    }
-    pthread_t *tid = (pthread_t *)calloc(threads, sizeof(pthread_t));
-    for (u32 t = 0; t < threads; ++t) pthread_create(&tid[t], 0, o_worker, &pool);
-    for (u32 t = 0; t < threads; ++t) pthread_join(tid[t], 0);
-    free(tid);
-    return 0;
+    OPool pool = {&m, method, ctl, L, lanes, lane_count, 0, 0};
+    return o_run_pool(&pool, threads);
+}
+
+/* The same advance in the FP32 factor-storage form (GpuKinLaunch precision 3;
+ * not a reference path): the factor of every attempt is o_factor_rows with
+ * store32, everything else is the FP64 advance above. */
+ORACLE_API s32 oracle_advance_store32(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                      const KinControl *ctl, const OracleLaunch *L, const u32 *lanes, u32 lane_count,
+                                      u32 threads) {
+    OMech m = o_view(mech, caps);
+    OPool pool = {&m, method, ctl, L, lanes, lane_count, 0, 1};
+    return o_run_pool(&pool, threads);
 }
 
 /* ---- f32 delta lanes (GpuKinLaunch precision 1) ----------------------------

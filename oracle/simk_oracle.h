@@ -27,6 +27,7 @@ ORACLE_API void oracle_fun(const void *mech, const SimkCaps *caps, const f64 *k,
 ORACLE_API void oracle_jac(const void *mech, const SimkCaps *caps, const f64 *k, const f64 *y, f64 *jac);
 ORACLE_API void oracle_neg_jac(const void *mech, const SimkCaps *caps, const f64 *k, const f64 *y, f64 *a);
 ORACLE_API void oracle_factor(const void *mech, const SimkCaps *caps, f64 ghinv, f64 *a);
+ORACLE_API void oracle_factor_store32(const void *mech, const SimkCaps *caps, f64 ghinv, f64 *a);
 ORACLE_API void oracle_solve(const void *mech, const SimkCaps *caps, const f64 *a, f64 *x);
 ORACLE_API f64 oracle_cbrt_clamped(f64 x);
 ORACLE_API f64 oracle_step_factor(f64 err, u32 order);
@@ -34,6 +35,9 @@ ORACLE_API void oracle_integrate(const void *mech, const SimkCaps *caps, const K
                                  const f64 *k, const f64 *src, f64 *y, f64 *t, f64 t_end, f64 *h, f64 *stats);
 ORACLE_API s32 oracle_advance(const void *mech, const SimkCaps *caps, const KinMethod *method, const KinControl *ctl,
                               const OracleLaunch *L, const u32 *lanes, u32 lane_count, u32 threads);
+ORACLE_API s32 oracle_advance_store32(const void *mech, const SimkCaps *caps, const KinMethod *method,
+                                      const KinControl *ctl, const OracleLaunch *L, const u32 *lanes, u32 lane_count,
+                                      u32 threads);
 ORACLE_API s32 oracle_advance_delta32(const void *mech, const SimkCaps *caps, const KinMethod *method,
                                       const KinControl *ctl, const OracleLaunch *L, const u32 *lanes, u32 lane_count,
                                       u32 threads);

@@ -137,6 +137,16 @@ void emit_ops(Out &w, const std::vector<Op> &ops, u32 batch, const char *indent,
 }  // namespace
 
 u32 lds_entries(const Mech &m, const GenOptions &o) {
+    if (o.precision == 3) {
+        // FP32 factor storage: the n FP64 pivots always sit in LDS (two f32
+        // words each). lds_slots is the lane's LDS budget in f32 words; the
+        // off-diagonal f32 entries fill what the pivots leave, in compact
+        // order, up to 160 KiB per wave. Returns the off-diagonal entries in LDS.
+        const u32 off = m.nnz - m.n, piv = 2u * m.n;
+        const u32 budget = o.lds_slots < 640u ? o.lds_slots : 640u;
+        const u32 avail = budget > piv ? budget - piv : 0u;
+        return off < avail ? off : avail;
+    }
     // 160 KiB of LDS per workgroup (gfx950): 320 f64 or 640 f32 entries of 64 lanes.
     const u32 most = o.precision != 0 ? 640u : 320u;
     const u32 cap = m.nnz < most ? m.nnz : most;
@@ -153,6 +163,11 @@ u32 lds_entries(const Mech &m, const GenOptions &o) {
 struct Layout {
     u32 lds, koff, foff, rhsoff, lu32off, k32off, kcp, scp, slots;
     bool k32;
+    // FP32 factor storage (precision 3): lds_rank[idx] is the LDS f32 row of
+    // an off-diagonal entry held in LDS (~0u otherwise); lu_slot[idx] the f32
+    // slot of one outside LDS; pivots are in LDS by row.
+    bool s32 = false;
+    std::vector<u32> lds_rank;
     // Paired FP64 workspace: [paired region: LU entries lds.. in the solves'
     // read order (forward: each row's lower entries; backward, rows n-1..0:
     // upper entries then the pivot), K stages and f at stride np (n rounded
@@ -163,7 +178,54 @@ struct Layout {
     std::vector<u32> lu_slot;   // workspace slot of LU entry idx >= lds
 };
 
+// FP32 factor storage (precision 3): [paired region: K stages and f at
+// stride np | unpaired region: k copy, src copy | f32 region: off-diagonal LU
+// entries outside LDS in the solves' read order (forward: each row's lower
+// entries; backward, rows n-1..0: upper entries), two per f64 slot]. Without
+// pairs the first two regions are single slots [K stages | f | k copy | src copy].
+static Layout layout_s32(const Mech &m, const KinMethod &me, const GenOptions &o) {
+    Layout L;
+    L.s32 = true;
+    L.k32 = false;
+    L.k32off = 0;
+    const u32 stored_stages = me.stages > 0 ? me.stages - 1 : 0;
+    L.lds = lds_entries(m, o);
+    L.lds_rank.assign(m.nnz, ~0u);
+    L.lu_slot.assign(m.nnz, 0);
+    std::vector<bool> diag(m.nnz, false);
+    for (u32 i = 0; i < m.n; ++i) diag[m.diag[i]] = true;
+    u32 q = 0;
+    for (u32 idx = 0; idx < m.nnz && q < L.lds; ++idx)
+        if (!diag[idx]) L.lds_rank[idx] = q++;
+    u32 t = 0;
+    for (u32 i = 0; i < m.n; ++i)
+        for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e)
+            if (L.lds_rank[m.lower[e].index] == ~0u) L.lu_slot[m.lower[e].index] = t++;
+    for (u32 ii = m.n; ii-- > 0;)
+        for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e)
+            if (L.lds_rank[m.upper[e].index] == ~0u) L.lu_slot[m.upper[e].index] = t++;
+    L.pairs = o.pair_slots != 0;
+    L.np = L.pairs ? m.n + (m.n & 1u) : m.n;
+    L.koff = 0;
+    L.foff = stored_stages * L.np;
+    if (L.pairs) {
+        L.paired = L.foff + L.np;
+        L.kcp = 0;
+        L.scp = m.R;
+        L.lu32off = L.paired + m.R + m.n;
+    } else {
+        L.paired = 0;
+        L.kcp = L.foff + m.n;
+        L.scp = L.kcp + m.R;
+        L.lu32off = L.scp + m.n;
+    }
+    L.rhsoff = L.lu32off;
+    L.slots = L.lu32off + (t + 1) / 2;
+    return L;
+}
+
 static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
+    if (o.precision == 3) return layout_s32(m, me, o);
     Layout L;
     const bool mixed = o.precision == 2;
     const u32 sweeps = mixed ? o.sweeps : 0;
@@ -233,6 +295,7 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
 // mixed kernel is slower with LDS entries.
 u32 auto_lds(const GenOptions &o) {
     if (o.precision == 1) return 160u;
+    if (o.precision == 3) return 320u;   // f32 words per lane: 80 KiB per wave, pivots included
     if (o.precision == 2) return o.sweeps ? 0u : 200u;
     return 160u;
 }
@@ -246,7 +309,7 @@ u32 auto_batch(const GenOptions &o) { return o.precision == 1 || (o.precision ==
 // kernel keeps 96 (larger windows spill there and run 25 % slower).
 u32 auto_reuse(const GenOptions &o, bool multi) {
     if (o.precision == 2 && o.sweeps == 0) return 1000u;
-    return o.precision == 0 && !multi ? 384u : 96u;
+    return (o.precision == 0 || o.precision == 3) && !multi ? 384u : 96u;
 }
 
 u32 auto_fun_in_jac(const GenOptions &) { return 1u; }
@@ -257,7 +320,7 @@ u32 auto_fun_in_jac(const GenOptions &) { return 1u; }
 // mechanism at 65,536 zones 16.0 -> 14.2 ms (+13 %); the three-mechanism
 // kernel at 221,000 cells 318 -> 286 ms (+11 %). Mixed-precision and f32
 // delta kernels keep their own layouts.
-u32 auto_pair(const GenOptions &o) { return o.precision == 0 ? 1u : 0u; }
+u32 auto_pair(const GenOptions &o) { return o.precision == 0 || o.precision == 3 ? 1u : 0u; }
 
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
@@ -288,6 +351,11 @@ void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
 static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const GenOptions &o, const char *sname) {
     const u32 n = m.n;
     const bool mixed = o.precision == 2;
+    // FP32 factor storage: FP64 arithmetic throughout; a finished row's
+    // off-diagonal entries are rounded to f32 and kept as that value (in
+    // registers too), stored as f32 (LDS or the f32 workspace region); the
+    // reciprocal pivots stay FP64 in LDS.
+    const bool s32 = o.precision == 3;
     const u32 sweeps = mixed ? o.sweeps : 0;
     // LU entries below lds are in LDS. The cell's rate constants and sources
     // are copied into the lane's slots (kcp, scp) when the lane takes the
@@ -297,6 +365,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const u32 lds = Lo.lds, koff = Lo.koff, foff = Lo.foff, rhsoff = Lo.rhsoff, lu32off = Lo.lu32off;
     const u32 kcp = Lo.kcp, scp = Lo.scp, ws_slots = Lo.slots;
     const bool pairs = Lo.pairs;
+    const bool lu_pairs = pairs && !s32;   // FP64 LU entries in paired slots
     w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
     w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u, K32OFF = %u;\n", koff, foff,
@@ -312,15 +381,23 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // lanes all have all-zero (+0.0) rows starts f from the literal +0.0,
     // which is what loading those rows would give.
     w("    simk_u32 srcnz;\n");
-    if (mixed) w("    SimkRows32 w32;\n");
-    if (lds) {
+    if (mixed || s32) w("    SimkRows32 w32;\n");
+    if (s32) {
+        w("    simk_u32 ln;\n");
+        w("    SIMK_DEV double P(simk_u32 i) const { return simk_lds[i * SIMK_WAVE + ln]; }\n");
+        w("    SIMK_DEV void PS(simk_u32 i, double v) const { simk_lds[i * SIMK_WAVE + ln] = v; }\n");
+        if (lds) {
+            w("    SIMK_DEV float L32(simk_u32 i) const { return simk_lds32[i * SIMK_WAVE + ln]; }\n");
+            w("    SIMK_DEV void LS32(simk_u32 i, float v) const { simk_lds32[i * SIMK_WAVE + ln] = v; }\n");
+        }
+    } else if (lds) {
         const char *T = mixed ? "float" : "double", *A = mixed ? "simk_lds32" : "simk_lds";
         w("    simk_u32 ln;\n");
         w("    SIMK_DEV %s L(simk_u32 i) const { return %s[i * SIMK_WAVE + ln]; }\n", T, A);
         w("    SIMK_DEV void LS(simk_u32 i, %s v) const { %s[i * SIMK_WAVE + ln] = v; }\n", T, A);
     }
     w("    SIMK_DEV void init(const SimkKinArgs &A, simk_u32 wave, simk_u32 lane) {\n");
-    if (lds) w("        ln = lane;\n");
+    if (lds || s32) w("        ln = lane;\n");
     w("        const simk_u32 stride = A.padded * 8u;\n");
     w("        kr.init(A.k, 0, stride, R * stride);\n");
     w("        sr.init(A.src, 0, stride, N * stride);\n");
@@ -331,7 +408,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     } else {
         w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
     }
-    if (mixed)
+    if (mixed || s32)
         w("        w32.init(A.ws + ((simk_u64)wave * A.ws_slots + %u) * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, %u * SIMK_WAVE * 8u);\n",
           lu32off, ws_slots - lu32off);
     w("    }\n");
@@ -357,7 +434,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         w("    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }\n");
         w("    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }\n");
     }
-    if (mixed) {
+    if (mixed || s32) {
         w("    SIMK_DEV float W32(simk_u32 i) const { return w32.ld(i); }\n");
         w("    SIMK_DEV void W32S(simk_u32 i, float v) const { w32.st(i, v); }\n");
     }
@@ -484,13 +561,33 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     auto near = [&](u32 idx, u32 i) { return window == 0 || i - row_of[idx] <= window; };
     // LU entry idx: LDS entry idx, or workspace entry idx - lds (W: f64
     // slots; W32: the FP32 LU region of mixed kernels).
+    // FP32 factor storage: pivot of row r at P(r); an off-diagonal entry at
+    // LDS f32 row lds_rank[idx] or f32 workspace slot lu_slot[idx], widened on load.
+    std::vector<u32> diag_row(m.nnz, ~0u);
+    for (u32 i = 0; i < n; ++i) diag_row[m.diag[i]] = i;
     auto lu_ld = [&](u32 idx) {
-        char buf[24];
+        char buf[40];
+        if (s32) {
+            if (diag_row[idx] != ~0u) snprintf(buf, sizeof(buf), "P(%u)", diag_row[idx]);
+            else if (Lo.lds_rank[idx] != ~0u) snprintf(buf, sizeof(buf), "L32(%u)", Lo.lds_rank[idx]);
+            else snprintf(buf, sizeof(buf), "W32(%u)", Lo.lu_slot[idx]);
+            return std::string(buf);
+        }
         snprintf(buf, sizeof(buf), idx < lds ? "L(%u)" : (mixed ? "W32(%u)" : "W(%u)"), idx < lds ? idx : Lo.lu_slot[idx]);
         return std::string(buf);
     };
+    // An f32 entry is loaded as a float (and widened, in FP32 factor storage,
+    // where the statement uses it): a conversion at the load would wait for
+    // the load there, ahead of the pipelined use.
+    auto lu_type = [&](u32 idx) { return std::string(mixed || (s32 && diag_row[idx] == ~0u) ? "float" : ""); };
     auto lu_st = [&](u32 idx, const char *v) {
-        char buf[40];
+        char buf[48];
+        if (s32) {
+            if (diag_row[idx] != ~0u) snprintf(buf, sizeof(buf), "PS(%u, %s%u);", diag_row[idx], v, idx);
+            else if (Lo.lds_rank[idx] != ~0u) snprintf(buf, sizeof(buf), "LS32(%u, (float)%s%u);", Lo.lds_rank[idx], v, idx);
+            else snprintf(buf, sizeof(buf), "W32S(%u, (float)%s%u);", Lo.lu_slot[idx], v, idx);
+            return std::string(buf);
+        }
         snprintf(buf, sizeof(buf), idx < lds ? "LS(%u, %s%u);" : (mixed ? "W32S(%u, %s%u);" : "WS(%u, %s%u);"),
                  idx < lds ? idx : Lo.lu_slot[idx], v, idx);
         return std::string(buf);
@@ -599,7 +696,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 Op op;
                 if (!kk_reg) {
                     op.load = lu_ld(dv.kk);
-                    if (pairs && dv.kk >= lds) op.slot = (int)Lo.lu_slot[dv.kk];
+                    op.type = lu_type(dv.kk);
+                    if (lu_pairs && dv.kk >= lds) op.slot = (int)Lo.lu_slot[dv.kk];
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %%s; %s%u = l%u;", T, d, V, dv.ik, V, dv.ik, d);
                 } else {
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %s%u; %s%u = l%u;", T, d, V, dv.ik, V, dv.kk, V,
@@ -615,7 +713,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %s%u, %s%u);", V, sb.ij, fn, d, V, sb.kj, V, sb.ij);
                     } else {
                         o2.load = lu_ld(sb.kj);
-                        if (pairs && sb.kj >= lds) o2.slot = (int)Lo.lu_slot[sb.kj];
+                        o2.type = lu_type(sb.kj);
+                        if (lu_pairs && sb.kj >= lds) o2.slot = (int)Lo.lu_slot[sb.kj];
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %%s, %s%u);", V, sb.ij, fn, d, V, sb.ij);
                     }
                     o2.stmt = buf;
@@ -626,6 +725,17 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 snprintf(buf, sizeof(buf), "b%u = 1.0f / (b%u + gf);", m.diag[i], m.diag[i]);
                 stmt(buf);
                 for (u32 idx : row) stmt(lu_st(idx, "b"));
+            } else if (s32) {
+                snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
+                stmt(buf);
+                // the row is final: its off-diagonal entries become their f32
+                // values, which later rows and the solves then use
+                for (u32 idx : row)
+                    if (idx != m.diag[i]) {
+                        snprintf(buf, sizeof(buf), "a%u = (double)(float)a%u;", idx, idx);
+                        stmt(buf);
+                    }
+                for (u32 idx : row) stmt(lu_st(idx, "a"));
             } else {
                 snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
                 stmt(buf);
@@ -682,7 +792,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             for (u32 e = m.lower_begin[i]; e < m.lower_begin[i + 1]; ++e) {
                 Op op;
                 op.load = lu_ld(m.lower[e].index);
-                if (pairs && m.lower[e].index >= lds) op.slot = (int)Lo.lu_slot[m.lower[e].index];
+                op.type = lu_type(m.lower[e].index);
+                if (lu_pairs && m.lower[e].index >= lds) op.slot = (int)Lo.lu_slot[m.lower[e].index];
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, i);
                 std::string st = buf + f + ", ";
@@ -694,7 +805,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             for (u32 e = m.upper_begin[ii]; e < m.upper_begin[ii + 1]; ++e) {
                 Op op;
                 op.load = lu_ld(m.upper[e].index);
-                if (pairs && m.upper[e].index >= lds) op.slot = (int)Lo.lu_slot[m.upper[e].index];
+                op.type = lu_type(m.upper[e].index);
+                if (lu_pairs && m.upper[e].index >= lds) op.slot = (int)Lo.lu_slot[m.upper[e].index];
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, ii);
                 std::string st = buf + f + ", ";
@@ -704,7 +816,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             }
             Op op;
             op.load = lu_ld(m.diag[ii]);
-            if (pairs && m.diag[ii] >= lds) op.slot = (int)Lo.lu_slot[m.diag[ii]];
+            op.type = lu_type(m.diag[ii]);
+            if (lu_pairs && m.diag[ii] >= lds) op.slot = (int)Lo.lu_slot[m.diag[ii]];
             snprintf(buf, sizeof(buf), "%s[%u] = %s[%u] * %%s;", vec, ii, vec, ii);
             op.stmt = buf;
             ops.push_back(op);
@@ -1137,6 +1250,7 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     char name[64];
     const bool mixed = o.precision == 2;
     if (mixed) snprintf(name, sizeof(name), "simk_mix%u_%016llx", o.sweeps, (unsigned long long)sig);
+    else if (o.precision == 3) snprintf(name, sizeof(name), "simk_f32s_%016llx", (unsigned long long)sig);
     else if (o.precision == 1) snprintf(name, sizeof(name), "simk_d32g_%016llx", (unsigned long long)sig);
     else snprintf(name, sizeof(name), "simk_gen_%016llx", (unsigned long long)sig);
     g.kernel = name;
@@ -1144,7 +1258,11 @@ GenResult generate(const Mech &m, const KinMethod &me, const GenOptions &o, cons
     w("// generated by simstack_amd codegen: %u species, %u reactions, %u non-zeros\n", m.n, m.R, m.nnz);
     w.s += device_header;
     w.s += "\n";
-    if (const u32 lds = lds_entries(m, o))
+    if (o.precision == 3) {
+        w("__shared__ double simk_lds[%u * SIMK_WAVE];\n", m.n);
+        if (const u32 lds = lds_entries(m, o)) w("__shared__ float simk_lds32[%u * SIMK_WAVE];\n", lds);
+        w("\n");
+    } else if (const u32 lds = lds_entries(m, o))
         w("__shared__ %s[%u * SIMK_WAVE];\n\n", o.precision != 0 ? "float simk_lds32" : "double simk_lds", lds);
     if (o.precision == 1) {
         // f32 delta lanes: the method and control come in the argument block

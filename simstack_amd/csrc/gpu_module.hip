@@ -278,6 +278,8 @@ struct State {
     bool mix_tuned = false;
     GenOptions mix_gen;
     u32 mix_grid = 0;
+    JitKernel jit_s32;         // FP32 factor-storage kernel, compiled on the first precision-3 launch
+    bool jit_s32_ok = false;
     JitKernel jit_d32;         // generated f32 delta kernel, compiled on the first precision-1 launch
     bool jit_d32_ok = false, jit_d32_tried = false;
     u32 sweeps = 1;
@@ -430,6 +432,8 @@ bool jit_code(const GenResult &gr, std::vector<char> &code, bool &cached, std::s
     return true;
 }
 
+void unload_s32();
+
 void free_mech_resources() {
     simk_cost_order_free(g.cost_order);   // recorded costs belong to the old mechanism's cells
     if (g.d_tables) (void)hipFree(g.d_tables);
@@ -442,6 +446,7 @@ void free_mech_resources() {
     if (g.jit_mixed.mod) (void)hipModuleUnload(g.jit_mixed.mod);
     g.jit_mixed = JitKernel();
     g.jit_mixed_ok = false;
+    unload_s32();
     if (g.jit_d32.mod) (void)hipModuleUnload(g.jit_d32.mod);
     g.jit_d32 = JitKernel();
     g.jit_d32_ok = g.jit_d32_tried = false;
@@ -759,6 +764,12 @@ bool load_generated_mech(const Mech &m, const GenOptions &opt, JitKernel &out) {
 }
 
 bool load_generated(const GenOptions &opt, JitKernel &out) { return load_generated_mech(g.mech, opt, out); }
+
+void unload_s32() {
+    if (g.jit_s32.mod) (void)hipModuleUnload(g.jit_s32.mod);
+    g.jit_s32 = JitKernel();
+    g.jit_s32_ok = false;
+}
 
 void unload_mixed() {
     if (g.jit_mixed.mod) (void)hipModuleUnload(g.jit_mixed.mod);
@@ -1140,7 +1151,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     if (!L->y || !L->h || !L->steps || !L->rejects || !L->pulses || !L->failed || !L->k || !L->src || !L->pulse ||
         !L->period || !L->pulse_first || !L->pulse_count)
         return hipErrorInvalidValue;
-    if (L->precision > 2) return hipErrorNotSupported;
+    if (L->precision > 3) return hipErrorNotSupported;
     if (g.mode == 1) return launch_wave(L, lanes, repeats);
     if (L->precision == 1) return launch_delta32(L, lanes, repeats);
     const JitKernel *jk = nullptr;
@@ -1155,6 +1166,17 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
             if (!g.jit_mixed_ok) return hipErrorInvalidImage;
         }
         jk = &g.jit_mixed;
+    } else if (L->precision == 3) {
+        // FP32 factor storage exists only as a generated kernel, built from
+        // the current generator options.
+        if (!g.jit_s32_ok) {
+            if (g.use_scratch) return hipErrorNotReady;   // loading a kernel allocates
+            GenOptions so = g.gen;
+            so.precision = 3;
+            g.jit_s32_ok = load_generated(so, g.jit_s32);
+            if (!g.jit_s32_ok) return hipErrorInvalidImage;
+        }
+        jk = &g.jit_s32;
     } else {
         if (g.kernel_kind == 1 && !g.jit_ok) return hipErrorInvalidImage;
         if (g.jit_ok && g.kernel_kind != 2) jk = &g.jit;
@@ -1194,7 +1216,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
     A.tables = (const SimkDevTables *)g.d_tables;
     A.ctl = g.ctl;
     A.method = g.dev_method;
-    g.last_kernel = use_gen ? (L->precision == 2 ? 3 : 1) : 2;
+    g.last_kernel = use_gen ? (L->precision == 2 ? 3 : (L->precision == 3 ? 9 : 1)) : 2;
     g.last_name = use_gen ? jk->name : std::string("simk_table_kernel");
     g.last_slots = slots;
     g.last_waves = waves;
@@ -1747,6 +1769,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.fun_in_jac = n.fun_in_jac;
     g.gen.pair_slots = n.pair_slots;
     g.gen.batch = n.batch;
+    unload_s32();   // the FP32 factor-storage kernel is built from these options
     // explicit options are what a later untune() restores
     if (g.tuned) g.user_gen = g.gen;
     const u32 sw = n.sweeps;

@@ -51,12 +51,15 @@ def oracle():
         _sig(lib, "oracle_jac", None, V, P(_ffi.SimkCaps), f64p, f64p, f64p)
         _sig(lib, "oracle_neg_jac", None, V, P(_ffi.SimkCaps), f64p, f64p, f64p)
         _sig(lib, "oracle_factor", None, V, P(_ffi.SimkCaps), C.c_double, f64p)
+        _sig(lib, "oracle_factor_store32", None, V, P(_ffi.SimkCaps), C.c_double, f64p)
         _sig(lib, "oracle_solve", None, V, P(_ffi.SimkCaps), f64p, f64p)
         _sig(lib, "oracle_cbrt_clamped", C.c_double, C.c_double)
         _sig(lib, "oracle_step_factor", C.c_double, C.c_double, C.c_uint32)
         _sig(lib, "oracle_integrate", None, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl),
              f64p, f64p, f64p, f64p, C.c_double, f64p, f64p)
         _sig(lib, "oracle_advance", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl),
+             P(OracleLaunch), V, C.c_uint32, C.c_uint32)
+        _sig(lib, "oracle_advance_store32", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl),
              P(OracleLaunch), V, C.c_uint32, C.c_uint32)
         _sig(lib, "oracle_advance_delta32", C.c_int32, V, P(_ffi.SimkCaps), P(_ffi.KinMethod), P(_ffi.KinControl),
              V, V, C.c_uint32, C.c_uint32)
@@ -122,6 +125,18 @@ def oracle_advance(mech, meth, ctl, z, duration, lanes=None, threads=8):
         n = len(arr)
     oracle().oracle_advance(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(ctl), C.byref(L),
                             None if arr is None else arr.ctypes.data, n, threads)
+
+
+def oracle_advance_store32(mech, meth, ctl, z, duration, lanes=None, threads=8):
+    """The FP32 factor-storage advance (GpuKinLaunch precision 3) of the oracle, in place."""
+    L = _launch(OracleLaunch, z, duration, {})
+    if lanes is None:
+        arr, n = None, z.count
+    else:
+        arr = np.ascontiguousarray(lanes, dtype=np.uint32)
+        n = len(arr)
+    oracle().oracle_advance_store32(mech.bytes.ctypes.data, C.byref(mech.caps), C.byref(meth), C.byref(ctl),
+                                    C.byref(L), None if arr is None else arr.ctypes.data, n, threads)
 
 
 def oracle_advance_delta32(mech, meth, ctl, z, duration, lanes=None, threads=8):

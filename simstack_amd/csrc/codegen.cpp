@@ -62,7 +62,8 @@ struct Op {
 
 // Paired workspace: an op loading slot s and one of the next `lookahead` ops
 // loading its partner s ^ 1 (no barrier between) share one 16-byte load.
-void pair_loads(std::vector<Op> &ops, size_t lookahead = 4, const char *pair_type = "SimkD2") {
+void pair_loads(std::vector<Op> &ops, size_t lookahead = 4, const char *pair_type = "SimkD2",
+                const char *accessor = "W2") {
     for (size_t i = 0; i < ops.size(); ++i) {
         Op &a = ops[i];
         if (a.slot < 0 || a.from >= 0 || !a.type.empty() || a.barrier) continue;
@@ -71,7 +72,7 @@ void pair_loads(std::vector<Op> &ops, size_t lookahead = 4, const char *pair_typ
             if (b.barrier) break;
             if (b.slot != (a.slot ^ 1) || b.from >= 0 || !b.type.empty()) continue;
             char buf[32];
-            snprintf(buf, sizeof(buf), "W2(%d)", a.slot >> 1);
+            snprintf(buf, sizeof(buf), "%s(%d)", accessor, a.slot >> 1);
             a.load = buf;
             a.type = pair_type;
             a.part = (a.slot & 1) ? ".y" : ".x";
@@ -366,6 +367,9 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const u32 kcp = Lo.kcp, scp = Lo.scp, ws_slots = Lo.slots;
     const bool pairs = Lo.pairs;
     const bool lu_pairs = pairs && !s32;   // FP64 LU entries in paired slots
+    // FP32 factor storage with pairs: the f32 region is [pair][64 lanes][2],
+    // two entries per 8-byte lane access (WQ2 / WQS2)
+    const bool q_pairs = pairs && s32;
     w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
     w("    static constexpr simk_u32 KOFF = %u, FOFF = %u, RHSOFF = %u, STAGES = %u, K32OFF = %u;\n", koff, foff,
@@ -381,7 +385,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // lanes all have all-zero (+0.0) rows starts f from the literal +0.0,
     // which is what loading those rows would give.
     w("    simk_u32 srcnz;\n");
-    if (mixed || s32) w("    SimkRows32 w32;\n");
+    if (mixed || (s32 && !q_pairs)) w("    SimkRows32 w32;\n");
+    if (q_pairs) w("    SimkRowPairs32 wq;\n");
     if (s32) {
         w("    simk_u32 ln;\n");
         w("    SIMK_DEV double P(simk_u32 i) const { return simk_lds[i * SIMK_WAVE + ln]; }\n");
@@ -408,9 +413,12 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     } else {
         w("        ws.init(A.ws + (simk_u64)wave * A.ws_slots * SIMK_WAVE, lane * 8u, SIMK_WAVE * 8u, A.ws_slots * SIMK_WAVE * 8u);\n");
     }
-    if (mixed || s32)
+    if (mixed || (s32 && !q_pairs))
         w("        w32.init(A.ws + ((simk_u64)wave * A.ws_slots + %u) * SIMK_WAVE, lane * 4u, SIMK_WAVE * 4u, %u * SIMK_WAVE * 8u);\n",
           lu32off, ws_slots - lu32off);
+    if (q_pairs)
+        w("        wq.init(A.ws + ((simk_u64)wave * A.ws_slots + %u) * SIMK_WAVE, lane, %u * SIMK_WAVE * 8u);\n", lu32off,
+          ws_slots - lu32off);
     w("    }\n");
     w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
     w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
@@ -434,7 +442,12 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         w("    SIMK_DEV double W(simk_u32 i) const { return ws.ld(i); }\n");
         w("    SIMK_DEV void WS(simk_u32 i, double v) const { ws.st(i, v); }\n");
     }
-    if (mixed || s32) {
+    if (q_pairs) {
+        w("    SIMK_DEV float W32(simk_u32 i) const { return wq.ld(i); }\n");
+        w("    SIMK_DEV void W32S(simk_u32 i, float v) const { wq.st(i, v); }\n");
+        w("    SIMK_DEV SimkF2 WQ2(simk_u32 pair) const { return wq.ld2(pair); }\n");
+        w("    SIMK_DEV void WQS2(simk_u32 pair, float a, float b) const { wq.st2(pair, a, b); }\n");
+    } else if (mixed || s32) {
         w("    SIMK_DEV float W32(simk_u32 i) const { return w32.ld(i); }\n");
         w("    SIMK_DEV void W32S(simk_u32 i, float v) const { w32.st(i, v); }\n");
     }
@@ -579,7 +592,19 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // An f32 entry is loaded as a float (and widened, in FP32 factor storage,
     // where the statement uses it): a conversion at the load would wait for
     // the load there, ahead of the pipelined use.
-    auto lu_type = [&](u32 idx) { return std::string(mixed || (s32 && diag_row[idx] == ~0u) ? "float" : ""); };
+    auto lu_type = [&](u32 idx) {
+        if (q_pairs && diag_row[idx] == ~0u && Lo.lds_rank[idx] == ~0u) return std::string();   // typed after pairing
+        return std::string(mixed || (s32 && diag_row[idx] == ~0u) ? "float" : "");
+    };
+    // the f32 workspace slot an op's load reads, for pairing (-1: none)
+    auto q_slot = [&](u32 idx) {
+        return q_pairs && diag_row[idx] == ~0u && Lo.lds_rank[idx] == ~0u ? (int)Lo.lu_slot[idx] : -1;
+    };
+    auto q_pair_loads = [&](std::vector<Op> &ops) {
+        pair_loads(ops, 4, "SimkF2", "WQ2");
+        for (Op &op : ops)
+            if (op.slot >= 0 && op.type.empty() && op.from < 0 && !op.load.empty()) op.type = "float";
+    };
     auto lu_st = [&](u32 idx, const char *v) {
         char buf[48];
         if (s32) {
@@ -697,6 +722,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 if (!kk_reg) {
                     op.load = lu_ld(dv.kk);
                     op.type = lu_type(dv.kk);
+                    if (q_pairs) op.slot = q_slot(dv.kk);
                     if (lu_pairs && dv.kk >= lds) op.slot = (int)Lo.lu_slot[dv.kk];
                     snprintf(buf, sizeof(buf), "const %s l%u = %s%u * %%s; %s%u = l%u;", T, d, V, dv.ik, V, dv.ik, d);
                 } else {
@@ -714,6 +740,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                     } else {
                         o2.load = lu_ld(sb.kj);
                         o2.type = lu_type(sb.kj);
+                        if (q_pairs) o2.slot = q_slot(sb.kj);
                         if (lu_pairs && sb.kj >= lds) o2.slot = (int)Lo.lu_slot[sb.kj];
                         snprintf(buf, sizeof(buf), "%s%u = %s%u, %%s, %s%u);", V, sb.ij, fn, d, V, sb.ij);
                     }
@@ -735,7 +762,28 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                         snprintf(buf, sizeof(buf), "a%u = (double)(float)a%u;", idx, idx);
                         stmt(buf);
                     }
-                for (u32 idx : row) stmt(lu_st(idx, "a"));
+                if (!q_pairs) {
+                    for (u32 idx : row) stmt(lu_st(idx, "a"));
+                } else {
+                    // workspace entries by slot, partners in one 8-byte store
+                    std::vector<std::pair<u32, u32>> ws_row;   // (slot, compact index)
+                    for (u32 idx : row) {
+                        if (diag_row[idx] != ~0u || Lo.lds_rank[idx] != ~0u) stmt(lu_st(idx, "a"));
+                        else ws_row.push_back({Lo.lu_slot[idx], idx});
+                    }
+                    std::sort(ws_row.begin(), ws_row.end());
+                    for (size_t q = 0; q < ws_row.size(); ++q) {
+                        const u32 sl = ws_row[q].first;
+                        if (sl % 2 == 0 && q + 1 < ws_row.size() && ws_row[q + 1].first == sl + 1) {
+                            snprintf(buf, sizeof(buf), "WQS2(%u, (float)a%u, (float)a%u);", sl / 2, ws_row[q].second,
+                                     ws_row[q + 1].second);
+                            ++q;
+                        } else {
+                            snprintf(buf, sizeof(buf), "W32S(%u, (float)a%u);", sl, ws_row[q].second);
+                        }
+                        stmt(buf);
+                    }
+                }
             } else {
                 snprintf(buf, sizeof(buf), "a%u = 1.0 / (a%u + g);", m.diag[i], m.diag[i]);
                 stmt(buf);
@@ -774,7 +822,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             }
         }
         barrier();
-        if (pairs) pair_loads(ops);
+        if (q_pairs) q_pair_loads(ops);
+        else if (pairs) pair_loads(ops);
         w("    SIMK_DEV void jac_factor(const Vec &y, const double g) const {\n");
         if (fij) w("        const bool anysrc = any_src();\n");
         if (mixed) w("        const float gf = (float)g;\n");
@@ -793,6 +842,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 Op op;
                 op.load = lu_ld(m.lower[e].index);
                 op.type = lu_type(m.lower[e].index);
+                if (q_pairs) op.slot = q_slot(m.lower[e].index);
                 if (lu_pairs && m.lower[e].index >= lds) op.slot = (int)Lo.lu_slot[m.lower[e].index];
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, i);
@@ -806,6 +856,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                 Op op;
                 op.load = lu_ld(m.upper[e].index);
                 op.type = lu_type(m.upper[e].index);
+                if (q_pairs) op.slot = q_slot(m.upper[e].index);
                 if (lu_pairs && m.upper[e].index >= lds) op.slot = (int)Lo.lu_slot[m.upper[e].index];
                 std::string f = FN;
                 snprintf(buf, sizeof(buf), "%s[%u] = ", vec, ii);
@@ -822,7 +873,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             op.stmt = buf;
             ops.push_back(op);
         }
-        if (pairs) pair_loads(ops);
+        if (q_pairs) q_pair_loads(ops);
+        else if (pairs) pair_loads(ops);
         return ops;
     };
     if (!mixed) {

@@ -8,8 +8,9 @@ are resident in HBM before timing starts.
 
 It runs the FP64 path (bit-identical to the FP64 oracle), the FP64 path
 started one ulp higher (the sensitivity floor of any comparison), and the
-mixed-precision variants (FP32 LU with 0, 1 or 2 FP64 refinement sweeps per
-stage). The accuracy criterion: the p99.9 over cells of each cell's max
+mixed-precision variants: FP32 LU and solves with 0, 1 or 2 FP64 refinement
+sweeps per stage (precision 2), and FP32 factor storage (precision 3: FP64
+arithmetic, off-diagonal factor entries stored as f32). The accuracy criterion: the p99.9 over cells of each cell's max
 relative error against the FP64 run, over species above atol/rtol, is at most
 1e-6. value = accepted cell-steps / wall seconds (the reference's definition,
 src/tests/test_kinetics.cpp:1037-1042) of the fastest variant that meets it.

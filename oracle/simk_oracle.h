@@ -4,6 +4,11 @@
  *
  * Every function names the reference file:line it follows. Mechanisms are the
  * reference KinMech byte layout at capacities `caps` (simk_mech_layout).
+ *
+ * oracle_factor_store32 / oracle_advance_store32 restate this build's FP32
+ * factor-storage form (GpuKinLaunch precision 3), which has no reference
+ * counterpart: the reference's FP64 advance with each finished factor row's
+ * off-diagonal entries rounded to f32.
  */
 #ifndef SIMK_ORACLE_H
 #define SIMK_ORACLE_H

@@ -85,6 +85,40 @@ void pair_loads(std::vector<Op> &ops, size_t lookahead = 4, const char *pair_typ
     }
 }
 
+// f32 slots in groups of four ([quad][64 lanes][4]): an op loading slot 4q
+// and ops loading 4q + 1 .. 4q + 3 within the next `lookahead` ops (no
+// barrier between) share one 16-byte load.
+void quad_loads(std::vector<Op> &ops, size_t lookahead, const char *quad_type, const char *accessor) {
+    static const char *kPart[4] = {".x", ".y", ".z", ".w"};
+    for (size_t i = 0; i < ops.size(); ++i) {
+        Op &a = ops[i];
+        if (a.slot < 0 || (a.slot & 3) != 0 || a.from >= 0 || !a.type.empty() || a.barrier) continue;
+        size_t at[4] = {i, 0, 0, 0};
+        int found = 1;
+        for (size_t j = i + 1; j < ops.size() && j <= i + lookahead && found < 4; ++j) {
+            Op &b = ops[j];
+            if (b.barrier) break;
+            const int k = b.slot - a.slot;
+            if (k < 1 || k > 3 || b.from >= 0 || !b.type.empty() || at[k]) continue;
+            at[k] = j;
+            ++found;
+        }
+        if (found < 4) continue;
+        char buf[32];
+        snprintf(buf, sizeof(buf), "%s(%d)", accessor, a.slot >> 2);
+        a.load = buf;
+        a.type = quad_type;
+        a.part = kPart[0];
+        a.immutable = false;
+        for (int k = 1; k < 4; ++k) {
+            Op &b = ops[at[k]];
+            b.load.clear();
+            b.from = (int)i;
+            b.part = kPart[k];
+        }
+    }
+}
+
 // Emits ops with their loads software-pipelined `batch` ops ahead: the loads
 // of ops [b + batch, b + 2 batch) are issued before the statements of
 // [b, b + batch) run. Every loaded value is its own SSA name, except that an
@@ -221,7 +255,8 @@ static Layout layout_s32(const Mech &m, const KinMethod &me, const GenOptions &o
         L.lu32off = L.scp + m.n;
     }
     L.rhsoff = L.lu32off;
-    L.slots = L.lu32off + (t + 1) / 2;
+    // f32 region: [quad][64 lanes][4] with pairs (16 bytes, two f64 slots, per quad), else [slot][64 lanes]
+    L.slots = L.lu32off + (L.pairs ? 2 * ((t + 3) / 4) : (t + 1) / 2);
     return L;
 }
 
@@ -367,8 +402,9 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     const u32 kcp = Lo.kcp, scp = Lo.scp, ws_slots = Lo.slots;
     const bool pairs = Lo.pairs;
     const bool lu_pairs = pairs && !s32;   // FP64 LU entries in paired slots
-    // FP32 factor storage with pairs: the f32 region is [pair][64 lanes][2],
-    // two entries per 8-byte lane access (WQ2 / WQS2)
+    // FP32 factor storage with pairs: the f32 region is [quad][64 lanes][4],
+    // four entries per 16-byte lane access (WQ4 / WQS4), two adjacent ones
+    // per 8-byte access (WQ2 / WQS2)
     const bool q_pairs = pairs && s32;
     w("struct %s {\n", sname);
     w("    static constexpr simk_u32 N = %u, R = %u;\n", n, m.R);
@@ -386,7 +422,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // which is what loading those rows would give.
     w("    simk_u32 srcnz;\n");
     if (mixed || (s32 && !q_pairs)) w("    SimkRows32 w32;\n");
-    if (q_pairs) w("    SimkRowPairs32 wq;\n");
+    if (q_pairs) w("    SimkRowQuads32 wq;\n");
     if (s32) {
         w("    simk_u32 ln;\n");
         w("    SIMK_DEV double P(simk_u32 i) const { return simk_lds[i * SIMK_WAVE + ln]; }\n");
@@ -447,6 +483,8 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         w("    SIMK_DEV void W32S(simk_u32 i, float v) const { wq.st(i, v); }\n");
         w("    SIMK_DEV SimkF2 WQ2(simk_u32 pair) const { return wq.ld2(pair); }\n");
         w("    SIMK_DEV void WQS2(simk_u32 pair, float a, float b) const { wq.st2(pair, a, b); }\n");
+        w("    SIMK_DEV SimkF4 WQ4(simk_u32 quad) const { return wq.ld4(quad); }\n");
+        w("    SIMK_DEV void WQS4(simk_u32 quad, float a, float b, float c, float d) const { wq.st4(quad, a, b, c, d); }\n");
     } else if (mixed || s32) {
         w("    SIMK_DEV float W32(simk_u32 i) const { return w32.ld(i); }\n");
         w("    SIMK_DEV void W32S(simk_u32 i, float v) const { w32.st(i, v); }\n");
@@ -601,6 +639,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         return q_pairs && diag_row[idx] == ~0u && Lo.lds_rank[idx] == ~0u ? (int)Lo.lu_slot[idx] : -1;
     };
     auto q_pair_loads = [&](std::vector<Op> &ops) {
+        quad_loads(ops, 12, "SimkF4", "WQ4");
         pair_loads(ops, 4, "SimkF2", "WQ2");
         for (Op &op : ops)
             if (op.slot >= 0 && op.type.empty() && op.from < 0 && !op.load.empty()) op.type = "float";
@@ -774,7 +813,11 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
                     std::sort(ws_row.begin(), ws_row.end());
                     for (size_t q = 0; q < ws_row.size(); ++q) {
                         const u32 sl = ws_row[q].first;
-                        if (sl % 2 == 0 && q + 1 < ws_row.size() && ws_row[q + 1].first == sl + 1) {
+                        if (sl % 4 == 0 && q + 3 < ws_row.size() && ws_row[q + 3].first == sl + 3) {
+                            snprintf(buf, sizeof(buf), "WQS4(%u, (float)a%u, (float)a%u, (float)a%u, (float)a%u);", sl / 4,
+                                     ws_row[q].second, ws_row[q + 1].second, ws_row[q + 2].second, ws_row[q + 3].second);
+                            q += 3;
+                        } else if (sl % 2 == 0 && q + 1 < ws_row.size() && ws_row[q + 1].first == sl + 1) {
                             snprintf(buf, sizeof(buf), "WQS2(%u, (float)a%u, (float)a%u);", sl / 2, ws_row[q].second,
                                      ws_row[q + 1].second);
                             ++q;

@@ -230,6 +230,47 @@ struct SimkRowPairs32 {
     }
 };
 
+// f32 workspace slots in groups of four per lane, [quad][64 lanes][4] f32:
+// four slots in one 16-byte access, two adjacent ones (2k, 2k + 1) in one
+// 8-byte access (the FP32 factor-storage LU).
+struct SimkF4 {
+    float x, y, z, w;
+};
+
+struct SimkRowQuads32 {
+    __amdgpu_buffer_rsrc_t rs;
+    simk_u32 voff;           // lane * 16
+    static constexpr simk_u32 kQuadBytes = SIMK_WAVE * 16u;
+    SIMK_DEV void init(const void *base, simk_u32 lane, simk_u32 bytes) {
+        rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)bytes, SIMK_RSRC_FLAGS);
+        voff = lane * 16u;
+    }
+    static SIMK_DEV int at(simk_u32 slot) { return (int)((slot >> 2) * kQuadBytes + (slot & 3u) * 4u); }
+    SIMK_DEV float ld(simk_u32 slot) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, at(slot), 0));
+    }
+    SIMK_DEV void st(simk_u32 slot, float x) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(simk_u32, x), rs, (int)voff, at(slot), 0);
+    }
+    // slots 2 pair and 2 pair + 1
+    SIMK_DEV SimkF2 ld2(simk_u32 pair) const {
+        return __builtin_bit_cast(SimkF2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, at(2u * pair), 0));
+    }
+    SIMK_DEV void st2(simk_u32 pair, float a, float b) const {
+        __builtin_amdgcn_raw_buffer_store_b64(simk_v2u{__builtin_bit_cast(simk_u32, a), __builtin_bit_cast(simk_u32, b)},
+                                              rs, (int)voff, at(2u * pair), 0);
+    }
+    // slots 4 quad .. 4 quad + 3
+    SIMK_DEV SimkF4 ld4(simk_u32 quad) const {
+        return __builtin_bit_cast(SimkF4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)(quad * kQuadBytes), 0));
+    }
+    SIMK_DEV void st4(simk_u32 quad, float a, float b, float c, float d) const {
+        __builtin_amdgcn_raw_buffer_store_b128(simk_v4u{__builtin_bit_cast(simk_u32, a), __builtin_bit_cast(simk_u32, b),
+                                                        __builtin_bit_cast(simk_u32, c), __builtin_bit_cast(simk_u32, d)},
+                                               rs, (int)voff, (int)(quad * kQuadBytes), 0);
+    }
+};
+
 // The same for f32 arrays (lane stride 4 bytes): the mixed-precision LU.
 struct SimkRows32 {
     __amdgpu_buffer_rsrc_t rs;

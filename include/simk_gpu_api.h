@@ -150,7 +150,7 @@ typedef struct SimkGenOptions {
                             40 KiB in f32 delta lanes; none for refined mixed kernels; ~0u: none; at most
                             160 KiB). FP32 factor storage: the lane's LDS budget in f32 words, the FP64
                             pivots (two words each, always in LDS) first, off-diagonal entries in the rest
-                            (0: 320 words = 80 KiB per wave) */
+                            (0: 304 words = 76 KiB per wave) */
     u32 fun_in_jac;      /* 0: default (on); 1: the -J pass also evaluates f(y) for stage 0 (one rate-constant
                             pass fewer per attempt); ~0u: stage 0 evaluates f(y) itself */
     u32 pair_slots;      /* FP64 kernels: 1: workspace slots in adjacent pairs per lane ([pair][64 lanes][2]), the

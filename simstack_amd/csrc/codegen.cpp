@@ -331,7 +331,11 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
 // mixed kernel is slower with LDS entries.
 u32 auto_lds(const GenOptions &o) {
     if (o.precision == 1) return 160u;
-    if (o.precision == 3) return 320u;   // f32 words per lane: 80 KiB per wave, pivots included
+    // FP32 factor storage, f32 words per lane with the pivots: 304 (76 KiB per
+    // wave, 208 off-diagonal entries of 48/150 in LDS) runs 2-3 % faster than
+    // 320 (80 KiB) on 48/150 at 262,144 cells, in repeated runs; 288 to 316
+    // scatter between the two; above 320 one wave fits a CU and loses 35 %.
+    if (o.precision == 3) return 304u;
     if (o.precision == 2) return o.sweeps ? 0u : 200u;
     return 160u;
 }

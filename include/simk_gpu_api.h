@@ -156,6 +156,9 @@ typedef struct SimkGenOptions {
     u32 pair_slots;      /* FP64 kernels: 1: workspace slots in adjacent pairs per lane ([pair][64 lanes][2]), the
                             LU entries in the triangular solves' read order, so that two slots move in one
                             16-byte access; ~0u: one 8-byte slot per access; 0: default (paired) */
+    u32 k_pairs;         /* paired layouts: 1: a second copy of the cell's rate constants in paired slots, which the
+                            f passes of stages read two per 16-byte access (the -J pass keeps the single copy);
+                            ~0u: none; 0: default (on for one-mechanism kernels) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 

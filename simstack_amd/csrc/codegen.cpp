@@ -203,6 +203,9 @@ struct Layout {
     // slot of one outside LDS; pivots are in LDS by row.
     bool s32 = false;
     std::vector<u32> lds_rank;
+    // paired layouts: a second copy of the rate constants in the paired
+    // region (kpoff, even; ~0u: none), which f passes read in pairs
+    u32 kpoff = ~0u;
     // Paired FP64 workspace: [paired region: LU entries lds.. in the solves'
     // read order (forward: each row's lower entries; backward, rows n-1..0:
     // upper entries then the pivot), K stages and f at stride np (n rounded
@@ -245,6 +248,10 @@ static Layout layout_s32(const Mech &m, const KinMethod &me, const GenOptions &o
     L.foff = stored_stages * L.np;
     if (L.pairs) {
         L.paired = L.foff + L.np;
+        if (o.k_pairs) {
+            L.kpoff = L.paired;
+            L.paired += m.R + (m.R & 1u);
+        }
         L.kcp = 0;
         L.scp = m.R;
         L.lu32off = L.paired + m.R + m.n;
@@ -311,6 +318,10 @@ static Layout layout(const Mech &m, const KinMethod &me, const GenOptions &o) {
         L.koff = t;
         L.foff = L.koff + stored_stages * L.np;
         L.paired = L.foff + L.np;
+        if (o.k_pairs) {
+            L.kpoff = L.paired;
+            L.paired += m.R + (m.R & 1u);
+        }
         L.rhsoff = L.lu32off = L.paired;
         L.kcp = 0;
         L.scp = m.R;
@@ -361,6 +372,16 @@ u32 auto_fun_in_jac(const GenOptions &) { return 1u; }
 // kernel at 221,000 cells 318 -> 286 ms (+11 %). Mixed-precision and f32
 // delta kernels keep their own layouts.
 u32 auto_pair(const GenOptions &o) { return o.precision == 0 || o.precision == 3 ? 1u : 0u; }
+
+// Paired layouts, f passes reading a paired copy of the rate constants (bits
+// unchanged): 48/150 at 262,144 cells, the FP64 kernel 4 % faster (1,672 ->
+// 1,601 ms) and the FP32 factor-storage kernel 3 % faster (1,121 -> 1,087
+// ms); the three-mechanism kernel at 221,000 cells 1.5 % faster. The extra R
+// slots per lane fit the cache-aware grid's budget at the same waves.
+u32 auto_k_pairs(const GenOptions &o, bool multi) {
+    (void)multi;
+    return o.precision == 0 || o.precision == 3 ? 1u : 0u;
+}
 
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
@@ -462,7 +483,16 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     w("    }\n");
     w("    SIMK_DEV void set_cell(const SimkKinArgs &, simk_u32 cell) {\n");
     w("        kr.voff = sr.voff = pr.voff = cell * 8u;\n");
-    w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) ws.st(%u + r, kr.ld(r));\n", kcp);
+    if (Lo.kpoff != ~0u) {
+        // both copies: single slots for the -J pass, pairs for the f passes
+        w("#pragma unroll 4\n        for (simk_u32 r = 0; r + 1 < R; r += 2) {\n"
+          "            const double a = kr.ld(r), b = kr.ld(r + 1);\n"
+          "            ws.st(%u + r, a); ws.st(%u + r + 1, b); WS2((%u + r) / 2, a, b);\n        }\n", kcp, kcp, Lo.kpoff);
+        if (m.R & 1u)
+            w("        { const double a = kr.ld(R - 1); ws.st(%u + R - 1, a); WS(%u + R - 1, a); }\n", kcp, Lo.kpoff);
+    } else {
+        w("#pragma unroll 8\n        for (simk_u32 r = 0; r < R; ++r) ws.st(%u + r, kr.ld(r));\n", kcp);
+    }
     w("        srcnz = 0;\n");
     w("#pragma unroll 8\n        for (simk_u32 s = 0; s < N; ++s) { const double v = sr.ld(s); srcnz |= "
       "(simk_u32)(__builtin_bit_cast(simk_u64, v) != 0); ws.st(%u + s, v); }\n", scp);
@@ -567,9 +597,14 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
         for (u32 r = 0; r < m.R; ++r) {
             const Reaction &x = m.rx[r];
             Op op;
-            snprintf(buf, sizeof(buf), "K(%u)", r);
+            if (Lo.kpoff != ~0u) {
+                snprintf(buf, sizeof(buf), "W(%u)", Lo.kpoff + r);
+                op.slot = (int)(Lo.kpoff + r);
+            } else {
+                snprintf(buf, sizeof(buf), "K(%u)", r);
+                op.immutable = true;
+            }
             op.load = buf;
-            op.immutable = true;
             std::string st = "{ const double rate = %s";
             for (u32 p = 0; p < x.reactant_count; ++p) {
                 snprintf(buf, sizeof(buf), " * %s.v[%u]", yv, x.reactant[p]);
@@ -585,6 +620,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
             op.stmt = st;
             ops.push_back(op);
         }
+        if (Lo.kpoff != ~0u) pair_loads(ops);
         return ops;
     };
     const char *src_init = "        if (any_src()) {\n#pragma unroll\n            for (simk_u32 s = 0; s < N; ++s) %s.v[s] = S(s);\n"

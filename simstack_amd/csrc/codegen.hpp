@@ -36,7 +36,10 @@ struct GenOptions {
     static constexpr u32 kAutoBatch = 0xFFFF;
     static constexpr u32 kAutoReuse = 0xFFFF;
     static constexpr u32 kAutoFunInJac = 0xFFFF;
+    u32 k_pairs = kAutoKPairs;   // paired layouts: 1: a second, paired copy of the rate constants that the f
+                                 // passes read two per 16-byte access; 0: none (kAutoKPairs: auto_k_pairs)
     static constexpr u32 kAutoPair = 0xFFFF;
+    static constexpr u32 kAutoKPairs = 0xFFFF;
 };
 
 struct GenResult {
@@ -68,6 +71,7 @@ u32 auto_reuse(const GenOptions &o, bool multi = false);
 // The default for f(y) in the -J pass: on.
 u32 auto_fun_in_jac(const GenOptions &o);
 u32 auto_pair(const GenOptions &o);
+u32 auto_k_pairs(const GenOptions &o, bool multi = false);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

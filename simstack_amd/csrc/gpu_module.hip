@@ -722,6 +722,7 @@ GenOptions resolved_multi(GenOptions o) {
     // kept in registers 8 rows back instead of 16 run 7-10 % faster (the
     // multi-mechanism autotuner's pick; the kernel is at the register limit).
     if (o.reg_window == GenOptions::kDefaultWindow) o.reg_window = 8u;
+    if (o.k_pairs == GenOptions::kAutoKPairs) o.k_pairs = auto_k_pairs(o, true);
     return resolved(o);
 }
 
@@ -731,6 +732,7 @@ GenOptions resolved(GenOptions o) {
     if (o.reuse == GenOptions::kAutoReuse) o.reuse = auto_reuse(o);
     if (o.fun_in_jac == GenOptions::kAutoFunInJac) o.fun_in_jac = auto_fun_in_jac(o);
     if (o.pair_slots == GenOptions::kAutoPair) o.pair_slots = auto_pair(o);
+    if (o.k_pairs == GenOptions::kAutoKPairs) o.k_pairs = auto_k_pairs(o);
     return o;
 }
 
@@ -1279,6 +1281,7 @@ GenOptions gen_options(const SimkGenOptions *options) {
                                           : (options->lds_slots == 0xFFFFFFFFu ? 0u : options->lds_slots);
     o.fun_in_jac = options->fun_in_jac == 0 ? GenOptions::kAutoFunInJac : (options->fun_in_jac == 0xFFFFFFFFu ? 0u : 1u);
     o.pair_slots = options->pair_slots == 0 ? GenOptions::kAutoPair : (options->pair_slots == 0xFFFFFFFFu ? 0u : 1u);
+    o.k_pairs = options->k_pairs == 0 ? GenOptions::kAutoKPairs : (options->k_pairs == 0xFFFFFFFFu ? 0u : 1u);
     return o;
 }
 
@@ -1768,6 +1771,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.lds_slots = n.lds_slots;
     g.gen.fun_in_jac = n.fun_in_jac;
     g.gen.pair_slots = n.pair_slots;
+    g.gen.k_pairs = n.k_pairs;
     g.gen.batch = n.batch;
     unload_s32();   // the FP32 factor-storage kernel is built from these options
     // explicit options are what a later untune() restores
@@ -1854,7 +1858,7 @@ bool same_gen(const GenOptions &a, const GenOptions &b) {
     return a.fence_rows == b.fence_rows && a.waves_per_eu == b.waves_per_eu && a.unroll_stages == b.unroll_stages &&
            a.batch == b.batch && a.precision == b.precision && a.sweeps == b.sweeps &&
            a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots &&
-           a.fun_in_jac == b.fun_in_jac && a.pair_slots == b.pair_slots;
+           a.fun_in_jac == b.fun_in_jac && a.pair_slots == b.pair_slots && a.k_pairs == b.k_pairs;
 }
 
 // Mixed precision (precision 2): the same protocol over the mixed kernel's

@@ -78,7 +78,8 @@ def test_store32_lds_placements_keep_bits(gm, lds):
     info = _device(gm, mech, meth, ctl, a, [0.2, 0.2], lds_slots=lds)
     off = mech.nnz - mech.n
     in_lds = {0xFFFFFFFF: 0, 160: 160 - 2 * mech.n, 0: 304 - 2 * mech.n, 640: off}[lds]
-    assert info.ws_slots == 3 * 48 + 48 + 150 + 48 + 2 * ((off - in_lds + 3) // 4)
+    # K stages and f, the paired copy of the rate constants, the single copy and the sources, the f32 quads
+    assert info.ws_slots == 3 * 48 + 48 + 150 + 150 + 48 + 2 * ((off - in_lds + 3) // 4)
     _oracle(mech, meth, ctl, b, [0.2, 0.2])
     assert compare_bits(a, b) == ZERO
 

@@ -174,9 +174,12 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * keeps the fastest for later FP64 advances of the current mechanism:
  *   1. persistent waves (the cache-aware default, 1, 1.5, 2, 2.5, 3 or 4 per CU);
  *   2. at the best grid, the generator's load-pipeline depth (8, 16), register
- *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48, 96)
- *      and LU entries in LDS (none, 80, 128, 160), one knob at a time;
- *   3. one wave per cell (gpu_kin_set_mode(1)) instead of one lane per cell.
+ *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48, 96),
+ *      a two-waves-per-SIMD register budget, single instead of paired workspace
+ *      slots, the paired rate-constant copy off, and LU entries in LDS (none,
+ *      80, 128, 160), one knob at a time;
+ *   3. teams of 64, 32, 16 or 8 lanes per cell (the cooperative form) instead
+ *      of one lane per cell.
  * Every candidate computes the same bits; only time differs. The tuned
  * settings last until the next gpu_kin_set_mech / simk_kin_set_mechs, which
  * restores the caller's own.
@@ -186,7 +189,7 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * per CU as that LDS share allows, then at the best pair the load-pipeline
  * depth (8, 12, 20, 24, 28, 32), register window (4, 16), rate-constant reuse
  * (none, 48, 96, 384) and unrolled stages. simk_kin_set_gen_options drops these
- * settings. Precision 1 is refused (hipErrorInvalidValue). */
+ * settings. Precisions 1 and 3 are refused (hipErrorInvalidValue). */
 typedef struct SimkTuneResult {
     u32 mode;            /* 0 one lane per cell, 1 one wave per cell */
     u32 lanes_per_cell;  /* 1 or 64 */

@@ -2094,6 +2094,12 @@ static s32 autotune_impl(const GpuKinLaunch *L, u32 sample_cells, f64 duration, 
         o.pair_slots = resolved(best_gen).pair_slots ? 0u : 1u;
         e = consider(0, best_grid, o);
     }
+    // the paired copy of the rate constants for the f passes, off (or on)
+    if (e == hipSuccess && resolved(best_gen).pair_slots) {
+        GenOptions o = best_gen;
+        o.k_pairs = resolved(best_gen).k_pairs ? 0u : 1u;
+        e = consider(0, best_grid, o);
+    }
     // LU entries in LDS: none, and shares for 4, 3 and 2 waves per CU
     const u32 lds_now = lds_entries(g.mech, resolved(best_gen));
     const u32 ldss[] = {0, 80, 128, 160};
@@ -2468,6 +2474,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     knob([](GenOptions &o) { o.lds_slots = 80; });
     knob([](GenOptions &o) { o.lds_slots = 128; });
     knob([](GenOptions &o) { o.pair_slots = resolved(o).pair_slots ? 0u : 1u; });
+    knob([](GenOptions &o) { o.k_pairs = resolved_multi(o).k_pairs ? 0u : 1u; });
     // teams of 64, 32, 16 and 8 lanes per cell (the cooperative form)
     const u32 saved_multi_team = g.multi_team;
     u32 best_team = 0;

@@ -1,6 +1,6 @@
 """Edge shapes of the device advance against the oracle, bit for bit: a single
 cell and partial waves (FP64 generated and table-walk kernels, f32 delta
-lanes), a prefix of the block (GpuKinLaunch.lane_count below the cell count:
+lanes, FP32 factor storage), a prefix of the block (GpuKinLaunch.lane_count below the cell count:
 the cells past it keep every bit), and a zero-length advance with pulses due
 at t = 0 (kin_advance applies them and takes no step, kin_solver.h:322-347)."""
 import numpy as np
@@ -26,7 +26,8 @@ def _device(gm, mech, meth, ctl, z, duration, kernel=1, precision=0, lane_count=
 
 
 def _oracle(mech, meth, ctl, z, duration, precision=0, lanes=None):
-    adv = oracle_ffi.oracle_advance_delta32 if precision == 1 else oracle_ffi.oracle_advance
+    adv = {1: oracle_ffi.oracle_advance_delta32, 3: oracle_ffi.oracle_advance_store32}.get(precision,
+                                                                                         oracle_ffi.oracle_advance)
     adv(mech, meth, ctl, z, duration, lanes=lanes)
 
 
@@ -37,7 +38,8 @@ def _oracle(mech, meth, ctl, z, duration, precision=0, lanes=None):
 # lane mask across the wave's divergent loop exits, was rewritten for lanes
 # waiting outside the loop. The flag is now recomputed where it is tested.
 @pytest.mark.parametrize("cells", [1, 63, 65, 130])
-@pytest.mark.parametrize("kernel,precision", [(1, 0), (2, 0), (1, 1)], ids=["generated", "table", "f32delta"])
+@pytest.mark.parametrize("kernel,precision", [(1, 0), (2, 0), (1, 1), (1, 3)],
+                         ids=["generated", "table", "f32delta", "fp32store"])
 def test_single_cell_and_partial_waves(gm, cells, kernel, precision):
     sp, mech, z = make_problem("c3c", cells)
     add_pulses(z, seed=cells)
@@ -61,7 +63,7 @@ def test_f32delta_partial_waves_pulses_inside_the_advance(gm, cells):
     assert b.pulses[:cells].sum() > 0
 
 
-@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "f32delta"])
+@pytest.mark.parametrize("precision", [0, 1, 3], ids=["fp64", "f32delta", "fp32store"])
 def test_prefix_lane_count_leaves_the_rest(gm, precision):
     sp, mech, z = make_problem("c3c", 200)
     meth, ctl = problem.method(2), problem.control()
@@ -74,13 +76,14 @@ def test_prefix_lane_count_leaves_the_rest(gm, precision):
     assert compare_bits(a, z, lanes=rest) == ZERO
 
 
-def test_zero_duration_applies_due_pulses_only(gm):
+@pytest.mark.parametrize("precision", [0, 3], ids=["fp64", "fp32store"])
+def test_zero_duration_applies_due_pulses_only(gm, precision):
     sp, mech, z = make_problem("c3c", 96)
     add_pulses(z, seed=11, frac=1.0)
     z.pulse_first[:48] = 0.0
     meth, ctl = problem.method(2), problem.control()
     a, b = z.copy(), z.copy()
-    _device(gm, mech, meth, ctl, a, 0.0)
-    _oracle(mech, meth, ctl, b, 0.0)
+    _device(gm, mech, meth, ctl, a, 0.0, precision=precision)
+    _oracle(mech, meth, ctl, b, 0.0, precision)
     assert compare_bits(a, b) == ZERO
     assert b.steps[:96].sum() == 0

@@ -86,11 +86,12 @@ def test_wave_delta32_repeats_and_60x200(gm):
     assert compare_bits(a, c) == ZERO
 
 
-def test_wave_mode_rejects_mixed_precision(gm):
+@pytest.mark.parametrize("precision", [2, 3], ids=["mixed", "fp32store"])
+def test_wave_mode_rejects_mixed_precision(gm, precision):
     sp, mech, z = make_problem("c1", 64)
     meth, ctl = problem.method(2), problem.control()
     with pytest.raises(gpu.GpuError):
-        _run(gm, mech, meth, ctl, z.copy(), [0.01], precision=2)
+        _run(gm, mech, meth, ctl, z.copy(), [0.01], precision=precision)
 
 
 @pytest.mark.parametrize("team", [8, 16, 32])

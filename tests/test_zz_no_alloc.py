@@ -29,7 +29,7 @@ def _run(gm, mech, meth, ctl, z, duration, repeats, static, precision=0):
     return info
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2], ids=["fp64", "f32delta", "mixed"])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3], ids=["fp64", "f32delta", "mixed", "fp32store"])
 def test_static_advance_equals_advance_n_twice(gm, precision):
     sp, mech, z1 = make_problem("c2", 9000)
     _, _, z2 = make_problem("c2", 700, first_cell=20000)
@@ -63,9 +63,10 @@ def test_static_advance_before_the_kernel_is_loaded(gm):
     gm.set_mech(mech, meth, ctl)
     dz = gpu.DeviceZones(gm, z.copy())
     dz.upload()
-    L = dz.launch_desc(0.1, precision=2)
-    ms = C.c_float()
-    rc = gm.lib.simk_kin_advance_static(C.byref(L), 1, C.byref(ms))
-    assert rc != 0 and gm.lib.gpu_error_string(rc)
-    dz.download(mutable_only=False)
-    assert dz.z.block.tobytes() == z.block.tobytes()
+    for precision in (2, 3):
+        L = dz.launch_desc(0.1, precision=precision)
+        ms = C.c_float()
+        rc = gm.lib.simk_kin_advance_static(C.byref(L), 1, C.byref(ms))
+        assert rc != 0 and gm.lib.gpu_error_string(rc)
+        dz.download(mutable_only=False)
+        assert dz.z.block.tobytes() == z.block.tobytes()

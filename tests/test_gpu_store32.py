@@ -84,6 +84,20 @@ def test_store32_lds_placements_keep_bits(gm, lds):
     assert compare_bits(a, b) == ZERO
 
 
+def test_store32_single_slots_keep_bits(gm):
+    """Single 8-byte workspace slots (no pairs, no quads, one copy of the rate
+    constants): another layout of the same values and operations."""
+    print("[case] store32 c2 1024 cells x 2 x 0.2 s, single slots", flush=True)
+    sp, mech, z = make_problem("c2", 1024)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    info = _device(gm, mech, meth, ctl, a, [0.2, 0.2], pair_slots=0xFFFFFFFF)
+    off = mech.nnz - mech.n
+    assert info.ws_slots == 3 * 48 + 48 + 150 + 48 + (off - (304 - 2 * mech.n) + 1) // 2
+    _oracle(mech, meth, ctl, b, [0.2, 0.2])
+    assert compare_bits(a, b) == ZERO
+
+
 def test_store32_c2_fused_repeats(gm):
     """4,096 cells of the 48/150 workload, 3 fused operator-split advances of 1 s."""
     print("[case] store32 c2 4096 cells x 3 x 1 s fused", flush=True)

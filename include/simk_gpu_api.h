@@ -188,8 +188,10 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * default, 320, 240, 160), each at the cache-aware grid and at as many waves
  * per CU as that LDS share allows, then at the best pair the load-pipeline
  * depth (8, 12, 20, 24, 28, 32), register window (4, 16), rate-constant reuse
- * (none, 48, 96, 384) and unrolled stages. simk_kin_set_gen_options drops these
- * settings. Precisions 1 and 3 are refused (hipErrorInvalidValue). */
+ * (none, 48, 96, 384) and unrolled stages. With L->precision 3 the same for the
+ * FP32 factor-storage kernel, its LDS budgets being the default (304), 320,
+ * 288, 256 and 208 f32 words per lane, pivots included. simk_kin_set_gen_options
+ * drops these settings. Precision 1 is refused (hipErrorInvalidValue). */
 typedef struct SimkTuneResult {
     u32 mode;            /* 0 one lane per cell, 1 one wave per cell */
     u32 lanes_per_cell;  /* 1 or 64 */

@@ -278,6 +278,10 @@ struct State {
     bool mix_tuned = false;
     GenOptions mix_gen;
     u32 mix_grid = 0;
+    // The same for the FP32 factor-storage kernel (precision 3).
+    bool s32_tuned = false;
+    GenOptions s32_gen;
+    u32 s32_grid = 0;
     JitKernel jit_s32;         // FP32 factor-storage kernel, compiled on the first precision-3 launch
     bool jit_s32_ok = false;
     JitKernel jit_d32;         // generated f32 delta kernel, compiled on the first precision-1 launch
@@ -784,6 +788,10 @@ void untune() {
         g.mix_tuned = false;
         unload_mixed();
     }
+    if (g.s32_tuned) {
+        g.s32_tuned = false;
+        unload_s32();
+    }
     if (!g.tuned) return;
     g.gen = g.user_gen;
     g.mode = g.user_mode;
@@ -1173,7 +1181,7 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
         // the current generator options.
         if (!g.jit_s32_ok) {
             if (g.use_scratch) return hipErrorNotReady;   // loading a kernel allocates
-            GenOptions so = g.gen;
+            GenOptions so = g.s32_tuned ? g.s32_gen : g.gen;
             so.precision = 3;
             g.jit_s32_ok = load_generated(so, g.jit_s32);
             if (!g.jit_s32_ok) return hipErrorInvalidImage;
@@ -1191,7 +1199,9 @@ hipError_t launch(const GpuKinLaunch *L, u32 repeats) {
         return hipErrorInvalidValue;
     const u32 needed = (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
     const int per_cu = use_gen ? jk->blocks_per_cu : g.blocks_per_cu_table;
-    const u32 grid = L->precision == 2 && g.mix_tuned ? g.mix_grid : g.grid_override;
+    const u32 grid = L->precision == 2 && g.mix_tuned   ? g.mix_grid
+                     : L->precision == 3 && g.s32_tuned ? g.s32_grid
+                                                        : g.grid_override;
     const u32 resident = resident_waves(per_cu, (u64)slots * SIMK_WAVE * 8, grid);
     u32 waves = needed < resident ? needed : resident;
     double *ws = nullptr;
@@ -1780,6 +1790,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     // explicit options replace tuned mixed-precision settings
     if (sw != g.sweeps || g.mix_tuned) unload_mixed();
     g.mix_tuned = false;
+    g.s32_tuned = false;
     g.sweeps = sw;
     return 0;
 }
@@ -1861,43 +1872,58 @@ bool same_gen(const GenOptions &a, const GenOptions &b) {
            a.fun_in_jac == b.fun_in_jac && a.pair_slots == b.pair_slots && a.k_pairs == b.k_pairs;
 }
 
-// Mixed precision (precision 2): the same protocol over the mixed kernel's
-// own knobs. The LU entries held in LDS set how many waves fit a CU, so each
-// LDS share is timed at the cache-aware grid and at every per-CU wave count
-// that share allows; then, at the best pair, load-pipeline depth, register
-// window, rate-constant reuse and stage unrolling one at a time. The winner
-// serves later mixed advances only; FP64 settings are untouched.
-s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResult *out) {
+// Mixed precision (precision 2) and FP32 factor storage (precision 3): the
+// same protocol over that kernel's own knobs. The LU entries held in LDS set
+// how many waves fit a CU, so each LDS share is timed at the cache-aware grid
+// and at every per-CU wave count that share allows; then, at the best pair,
+// load-pipeline depth, register window, rate-constant reuse and stage
+// unrolling one at a time. The winner serves later advances of that precision
+// only; FP64 settings are untouched.
+s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResult *out, u32 prec) {
     hipError_t e = hipStreamSynchronize(g.stream);
     if (e != hipSuccess) return err_code(e);
+    const bool p2 = prec == 2;
+    bool &tuned = p2 ? g.mix_tuned : g.s32_tuned;
+    GenOptions &tgen = p2 ? g.mix_gen : g.s32_gen;
+    u32 &tgrid = p2 ? g.mix_grid : g.s32_grid;
+    JitKernel &jk = p2 ? g.jit_mixed : g.jit_s32;
+    bool &jk_ok = p2 ? g.jit_mixed_ok : g.jit_s32_ok;
+    auto unload = [&]() {
+        if (p2) unload_mixed();
+        else unload_s32();
+    };
+    // the options a kernel of this precision is generated from
+    auto kernel_options = [&](const GenOptions &opt) {
+        GenOptions mo = opt;
+        mo.precision = prec;
+        if (p2) mo.sweeps = g.sweeps;
+        return mo;
+    };
     TuneSample S;
     e = tune_sample(g.mech, L, cells, S);
     if (duration > 0.0) S.orig.duration = S.work.duration = duration;
-    S.work.precision = S.orig.precision = 2;
-    const bool saved_tuned = g.mix_tuned;
-    const GenOptions saved_gen = g.mix_tuned ? g.mix_gen : g.gen;
-    const u32 saved_grid = g.mix_tuned ? g.mix_grid : g.grid_override;
+    S.work.precision = S.orig.precision = prec;
+    const bool saved_tuned = tuned;
+    const GenOptions saved_gen = tuned ? tgen : g.gen;
+    const u32 saved_grid = tuned ? tgrid : g.grid_override;
     const u32 cus = g.cus ? g.cus : 1;
     f32 best = 1e30f, t = 0.0f;
     GenOptions best_gen = saved_gen, loaded = saved_gen;
     u32 best_grid = saved_grid;
     bool have_loaded = false;
-    g.mix_tuned = true;
+    tuned = true;
     auto consider = [&](u32 grid, const GenOptions &opt) -> hipError_t {
         if (!have_loaded || !same_gen(opt, loaded)) {
-            GenOptions mo = opt;
-            mo.precision = 2;
-            mo.sweeps = g.sweeps;
             JitKernel k;
-            if (!load_generated(mo, k)) return hipSuccess;   // variant does not build: skip it
-            unload_mixed();
-            g.jit_mixed = k;
-            g.jit_mixed_ok = true;
+            if (!load_generated(kernel_options(opt), k)) return hipSuccess;   // variant does not build: skip it
+            unload();
+            jk = k;
+            jk_ok = true;
             loaded = opt;
             have_loaded = true;
         }
-        g.mix_gen = opt;
-        g.mix_grid = grid;
+        tgen = opt;
+        tgrid = grid;
         hipError_t r = tune_time(S, 2, t);
         if (r != hipSuccess) return r;
         out->candidates++;
@@ -1909,18 +1935,21 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
         }
         return hipSuccess;
     };
-    // 1. LDS share x grid
+    // 1. LDS share x grid (precision 2: f32 LU entries; precision 3: the
+    // lane's f32 words, the FP64 pivots included)
     e = e == hipSuccess ? consider(saved_grid, saved_gen) : e;
-    const u32 ldss[] = {GenOptions::kAutoLds, 320, 240, 160};
-    for (u32 l : ldss) {
+    const u32 ldss2[] = {GenOptions::kAutoLds, 320, 240, 160};
+    const u32 ldss3[] = {GenOptions::kAutoLds, 320, 288, 256, 208};
+    const u32 *ldss = p2 ? ldss2 : ldss3;
+    const u32 nlds = p2 ? 4u : 5u;
+    for (u32 li = 0; li < nlds; ++li) {
         GenOptions o = saved_gen;
-        o.lds_slots = l;
-        GenOptions ro = o;
-        ro.precision = 2;
-        ro.sweeps = g.sweeps;
-        const u32 entries = lds_entries(g.mech, resolved(ro));
-        // waves per CU that this LDS share allows (160 KiB per CU, f32 entries of 64 lanes)
-        const u32 per_cu = entries ? (u32)((160u * 1024u) / (entries * 4u * SIMK_WAVE)) : 4u;
+        o.lds_slots = ldss[li];
+        const GenOptions ro = resolved(kernel_options(o));
+        const u32 entries = lds_entries(g.mech, ro);
+        // LDS bytes per wave, and the waves per CU they allow (160 KiB per CU)
+        const u32 bytes = (p2 ? entries * 4u : g.mech.n * 8u + entries * 4u) * SIMK_WAVE;
+        const u32 per_cu = bytes ? (u32)((160u * 1024u) / bytes) : 4u;
         const u32 grids[] = {0, cus * (per_cu < 4 ? per_cu : 4)};
         for (u32 gr : grids)
             if (e == hipSuccess) e = consider(gr, o);
@@ -1954,26 +1983,24 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
     }
     if (S.mem) (void)hipFree(S.mem);
     if (e != hipSuccess) {
-        g.mix_tuned = saved_tuned;
-        g.mix_gen = saved_gen;
-        g.mix_grid = saved_grid;
-        unload_mixed();
+        tuned = saved_tuned;
+        tgen = saved_gen;
+        tgrid = saved_grid;
+        unload();
         return err_code(e);
     }
-    g.mix_gen = best_gen;
-    g.mix_grid = best_grid;
-    if (!have_loaded || !same_gen(best_gen, loaded)) unload_mixed();   // reloaded on the next mixed launch
-    GenOptions ro = best_gen;
-    ro.precision = 2;
-    ro.sweeps = g.sweeps;
+    tgen = best_gen;
+    tgrid = best_grid;
+    if (!have_loaded || !same_gen(best_gen, loaded)) unload();   // reloaded on the next launch of that precision
+    const GenOptions ro = resolved(kernel_options(best_gen));
     out->mode = 0;
     out->lanes_per_cell = 1;
     out->waves = best_grid;
-    out->batch = resolved(ro).batch;
+    out->batch = ro.batch;
     out->reg_window = best_gen.reg_window;
     out->unroll_stages = best_gen.unroll_stages;
-    out->reuse = resolved(ro).reuse;
-    out->lds_slots = lds_entries(g.mech, resolved(ro));
+    out->reuse = ro.reuse;
+    out->lds_slots = lds_entries(g.mech, ro);
     out->best_ms = best;
     out->sample_cells = cells;
     return 0;
@@ -1994,14 +2021,14 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
 }
 
 static s32 autotune_impl(const GpuKinLaunch *L, u32 sample_cells, f64 duration, SimkTuneResult *out) {
-    if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision == 1 || L->precision > 2)
+    if (!g.have_mech || !L || !out || !launch_ok(L) || L->precision == 1 || L->precision > 3)
         return err_code(hipErrorInvalidValue);
-    if (L->precision == 2) {
+    if (L->precision >= 2) {
         memset(out, 0, sizeof(*out));
         const u32 lanes = L->lane_count ? L->lane_count : L->padded;
         const u32 cells = sample_cells && sample_cells < lanes ? sample_cells : lanes;
         if (cells == 0) return err_code(hipErrorInvalidValue);
-        return autotune_mixed(L, cells, duration, out);
+        return autotune_mixed(L, cells, duration, out, L->precision);
     }
     memset(out, 0, sizeof(*out));
     const u32 lanes = L->lane_count ? L->lane_count : L->padded;

@@ -98,6 +98,32 @@ def test_store32_single_slots_keep_bits(gm):
     assert compare_bits(a, b) == ZERO
 
 
+def test_store32_autotune_keeps_bits(gm):
+    """simk_kin_autotune with precision 3 times the FP32 factor-storage
+    kernel's LDS budgets and knobs on a copy of the block (left untouched);
+    the tuned kernel's advance equals the oracle form bit for bit, and the
+    FP64 settings are not touched."""
+    print("[case] store32 autotune, 24/70, 3000 cells", flush=True)
+    sp, mech, z = make_problem("c3c", 3000)
+    meth, ctl = problem.method(2), problem.control()
+    a, b = z.copy(), z.copy()
+    gm.arena_reset()
+    gm.set_kernel(0)
+    gm.set_mech(mech, meth, ctl)
+    dz = gpu.DeviceZones(gm, a)
+    dz.upload()
+    res = dz.autotune(duration=0.02, precision=3)
+    assert res.candidates >= 10 and 0.0 < res.best_ms <= res.default_ms
+    dz.download(mutable_only=False)
+    assert a.block.tobytes() == z.block.tobytes()     # tuning ran on a copy
+    dz.advance(0.2, precision=3)
+    dz.download(mutable_only=False)
+    assert gm.kin_info().kernel == 9
+    _oracle(mech, meth, ctl, b, [0.2])
+    assert compare_bits(a, b) == ZERO
+    gm.set_gen_options()
+
+
 def test_store32_c2_fused_repeats(gm):
     """4,096 cells of the 48/150 workload, 3 fused operator-split advances of 1 s."""
     print("[case] store32 c2 4096 cells x 3 x 1 s fused", flush=True)

@@ -209,18 +209,20 @@ struct Layout {
     // Paired FP64 workspace: [paired region: LU entries lds.. in the solves'
     // read order (forward: each row's lower entries; backward, rows n-1..0:
     // upper entries then the pivot), K stages and f at stride np (n rounded
-    // up to even) | unpaired region: k copy, src copy]; kcp and scp are
-    // offsets in the unpaired region. Otherwise np = n and paired = 0.
+    // up to even), then (k_pairs) the paired copy of the rate constants at
+    // kpoff | unpaired region: k copy, src copy]; kcp and scp are offsets in
+    // the unpaired region. Otherwise np = n and paired = 0.
     bool pairs = false;
     u32 np = 0, paired = 0;
     std::vector<u32> lu_slot;   // workspace slot of LU entry idx >= lds
 };
 
 // FP32 factor storage (precision 3): [paired region: K stages and f at
-// stride np | unpaired region: k copy, src copy | f32 region: off-diagonal LU
-// entries outside LDS in the solves' read order (forward: each row's lower
-// entries; backward, rows n-1..0: upper entries), two per f64 slot]. Without
-// pairs the first two regions are single slots [K stages | f | k copy | src copy].
+// stride np, then (k_pairs) the paired copy of the rate constants | unpaired
+// region: k copy, src copy | f32 region: off-diagonal LU entries outside LDS
+// in the solves' read order (forward: each row's lower entries; backward,
+// rows n-1..0: upper entries), [quad][64 lanes][4]]. Without pairs: single
+// slots [K stages | f | k copy | src copy | f32 entries, two per f64 slot].
 static Layout layout_s32(const Mech &m, const KinMethod &me, const GenOptions &o) {
     Layout L;
     L.s32 = true;

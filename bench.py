@@ -247,6 +247,23 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
         tuned["tuning_wall_s"] = time.perf_counter() - t0
     graph_wall, graph_acc, graph_att, graph_steps = graph_run()
     graph_flops = totals.flops
+    # the same steps in FP32 factor storage (precision 3), one launch per
+    # step, judged by the accuracy criterion against the FP64 run just made
+    ref = [d.z.copy() for d in dzs]
+    reset()
+    gm.lib.gpu_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gm.advance_multi(dzs, duration, precision=3)
+    gm.lib.gpu_sync()
+    s32_wall = time.perf_counter() - t0
+    s32_acc, _, _ = totals()
+    s32_name = gm.kin_info().kernel_name.decode()
+    errs = np.concatenate([cell_errors(d.z, r, ctl.atol, ctl.rtol, d.z.count) for d, r in zip(dzs, ref)])
+    s32 = {"value": s32_acc / s32_wall, "vs_fp64_eager": (s32_acc / s32_wall) / (eager_acc / eager_wall),
+           "kernel": s32_name, "cell_p99_9": float(np.percentile(errs, 99.9)), "max_rel_err": float(errs.max()),
+           "cells_above_1e-6": float(np.mean(errs > 1e-6)),
+           "meets_criterion": bool(np.percentile(errs, 99.9) <= CRITERION_P999)}
     per_cell = np.concatenate(graph_steps)
     same = all(np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d)
                for a, b, c, d in zip(graph_steps, eager_steps, sep_steps, untuned_steps))
@@ -257,7 +274,7 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
                 "attempts_per_s": graph_att / graph_wall, "algorithmic_TFLOPs": graph_flops / graph_wall / 1e12,
                 "kernel_ms_per_step": float(np.mean(kms)),
                 "steps_per_cell_p5_p50_p95": [float(np.percentile(per_cell, q)) for q in (5, 50, 95)],
-                "identical_step_counts_all_modes": bool(same)})
+                "identical_step_counts_all_modes": bool(same), "fp32store": s32})
     return out
 
 

@@ -216,7 +216,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
 typedef struct SimkKinInfo {
     u32 kernel;          /* last launch: 1 generated FP64, 2 table FP64, 3 mixed precision, 4 multi-mechanism,
                             5 f32 delta (table walk), 6 / 7 one wave per cell (FP64 / f32 delta), 8 generated f32 delta,
-                            9 FP32 factor storage */
+                            9 FP32 factor storage, 10 multi-mechanism FP32 factor storage */
     u32 ws_slots;        /* workspace f64 slots per lane */
     u32 waves;           /* persistent waves of the last launch */
     u32 ordered;         /* 1: the last advance took its cells longest first (simk_kin_set_queue_policy) */

@@ -1430,21 +1430,42 @@ GenResult generate_multi(const std::vector<const Mech *> &ms, const KinMethod &m
         sig = fnv1a64(&s1, 8, sig);
     }
     char name[64];
-    snprintf(name, sizeof(name), "simk_multi%u_%016llx", (u32)ms.size(), (unsigned long long)sig);
+    snprintf(name, sizeof(name), o.precision == 3 ? "simk_mf32s%u_%016llx" : "simk_multi%u_%016llx", (u32)ms.size(),
+             (unsigned long long)sig);
     g.kernel = name;
     Out w;
     w("// generated by simstack_amd codegen: %u mechanisms in one persistent launch\n", (u32)ms.size());
     w.s += device_header;
     w.s += "\n";
-    // One LDS array serves every mechanism: a wave runs one mechanism at a time.
+    // One LDS array (FP32 factor storage: one for the pivots, one for the f32
+    // entries) serves every mechanism: a wave runs one mechanism at a time.
+    u32 piv = 0;
+    for (const Mech *m : ms)
+        if (m->n > piv) piv = m->n;
+    // FP32 factor storage: every mechanism's f32 share is the LDS budget less
+    // the shared pivot array of the largest mechanism
+    auto mech_options = [&](const Mech &m) {
+        GenOptions om = o;
+        if (o.precision == 3) {
+            const u32 extra = 2u * (piv - m.n);
+            om.lds_slots = o.lds_slots > extra ? o.lds_slots - extra : 0u;
+        }
+        return om;
+    };
     u32 lds = 0;
     for (const Mech *m : ms)
-        if (lds_entries(*m, o) > lds) lds = lds_entries(*m, o);
-    if (lds) w("__shared__ double simk_lds[%u * SIMK_WAVE];\n\n", lds);
+        if (lds_entries(*m, mech_options(*m)) > lds) lds = lds_entries(*m, mech_options(*m));
+    if (o.precision == 3) {
+        w("__shared__ double simk_lds[%u * SIMK_WAVE];\n", piv);
+        if (lds) w("__shared__ float simk_lds32[%u * SIMK_WAVE];\n", lds);
+        w("\n");
+    } else if (lds) {
+        w("__shared__ double simk_lds[%u * SIMK_WAVE];\n\n", lds);
+    }
     for (size_t i = 0; i < ms.size(); ++i) {
         char sname[32];
         snprintf(sname, sizeof(sname), "GenOps%u", (u32)i);
-        g.mech_slots.push_back(emit_ops_struct(w, *ms[i], me, o, sname));
+        g.mech_slots.push_back(emit_ops_struct(w, *ms[i], me, mech_options(*ms[i]), sname));
         if (g.mech_slots.back() > g.ws_slots) g.ws_slots = g.mech_slots.back();
     }
     emit_method(w, me, o);

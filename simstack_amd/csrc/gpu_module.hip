@@ -293,6 +293,7 @@ struct State {
     std::vector<SimkDevCtl> multi_ctl;
     JitKernel multi_jit;
     bool multi_ok = false;
+    JitKernel multi_jit_s32;   // FP32 factor storage form, compiled on the first precision-3 ragged launch
     // tables of each mechanism for the cooperative (team) form of the
     // multi-mechanism launch, and its team size (0: one lane per cell)
     void *multi_dtables[4] = {}, *multi_dwave[4] = {};
@@ -1315,19 +1316,42 @@ struct MultiPlan {
     u32 waves = 0, slots = 0;
     double *ws = nullptr;
     u32 *queue = nullptr;
+    const JitKernel *jk = nullptr;   // the kernel launched (nullptr: the FP64 multi-mechanism kernel)
 };
+
+u32 multi_team_size();
+
+}  // namespace
+
+// Defined below, in the C-linkage block, outside the unnamed namespace.
+extern "C" hipError_t load_multi(GenOptions o, JitKernel &out, u32 precision = 0);
+
+namespace {
 
 // own = true: allocate a workspace and queue counters for this plan alone (a
 // captured graph keeps them); else use the module's shared ones.
 hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPlan &P, bool own = false) {
     if (!g.multi_ok || count != g.multi_mechs.size() || repeats == 0 || !Ls) return hipErrorInvalidValue;
+    // every block in FP64 (precision 0) or every block in FP32 factor storage (3)
+    const u32 prec = Ls[0].precision;
+    if (prec != 0 && prec != 3) return hipErrorInvalidValue;
+    if (prec == 3) {
+        if (multi_team_size()) return hipErrorNotSupported;   // no cooperative form
+        if (!g.multi_jit_s32.mod) {
+            const hipError_t le = load_multi(g.gen, g.multi_jit_s32, 3);
+            if (le != hipSuccess) return le;
+        }
+        P.jk = &g.multi_jit_s32;
+    } else {
+        P.jk = &g.multi_jit;
+    }
     memset(&P.MA, 0, sizeof(P.MA));
     P.MA.count = count;
     u64 total_waves = 0;
     f64 weight[4];   // work share of each mechanism, for the cache footprint
     for (u32 i = 0; i < count; ++i) {
         const GpuKinLaunch *L = Ls + i;
-        if (!launch_ok(L) || L->precision != 0) return hipErrorInvalidValue;
+        if (!launch_ok(L) || L->precision != prec) return hipErrorInvalidValue;
         const Mech &m = g.multi_mechs[i];
         const u64 rows = m.R > m.n ? m.R : m.n;
         if (rows * L->padded * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
@@ -1335,9 +1359,9 @@ hipError_t prepare_multi(u32 count, const GpuKinLaunch *Ls, u32 repeats, MultiPl
         total_waves += (lanes + SIMK_WAVE - 1) / SIMK_WAVE;
         weight[i] = (f64)lanes * (f64)(m.nnz + 4 * m.n);   // per-cell cost ~ LU plus stage vectors
     }
-    P.slots = g.multi_jit.ws_slots;
+    P.slots = P.jk->ws_slots;
     if ((u64)P.slots * SIMK_WAVE * 8ull >= (1ull << 31)) return hipErrorInvalidValue;
-    const u32 resident = resident_waves(g.multi_jit.blocks_per_cu, multi_wave_bytes(weight, count), g.grid_override);
+    const u32 resident = resident_waves(P.jk->blocks_per_cu, multi_wave_bytes(weight, count), g.grid_override);
     P.waves = (u32)(total_waves < resident ? total_waves : resident);
     if (P.waves == 0) P.waves = 1;
     const u64 ws_bytes = (u64)P.waves * P.slots * SIMK_WAVE * sizeof(double);
@@ -1424,7 +1448,9 @@ hipError_t enqueue_multi_team(const MultiPlan &P) {
 u32 multi_team_size() { return g.multi_team ? g.multi_team : (g.mode == 1 ? g.team_lanes : 0u); }
 
 hipError_t enqueue_multi(const MultiPlan &P, bool allow_team = true) {
-    if (allow_team && multi_team_size()) {
+    const JitKernel &jk = P.jk ? *P.jk : g.multi_jit;
+    const bool s32 = P.jk == &g.multi_jit_s32;
+    if (allow_team && multi_team_size() && !s32) {
         switch (multi_team_size()) {
         case 8: return enqueue_multi_team<8>(P);
         case 16: return enqueue_multi_team<16>(P);
@@ -1436,11 +1462,11 @@ hipError_t enqueue_multi(const MultiPlan &P, bool allow_team = true) {
     SimkMultiArgs MA = P.MA;
     size_t size = sizeof(MA);
     void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &MA, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
-    g.last_kernel = 4;
+    g.last_kernel = s32 ? 10 : 4;
     g.last_slots = P.slots;
     g.last_waves = P.waves;
-    g.last_name = g.multi_jit.name;
-    return hipModuleLaunchKernel(g.multi_jit.fn, P.waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
+    g.last_name = jk.name;
+    return hipModuleLaunchKernel(jk.fn, P.waves, 1, 1, SIMK_WAVE, 1, 1, 0, g.stream, nullptr, cfg);
 }
 
 }  // namespace
@@ -1783,7 +1809,9 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.pair_slots = n.pair_slots;
     g.gen.k_pairs = n.k_pairs;
     g.gen.batch = n.batch;
-    unload_s32();   // the FP32 factor-storage kernel is built from these options
+    unload_s32();   // the FP32 factor-storage kernels are built from these options
+    if (g.multi_jit_s32.mod) (void)hipModuleUnload(g.multi_jit_s32.mod);
+    g.multi_jit_s32 = JitKernel();
     // explicit options are what a later untune() restores
     if (g.tuned) g.user_gen = g.gen;
     const u32 sw = n.sweeps;
@@ -2250,7 +2278,7 @@ GPU_API s32 simk_jit_precompile_multi(u32 count, const void *const *mechs, const
         ptrs.push_back(&ms[i]);
     }
     GenOptions o = gen_options(options);
-    o.precision = 0;
+    if (o.precision != 3) o.precision = 0;   // the multi-mechanism kernel: FP64 or FP32 factor storage
     KinMethod me = *method;
     me.name = nullptr;
     GenResult gr = generate_multi(ptrs, me, resolved_multi(o), kDeviceKinSource);
@@ -2293,11 +2321,11 @@ GPU_API s32 simk_copy_device_b64(void *dst, const void *src, u64 bytes, f32 *ms)
 
 // Generates, compiles (or loads from the cache) and loads the multi-mechanism
 // kernel of g.multi_mechs under options o.
-hipError_t load_multi(GenOptions o, JitKernel &out) {
+hipError_t load_multi(GenOptions o, JitKernel &out, u32 precision) {
     auto t0 = std::chrono::steady_clock::now();
     std::vector<const Mech *> ptrs;
     for (const Mech &m : g.multi_mechs) ptrs.push_back(&m);
-    o.precision = 0;
+    o.precision = precision;
     GenResult gr = generate_multi(ptrs, g.method, resolved_multi(o), kDeviceKinSource);
     std::vector<char> code;
     bool cached = false;
@@ -2338,6 +2366,8 @@ GPU_API s32 simk_kin_set_mechs(u32 count, const void *const *mechs, const SimkCa
     destroy_graphs();   // they launch the kernel of the module unloaded here
     if (g.multi_jit.mod) (void)hipModuleUnload(g.multi_jit.mod);
     g.multi_jit = JitKernel();
+    if (g.multi_jit_s32.mod) (void)hipModuleUnload(g.multi_jit_s32.mod);
+    g.multi_jit_s32 = JitKernel();
     g.multi_ok = false;
     for (u32 i = 0; i < 4; ++i) {
         if (g.multi_dtables[i]) (void)hipFree(g.multi_dtables[i]);

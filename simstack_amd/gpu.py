@@ -207,9 +207,10 @@ class GpuModule:
         self._check(self.lib.simk_kin_set_mechs(len(mechs), ptrs, caps, C.byref(meth), C.byref(ctl)),
                     "simk_kin_set_mechs")
 
-    def advance_multi(self, dzs, duration, repeats=1):
-        """One launch advancing every DeviceZones in dzs (mechanism i = dzs[i])."""
-        Ls = (_ffi.GpuKinLaunch * len(dzs))(*[d.launch_desc(duration) for d in dzs])
+    def advance_multi(self, dzs, duration, repeats=1, precision=0):
+        """One launch advancing every DeviceZones in dzs (mechanism i = dzs[i]);
+        precision 0 (FP64) or 3 (FP32 factor storage)."""
+        Ls = (_ffi.GpuKinLaunch * len(dzs))(*[d.launch_desc(duration, precision=precision) for d in dzs])
         ms = C.c_float()
         self._check(self.lib.simk_kin_advance_multi(len(dzs), Ls, repeats, C.byref(ms)), "simk_kin_advance_multi")
         return ms.value

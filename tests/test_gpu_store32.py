@@ -124,6 +124,31 @@ def test_store32_autotune_keeps_bits(gm):
     gm.set_gen_options()
 
 
+def test_store32_ragged_three_mechanisms(gm):
+    """The ragged batch (60/200, 40/130, 24/70) in one multi-mechanism launch
+    in FP32 factor storage, two advances: every cell of every mechanism equals
+    the oracle form."""
+    print("[case] store32 ragged 60/200 + 40/130 + 24/70, 2 x 0.5 s", flush=True)
+    blocks = [make_problem(name, cells) for name, cells in (("c3a", 300), ("c3b", 700), ("c3c", 1500))]
+    meth, ctl = problem.method(2), problem.control()
+    gm.arena_reset()
+    gm.set_grid(0)
+    gm.set_mechs([m for _, m, _ in blocks], meth, ctl)
+    work = [z.copy() for _, _, z in blocks]
+    dzs = [gpu.DeviceZones(gm, z) for z in work]
+    for d in dzs:
+        d.upload()
+    for _ in range(2):
+        gm.advance_multi(dzs, 0.5, precision=3)
+    assert gm.kin_info().kernel == 10
+    for d in dzs:
+        d.download(mutable_only=False)
+    for (_, mech, z), got in zip(blocks, work):
+        want = z.copy()
+        _oracle(mech, meth, ctl, want, [0.5, 0.5])
+        assert compare_bits(got, want) == ZERO
+
+
 def test_store32_c2_fused_repeats(gm):
     """4,096 cells of the 48/150 workload, 3 fused operator-split advances of 1 s."""
     print("[case] store32 c2 4096 cells x 3 x 1 s fused", flush=True)

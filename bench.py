@@ -278,6 +278,34 @@ def ragged_block(gpu, problem, steps, cells=(1000, 20000, 200000), duration=1.0,
     return out
 
 
+def team_sweep(gpu, problem, cells=16384, duration=0.2):
+    """One advance of the 48/150 workload's first `cells` cells per lane
+    layout: one cell per lane, then teams of 64, 32, 16 and 8 lanes per cell
+    (gpu_kin_set_mode(1) with simk_kin_set_team), from the same state."""
+    sp = problem.spec("c2")
+    mech = problem.Mechanism.synthetic(sp)
+    meth, ctl = problem.method(2), problem.control()
+    z0 = problem.Zones(mech.n, mech.R, cells).fill_synthetic(sp, mech.params)
+    gm = gpu.GpuModule(arena_bytes=z0.block.nbytes + (64 << 20))
+    gm.set_kernel(0)
+    out = []
+    for team in (0, 64, 32, 16, 8):
+        gm.arena_reset()
+        gm.lib.gpu_kin_set_mode(1 if team else 0)
+        if team:
+            gm.lib.simk_kin_set_team(team)
+        gm.set_mech(mech, meth, ctl)
+        dz = gpu.DeviceZones(gm, z0.copy())
+        dz.upload()
+        ms = dz.advance(duration)
+        dz.download()
+        att = float(dz.z.steps[:cells].sum() + dz.z.rejects[:cells].sum())
+        out.append({"lanes_per_cell": team or 1, "kernel_ms": ms, "attempts": att,
+                    "kernel": gm.kin_info().kernel_name.decode()})
+    gm.lib.gpu_kin_set_mode(0)
+    return {"workload": f"48 species / 150 reactions, {cells} cells x {duration:g} s, FP64", "layouts": out}
+
+
 def advect_block(species=48, channels=512, nodes=4096, substeps=4, reps=50):
     """The NFT upwind row update (simk_upwind_f64) on a [species][channels *
     nodes] f64 field of 805 MB, beyond the 256 MB Infinity Cache. One launch
@@ -491,6 +519,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="the overload arrival rate")
     ap.add_argument("--serve-sustained", type=float, default=3.0,
                     help="the highest arrival rate the GPU sustains (req/s; profiles/serving_latency_curve.md)")
+    ap.add_argument("--team-sweep", action="store_true",
+                    help="only time one advance per lane layout (one lane, teams of 64/32/16/8 lanes per cell)")
     ap.add_argument("--serve-curve", default="",
                     help="comma-separated arrival rates: latency at each over a 10-s window (adds minutes)")
     args = ap.parse_args()
@@ -500,6 +530,9 @@ def main():
         sys.exit(2)
 
     from simstack_amd import gpu, problem
+    if args.team_sweep:
+        print(json.dumps({"team_sweep": team_sweep(gpu, problem)}), flush=True)
+        return
     sp = problem.spec("c2")
     mech = problem.Mechanism.synthetic(sp)
     meth, ctl = problem.method(2), problem.control()

@@ -71,7 +71,7 @@ def cpu_baseline(mech, meth, ctl, sp, cells, duration, repeats=1, threads=0, por
 
 def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precision=0, sweeps=0, separate=False,
                  kernel=0, grid=0, reg_window=0, waves_per_eu=0, batch=0, tune=0.0, lds_slots=0, perturb=False,
-                 reuse=0, order=0):
+                 reuse=0, order=0, stage_carry=0):
     """Advances `cells` cells of the workload `steps` x 1 s from the seeded initial
     state (after `warmup` untimed seconds on a copy). Returns timings and the block.
     perturb: every initial concentration moved up by one ulp. order: the
@@ -91,7 +91,7 @@ def run_workload(gpu, problem, mech, sp, meth, ctl, cells, steps, warmup, precis
     gm.set_kernel(kernel)
     gm.lib.gpu_kin_set_mode(0)
     gm.set_gen_options(reg_window=reg_window, waves_per_eu=waves_per_eu, batch=batch, sweeps=sweeps,
-                       lds_slots=lds_slots, reuse=reuse)
+                       lds_slots=lds_slots, reuse=reuse, stage_carry=stage_carry)
     gm.set_grid(grid)
     gm.set_queue_policy(order)
     gm.set_mech(mech, meth, ctl)

@@ -159,6 +159,11 @@ typedef struct SimkGenOptions {
     u32 k_pairs;         /* paired layouts: 1: a second copy of the cell's rate constants in paired slots, which the
                             f passes of stages read two per 16-byte access (the -J pass keeps the single copy);
                             ~0u: none; 0: default (on for one-mechanism kernels) */
+    u32 stage_carry;     /* 1: a stage whose combination ends with the previous stage's K (the point of an f
+                            evaluation, or the saved f plus the c-combination) takes that term from the
+                            registers that still hold it at the end of the previous stage: one stage-vector
+                            load fewer per such stage, the same bits; ~0u: every term is loaded; 0: default
+                            (off; the autotuners try it) */
 } SimkGenOptions;
 GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *options);
 
@@ -176,8 +181,8 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  *   2. at the best grid, the generator's load-pipeline depth (8, 16), register
  *      window (8, 32), unrolled stages, rate-constant reuse window (none, 48, 96),
  *      a two-waves-per-SIMD register budget, single instead of paired workspace
- *      slots, the paired rate-constant copy off, and LU entries in LDS (none,
- *      80, 128, 160), one knob at a time;
+ *      slots, the paired rate-constant copy off, the stage carry on, and LU
+ *      entries in LDS (none, 80, 128, 160), one knob at a time;
  *   3. teams of 64, 32, 16 or 8 lanes per cell (the cooperative form) instead
  *      of one lane per cell.
  * Every candidate computes the same bits; only time differs. The tuned
@@ -188,7 +193,7 @@ GPU_API s32 simk_kin_set_cache_budget(u64 bytes);
  * default, 320, 240, 160), each at the cache-aware grid and at as many waves
  * per CU as that LDS share allows, then at the best pair the load-pipeline
  * depth (8, 12, 20, 24, 28, 32), register window (4, 16), rate-constant reuse
- * (none, 48, 96, 384) and unrolled stages. With L->precision 3 the same for the
+ * (none, 48, 96, 384), unrolled stages and the stage carry. With L->precision 3 the same for the
  * FP32 factor-storage kernel, its LDS budgets being the default (304), 320,
  * 288, 256 and 208 f32 words per lane, pivots included. simk_kin_set_gen_options
  * drops these settings. Precision 1 is refused (hipErrorInvalidValue). */
@@ -206,7 +211,8 @@ GPU_API s32 simk_kin_autotune(const GpuKinLaunch *L, u32 sample_cells, f64 durat
 /* The same for the multi-mechanism kernel of simk_kin_set_mechs (ragged
  * batches): a sample of sample_cells cells of every launches[i] (all when 0),
  * over the persistent grid and, at the best grid, the load-pipeline depth,
- * register window, unrolled stages and LU entries in LDS. One lane per cell
+ * register window, unrolled stages, LU entries in LDS, single workspace
+ * slots, the paired rate-constant copy and the stage carry. One lane per cell
  * is this kernel's only lane layout. Destroys every captured graph. The tuned
  * settings last until the next simk_kin_set_mechs / gpu_kin_set_mech. */
 GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32 sample_cells, f64 duration,

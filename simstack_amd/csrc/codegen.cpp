@@ -385,6 +385,17 @@ u32 auto_k_pairs(const GenOptions &o, bool multi) {
     return o.precision == 0 || o.precision == 3 ? 1u : 0u;
 }
 
+// Off by default; the autotuners try it. The carried vector stays live across
+// the stage loop. On 48/150 at 262,144 cells, 20 x 1 s (same bits either
+// way): FP64 1,596 -> 1,584 ms with the carry, although the code object then
+// spills 116 bytes per lane; FP32 factor storage 1,092-1,099 -> 1,113-1,123
+// ms (508 registers, no scratch), fewer bytes but a longer critical path.
+u32 auto_stage_carry(const GenOptions &o, bool multi) {
+    (void)o;
+    (void)multi;
+    return 0u;
+}
+
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]) {
     // f holds f(point of stage `holder`); at_y: that point is y itself.
     u32 holder = 0;
@@ -441,6 +452,7 @@ static u32 emit_ops_struct(Out &w, const Mech &m, const KinMethod &me, const Gen
     // jac_factor also evaluates f(y) and leaves it in the f slots (fij).
     const bool fij = o.fun_in_jac != 0;
     w("    static constexpr bool kFunInJac = %s;\n", fij ? "true" : "false");
+    w("    static constexpr bool kStageCarry = %s;\n", o.stage_carry ? "true" : "false");
     w("    using Vec = SimkRegVec<N>;\n");
     w("    SimkRows kr, sr, pr, ws;\n");
     if (pairs) w("    SimkRowPairs wp;\n");

@@ -40,6 +40,11 @@ struct GenOptions {
                                  // passes read two per 16-byte access; 0: none (kAutoKPairs: auto_k_pairs)
     static constexpr u32 kAutoPair = 0xFFFF;
     static constexpr u32 kAutoKPairs = 0xFFFF;
+    u32 stage_carry = kAutoCarry;   // 1: a stage whose combination ends with the previous stage's K takes
+                                    // that term from the registers still holding it (one K load fewer per
+                                    // such stage, same bits); 0: every K term is loaded (kAutoCarry:
+                                    // auto_stage_carry)
+    static constexpr u32 kAutoCarry = 0xFFFF;
 };
 
 struct GenResult {
@@ -72,6 +77,8 @@ u32 auto_reuse(const GenOptions &o, bool multi = false);
 u32 auto_fun_in_jac(const GenOptions &o);
 u32 auto_pair(const GenOptions &o);
 u32 auto_k_pairs(const GenOptions &o, bool multi = false);
+// The default stage carry: off (an autotuner knob).
+u32 auto_stage_carry(const GenOptions &o, bool multi = false);
 
 // Stage plan shared with the host launch code: which stages evaluate f.
 void stage_plan(const KinMethod &me, u32 eval[4], u32 save_f[4]);

@@ -322,8 +322,10 @@ struct SimkWsVec {
 //
 // Order of work inside one step attempt differs from the reference only where
 // results cannot depend on it: -J/LU is formed before f(y) (the two read y
-// only), and a stage whose point equals the previous evaluation point reuses
-// the saved f instead of recomputing the same bits (Rodas3 stage 2).
+// only), a stage whose point equals the previous evaluation point reuses
+// the saved f instead of recomputing the same bits (Rodas3 stage 2), and a
+// stage's combination whose last term is the previous stage's K is finished
+// at the end of that stage, from the K still in registers.
 
 // One step attempt of kin_integrate's loop body (kin_solver.h:216-311) for the
 // active lanes: step size, -J/LU, the stages, error norm, accept or reject,
@@ -345,6 +347,20 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
     const double hinv = 1.0 / hs;
 
     typename P::Vec x = p.vec(3);
+    const simk_u32 S = me.stages();
+    // Stage st >= 1 can be started at the end of stage st - 1, whose K is
+    // still in x, when that K is the last term of st's combination (terms run
+    // over j ascending, so the K_{st-1} term is applied last): its point
+    // (a stage that evaluates f) or its right-hand side, the saved f plus the
+    // c-combination (a stage that reuses f). Either is left in x for the next
+    // stage. Every element sees the same fma sequence; the carried stage
+    // skips one load of K_{st-1}.
+    auto carry_point = [&](const simk_u32 st) {
+        return P::kStageCarry && st >= 1 && st < S && me.eval(st) && me.a(st * (st - 1) / 2 + st - 1) != 0.0;
+    };
+    auto carry_rhs = [&](const simk_u32 st) {
+        return P::kStageCarry && st >= 1 && st < S && !me.eval(st) && me.c(st * (st - 1) / 2 + st - 1) != 0.0;
+    };
     // One stage: its point, f there (or the saved f), the c-combination of
     // earlier stages, the solve. Rolled by default so the solve and f are
     // emitted once; the generator may unroll it. Every vector update runs
@@ -352,6 +368,7 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
     // reference's loops (kin_solver.h:238-259); only the loop nesting differs.
     auto stage = [&](const simk_u32 st) {
         const simk_u32 packed = st * (st - 1) / 2;
+        const bool carried_rhs = carry_rhs(st);
         bool have_f = false;
         if constexpr (P::kFunInJac) {
             // jac_factor left f(y) in the f slots, where a later stage that
@@ -364,23 +381,43 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
         if (have_f) {
         } else if (st == 0 || me.eval(st)) {
             typename P::Vec pt = p.vec(2);
+            const bool carried_point = carry_point(st);
 #pragma unroll
             for (simk_u32 s = 0; s < n; ++s) pt.set(s, y.get(s));
-            for (simk_u32 j = 0; j < st; ++j) {
+            for (simk_u32 j = 0; j < st - (carried_point ? 1u : 0u); ++j) {
                 const double a = me.a(packed + j);
                 if (a != 0.0) p.kaxpy(j, a, pt);
             }
+            if (carried_point) {
+                const double a = me.a(packed + st - 1);
+#pragma unroll
+                for (simk_u32 s = 0; s < n; ++s) pt.set(s, simk_fma(a, x.get(s), pt.get(s)));
+            }
             p.fun(pt, x);
             if (me.save_f(st)) p.fsave(x);
-        } else {
+        } else if (!carried_rhs) {
             p.fload(x);
         }
-        for (simk_u32 j = 0; j < st; ++j) {
-            const double c = me.c(packed + j);
-            if (c != 0.0) p.kaxpy(j, c * hinv, x);
+        if (!carried_rhs) {
+            for (simk_u32 j = 0; j < st; ++j) {
+                const double c = me.c(packed + j);
+                if (c != 0.0) p.kaxpy(j, c * hinv, x);
+            }
         }
         p.solve(x, y, ghinv);
         p.kstore(st, x);
+        const simk_u32 nx = st + 1, npacked = nx * st / 2;
+        if (carry_rhs(nx)) {
+            typename P::Vec acc = p.vec(2);
+            p.fload(acc);
+            for (simk_u32 j = 0; j < st; ++j) {
+                const double c = me.c(npacked + j);
+                if (c != 0.0) p.kaxpy(j, c * hinv, acc);
+            }
+            const double c = me.c(npacked + st) * hinv;
+#pragma unroll
+            for (simk_u32 s = 0; s < n; ++s) x.set(s, simk_fma(c, x.get(s), acc.get(s)));
+        }
     };
     if constexpr (M::kUnrollStages) {
 #pragma unroll
@@ -395,7 +432,6 @@ SIMK_DEV void simk_attempt(P &p, const M &me, const SimkDevCtl &ctl, typename P:
     // ascending order per element; kin_solver.h:264-277). The last stage's K
     // is still in x. The error vector is only materialised when an earlier
     // stage contributes to it.
-    const simk_u32 S = me.stages();
     typename P::Vec ynew = p.vec(2);
 #pragma unroll
     for (simk_u32 s = 0; s < n; ++s) ynew.set(s, y.get(s));

@@ -33,6 +33,7 @@ using namespace simk;
 
 struct TableOps {
     static constexpr bool kFunInJac = false;
+    static constexpr bool kStageCarry = false;
     using Vec = SimkWsVec;
     const SimkDevTables *T;
     simk_u32 n_, R_, nnz_;
@@ -728,6 +729,7 @@ GenOptions resolved_multi(GenOptions o) {
     // multi-mechanism autotuner's pick; the kernel is at the register limit).
     if (o.reg_window == GenOptions::kDefaultWindow) o.reg_window = 8u;
     if (o.k_pairs == GenOptions::kAutoKPairs) o.k_pairs = auto_k_pairs(o, true);
+    if (o.stage_carry == GenOptions::kAutoCarry) o.stage_carry = auto_stage_carry(o, true);
     return resolved(o);
 }
 
@@ -738,6 +740,7 @@ GenOptions resolved(GenOptions o) {
     if (o.fun_in_jac == GenOptions::kAutoFunInJac) o.fun_in_jac = auto_fun_in_jac(o);
     if (o.pair_slots == GenOptions::kAutoPair) o.pair_slots = auto_pair(o);
     if (o.k_pairs == GenOptions::kAutoKPairs) o.k_pairs = auto_k_pairs(o);
+    if (o.stage_carry == GenOptions::kAutoCarry) o.stage_carry = auto_stage_carry(o);
     return o;
 }
 
@@ -1293,6 +1296,8 @@ GenOptions gen_options(const SimkGenOptions *options) {
     o.fun_in_jac = options->fun_in_jac == 0 ? GenOptions::kAutoFunInJac : (options->fun_in_jac == 0xFFFFFFFFu ? 0u : 1u);
     o.pair_slots = options->pair_slots == 0 ? GenOptions::kAutoPair : (options->pair_slots == 0xFFFFFFFFu ? 0u : 1u);
     o.k_pairs = options->k_pairs == 0 ? GenOptions::kAutoKPairs : (options->k_pairs == 0xFFFFFFFFu ? 0u : 1u);
+    o.stage_carry = options->stage_carry == 0 ? GenOptions::kAutoCarry
+                                              : (options->stage_carry == 0xFFFFFFFFu ? 0u : 1u);
     return o;
 }
 
@@ -1808,6 +1813,7 @@ GPU_API s32 simk_kin_set_gen_options(const SimkGenOptions *o) {
     g.gen.fun_in_jac = n.fun_in_jac;
     g.gen.pair_slots = n.pair_slots;
     g.gen.k_pairs = n.k_pairs;
+    g.gen.stage_carry = n.stage_carry;
     g.gen.batch = n.batch;
     unload_s32();   // the FP32 factor-storage kernels are built from these options
     if (g.multi_jit_s32.mod) (void)hipModuleUnload(g.multi_jit_s32.mod);
@@ -1897,7 +1903,8 @@ bool same_gen(const GenOptions &a, const GenOptions &b) {
     return a.fence_rows == b.fence_rows && a.waves_per_eu == b.waves_per_eu && a.unroll_stages == b.unroll_stages &&
            a.batch == b.batch && a.precision == b.precision && a.sweeps == b.sweeps &&
            a.reg_window == b.reg_window && a.reuse == b.reuse && a.lds_slots == b.lds_slots &&
-           a.fun_in_jac == b.fun_in_jac && a.pair_slots == b.pair_slots && a.k_pairs == b.k_pairs;
+           a.fun_in_jac == b.fun_in_jac && a.pair_slots == b.pair_slots && a.k_pairs == b.k_pairs &&
+           a.stage_carry == b.stage_carry;
 }
 
 // Mixed precision (precision 2) and FP32 factor storage (precision 3): the
@@ -2007,6 +2014,11 @@ s32 autotune_mixed(const GpuKinLaunch *L, u32 cells, f64 duration, SimkTuneResul
     if (e == hipSuccess) {
         GenOptions o = best_gen;
         o.unroll_stages = 1;
+        e = consider(best_grid, o);
+    }
+    if (e == hipSuccess) {
+        GenOptions o = best_gen;
+        o.stage_carry = resolved(kernel_options(best_gen)).stage_carry ? 0u : 1u;
         e = consider(best_grid, o);
     }
     if (S.mem) (void)hipFree(S.mem);
@@ -2153,6 +2165,12 @@ static s32 autotune_impl(const GpuKinLaunch *L, u32 sample_cells, f64 duration, 
     if (e == hipSuccess && resolved(best_gen).pair_slots) {
         GenOptions o = best_gen;
         o.k_pairs = resolved(best_gen).k_pairs ? 0u : 1u;
+        e = consider(0, best_grid, o);
+    }
+    // the stage carry, on (or off)
+    if (e == hipSuccess) {
+        GenOptions o = best_gen;
+        o.stage_carry = resolved(best_gen).stage_carry ? 0u : 1u;
         e = consider(0, best_grid, o);
     }
     // LU entries in LDS: none, and shares for 4, 3 and 2 waves per CU
@@ -2532,6 +2550,7 @@ GPU_API s32 simk_kin_autotune_multi(u32 count, const GpuKinLaunch *launches, u32
     knob([](GenOptions &o) { o.lds_slots = 128; });
     knob([](GenOptions &o) { o.pair_slots = resolved(o).pair_slots ? 0u : 1u; });
     knob([](GenOptions &o) { o.k_pairs = resolved_multi(o).k_pairs ? 0u : 1u; });
+    knob([](GenOptions &o) { o.stage_carry = resolved_multi(o).stage_carry ? 0u : 1u; });
     // teams of 64, 32, 16 and 8 lanes per cell (the cooperative form)
     const u32 saved_multi_team = g.multi_team;
     u32 best_team = 0;

@@ -18,7 +18,8 @@ class SimkGenOptions(C.Structure):
     _fields_ = [("fence_rows", C.c_uint32), ("waves_per_eu", C.c_uint32), ("unroll_stages", C.c_uint32),
                 ("reg_window", C.c_uint32), ("batch", C.c_uint32), ("precision", C.c_uint32),
                 ("sweeps", C.c_uint32), ("reuse", C.c_uint32), ("lds_slots", C.c_uint32),
-                ("fun_in_jac", C.c_uint32), ("pair_slots", C.c_uint32), ("k_pairs", C.c_uint32)]
+                ("fun_in_jac", C.c_uint32), ("pair_slots", C.c_uint32), ("k_pairs", C.c_uint32),
+                ("stage_carry", C.c_uint32)]
 
 
 class SimkKinInfo(C.Structure):
@@ -195,9 +196,9 @@ class GpuModule:
         self._check(self.lib.simk_kin_set_kernel(kind), "simk_kin_set_kernel")
 
     def set_gen_options(self, fence_rows=0, waves_per_eu=0, unroll_stages=0, reg_window=0, batch=0, sweeps=0,
-                        reuse=0, lds_slots=0, fun_in_jac=0, pair_slots=0, k_pairs=0):
+                        reuse=0, lds_slots=0, fun_in_jac=0, pair_slots=0, k_pairs=0, stage_carry=0):
         opt = SimkGenOptions(fence_rows, waves_per_eu, unroll_stages, reg_window, batch, 0, sweeps, reuse, lds_slots,
-                             fun_in_jac, pair_slots, k_pairs)
+                             fun_in_jac, pair_slots, k_pairs, stage_carry)
         self._check(self.lib.simk_kin_set_gen_options(C.byref(opt)), "simk_kin_set_gen_options")
 
     def set_mechs(self, mechs, meth, ctl):

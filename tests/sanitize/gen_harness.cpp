@@ -3,6 +3,7 @@
 // given capacities) and generates every kernel form for each of them: the
 // four precisions, LDS shares from none to all, single and paired workspace
 // slots, with and without the paired rate-constant copy, refinement sweeps,
+// the stage carry on and off (alternating over the other options),
 // and the multi-mechanism kernels in FP64 and FP32 factor storage. Built and
 // run by tests/test_generator_sanitized.py (g++, host code only).
 //
@@ -59,12 +60,14 @@ int main(int argc, char **argv) {
                     for (u32 kp : {0u, 1u})
                         for (u32 sweeps : {0u, 1u, 2u}) {
                             if (prec != 2 && sweeps) continue;
+                            const u32 carry = (lds + kp + sweeps) & 1u;
                             GenOptions o;
                             o.precision = prec;
                             o.sweeps = sweeps;
                             o.lds_slots = lds;
                             o.pair_slots = pair;
                             o.k_pairs = kp;
+                            o.stage_carry = carry;
                             o.batch = 24;
                             o.reuse = 384;
                             o.fun_in_jac = 1;
@@ -82,6 +85,7 @@ int main(int argc, char **argv) {
             o.lds_slots = prec == 3 ? 304u : 160u;
             o.pair_slots = 1;
             o.k_pairs = kp;
+            o.stage_carry = kp;
             o.batch = 24;
             o.reuse = 96;
             o.fun_in_jac = 1;

@@ -16,7 +16,8 @@
 // same slot one lane down (DPP wave_shr:1); lane 0 takes lane 63 of the slot
 // before (DPP wave_ror:1 of that slot), slots updated from the last down so
 // every read sees the old value. Only a part's first node crosses waves
-// (through LDS, one barrier per substep). The plug-flow chain: one 256-thread
+// (through LDS, one barrier per substep). f64 groups of fields with even row
+// lengths hold two nodes per lane and slot (16-byte accesses). The plug-flow chain: one 256-thread
 // workgroup per species row, a chunk of consecutive zones per thread, chunk
 // boundaries through LDS.
 #include <hip/hip_runtime.h>
@@ -128,6 +129,33 @@ __device__ __forceinline__ double lane_value(double v, unsigned lane) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Raw buffer access (gfx9 family descriptor word 3: raw buffer, 32-bit data
+// format). Loads at or beyond the resource's extent return 0 and stores there
+// are dropped; the range check covers the lane offset and the instruction's
+// immediate offset, and the scalar offset is always 0 here.
+constexpr int kRawBufferFlags = 0x00027000;
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <class T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, T v);
+template <>
+__device__ __forceinline__ void buf_store<double>(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, (int)off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<float>(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+
 // One part of a row: nodes [part * SLOTS * 64, (part + 1) * SLOTS * 64) of a
 // row of `nodes` values, advanced by one wave; a row of WPR parts is advanced
 // by WPR consecutive waves of the workgroup. Each substep a part's first node
@@ -220,7 +248,10 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_group_kernel(T *x, unsig
                                                                      const T *inflow, T *outlet_sum) {
     constexpr unsigned kPart = SLOTS * 64u;
     __shared__ T edge[2][kRowsPerBlock];
-    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x / 64u, part = wave % WPR;
+    // The wave index is uniform across the wave; read from lane 0 it is a
+    // scalar, so every row base below is one too.
+    const unsigned lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
+    const unsigned part = wave % WPR;
     const unsigned groups = (fields + G - 1) / G;
     const unsigned gidx = blockIdx.x * (kRowsPerBlock / WPR) + wave / WPR;
     const bool gactive = gidx < groups * channels;
@@ -228,25 +259,46 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_group_kernel(T *x, unsig
     const unsigned gi = gactive ? gidx : 0;
     const unsigned ch = gi % channels, fg = gi / channels;
     const unsigned base = part * kPart;
-    const T *cr = courant + (unsigned long long)ch * nodes;
+    // This wave's nodes of a row through a buffer resource whose extent ends
+    // at the row's last node: a lane past the end (or every lane of an
+    // inactive row, extent 0) reads 0 and its store is dropped, so no access
+    // needs a per-slot guard.
+    const unsigned part_bytes = base < nodes ? (nodes - base) * (unsigned)sizeof(T) : 0u;
+    auto row_rsrc = [&](const T *row, const bool active) {
+        return __builtin_amdgcn_make_buffer_rsrc((void *)(row + base), 0, active ? (int)part_bytes : 0,
+                                                 kRawBufferFlags);
+    };
+    const unsigned loff = lane * (unsigned)sizeof(T);
+    const __amdgpu_buffer_rsrc_t crs = row_rsrc(courant + (unsigned long long)ch * nodes, gactive);
     T c[SLOTS];
 #pragma unroll
-    for (unsigned k = 0; k < SLOTS; ++k) {
-        const unsigned j = base + k * 64u + lane;
-        c[k] = (gactive && j < nodes) ? cr[j] : T(0);
-    }
+    for (unsigned k = 0; k < SLOTS; ++k) c[k] = buf_load<T>(crs, loff + k * 64u * (unsigned)sizeof(T));
     const bool outlet_part = outlet_sum && (nodes - 1) / kPart == part;
     const unsigned oslot = ((nodes - 1) % kPart) / 64u, olane = (nodes - 1) % 64u;
+    auto field_rsrc = [&](const unsigned i) {
+        const unsigned f = fg * G + i;
+        const bool active = gactive && f < fields;
+        return row_rsrc(x + (unsigned long long)(active ? f : 0u) * padded + (unsigned long long)ch * nodes, active);
+    };
+    // Field i + 1's part is loaded before field i is advanced, so its loads
+    // are in flight during field i's substeps and stores.
+    T v[SLOTS], next[SLOTS];
+    {
+        const __amdgpu_buffer_rsrc_t rs = field_rsrc(0);
+#pragma unroll
+        for (unsigned k = 0; k < SLOTS; ++k) next[k] = buf_load<T>(rs, loff + k * 64u * (unsigned)sizeof(T));
+    }
     unsigned step = 0;
     for (unsigned i = 0; i < G; ++i) {
         const unsigned f = fg * G + i;
         const bool active = gactive && f < fields;
-        T *xr = x + (unsigned long long)(active ? f : 0u) * padded + (unsigned long long)ch * nodes;
-        T v[SLOTS];
+        const __amdgpu_buffer_rsrc_t rs = field_rsrc(i);
 #pragma unroll
-        for (unsigned k = 0; k < SLOTS; ++k) {
-            const unsigned j = base + k * 64u + lane;
-            v[k] = (active && j < nodes) ? xr[j] : T(0);
+        for (unsigned k = 0; k < SLOTS; ++k) v[k] = next[k];
+        if (i + 1 < G) {
+            const __amdgpu_buffer_rsrc_t nrs = field_rsrc(i + 1);
+#pragma unroll
+            for (unsigned k = 0; k < SLOTS; ++k) next[k] = buf_load<T>(nrs, loff + k * 64u * (unsigned)sizeof(T));
         }
         const bool owns_outlet = active && outlet_part;
         const unsigned r = (active ? f : 0u) * channels + ch;
@@ -274,13 +326,102 @@ __global__ void __launch_bounds__(kThreads) simk_upwind_group_kernel(T *x, unsig
             }
         }
         if (owns_outlet && lane == 0) outlet_sum[r] = out;
-        if (active) {
 #pragma unroll
-            for (unsigned k = 0; k < SLOTS; ++k) {
-                const unsigned j = base + k * 64u + lane;
-                if (j < nodes) xr[j] = v[k];
+        for (unsigned k = 0; k < SLOTS; ++k) buf_store<T>(rs, loff + k * 64u * (unsigned)sizeof(T), v[k]);
+    }
+}
+
+// The f64 group kernel with 16-byte lane accesses: lane l of slot k holds
+// nodes base + 128 k + 2 l and 2 l + 1 of its row, so every load and store
+// of a row moves 1 KiB per wave instruction. Node 2 l + 1 takes its upstream
+// value from node 2 l of the same lane; node 2 l from node 2 l - 1, the .y of
+// lane l - 1 (DPP wave_shr:1; lane 0 takes lane 63 of the slot before, DPP
+// wave_ror:1). Both come from old values, and each node's update is the same
+// fused multiply-add as in the kernels above. Used when every row base is
+// 16-byte aligned and the row length is even, so a lane's pair is either
+// wholly inside its row or wholly past the end.
+struct SimkD2A {
+    double x, y;
+};
+__device__ __forceinline__ SimkD2A buf_load2(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u q = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    return {__builtin_bit_cast(double, v2u{q.x, q.y}), __builtin_bit_cast(double, v2u{q.z, q.w})};
+}
+__device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t r, unsigned off, SimkD2A v) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u a = __builtin_bit_cast(v2u, v.x), b = __builtin_bit_cast(v2u, v.y);
+    __builtin_amdgcn_raw_buffer_store_b128(v4u{a.x, a.y, b.x, b.y}, r, (int)off, 0, 0);
+}
+template <unsigned SLOTS, unsigned WPR, unsigned G>
+__global__ void __launch_bounds__(kThreads) simk_upwind_group2_kernel(double *x, unsigned long long padded,
+                                                                      unsigned fields, unsigned channels,
+                                                                      unsigned nodes, const double *courant,
+                                                                      unsigned substeps, const double *inflow) {
+    constexpr unsigned kPart = SLOTS * 128u;
+    __shared__ double edge[2][kRowsPerBlock];
+    const unsigned lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
+    const unsigned part = wave % WPR;
+    const unsigned groups = (fields + G - 1) / G;
+    const unsigned gidx = blockIdx.x * (kRowsPerBlock / WPR) + wave / WPR;
+    const bool gactive = gidx < groups * channels;
+    if (WPR == 1 && !gactive) return;   // whole waves, no barriers
+    const unsigned gi = gactive ? gidx : 0;
+    const unsigned ch = gi % channels, fg = gi / channels;
+    const unsigned base = part * kPart;
+    const unsigned part_bytes = base < nodes ? (nodes - base) * 8u : 0u;
+    auto row_rsrc = [&](const double *row, const bool active) {
+        return __builtin_amdgcn_make_buffer_rsrc((void *)(row + base), 0, active ? (int)part_bytes : 0,
+                                                 kRawBufferFlags);
+    };
+    const unsigned loff = lane * 16u;
+    const __amdgpu_buffer_rsrc_t crs = row_rsrc(courant + (unsigned long long)ch * nodes, gactive);
+    SimkD2A c[SLOTS];
+#pragma unroll
+    for (unsigned k = 0; k < SLOTS; ++k) c[k] = buf_load2(crs, loff + k * 1024u);
+    auto field_rsrc = [&](const unsigned i) {
+        const unsigned f = fg * G + i;
+        const bool active = gactive && f < fields;
+        return row_rsrc(x + (unsigned long long)(active ? f : 0u) * padded + (unsigned long long)ch * nodes, active);
+    };
+    SimkD2A v[SLOTS], next[SLOTS];
+    {
+        const __amdgpu_buffer_rsrc_t rs = field_rsrc(0);
+#pragma unroll
+        for (unsigned k = 0; k < SLOTS; ++k) next[k] = buf_load2(rs, loff + k * 1024u);
+    }
+    unsigned step = 0;
+    for (unsigned i = 0; i < G; ++i) {
+        const unsigned f = fg * G + i;
+        const bool active = gactive && f < fields;
+        const __amdgpu_buffer_rsrc_t rs = field_rsrc(i);
+#pragma unroll
+        for (unsigned k = 0; k < SLOTS; ++k) v[k] = next[k];
+        if (i + 1 < G) {
+            const __amdgpu_buffer_rsrc_t nrs = field_rsrc(i + 1);
+#pragma unroll
+            for (unsigned k = 0; k < SLOTS; ++k) next[k] = buf_load2(nrs, loff + k * 1024u);
+        }
+        const double in = (active && inflow) ? inflow[f] : 0.0;
+        for (unsigned s = 0; s < substeps; ++s, ++step) {
+            double first = in;
+            if constexpr (WPR > 1) {
+                if (lane == 63) edge[step & 1u][wave] = v[SLOTS - 1].y;
+                __syncthreads();
+                if (part) first = edge[step & 1u][wave - 1];
+            }
+#pragma unroll
+            for (unsigned k = SLOTS; k-- > 0;) {
+                const double prev = k ? dpp<kWaveRor1>(0.0, v[k - 1].y) : first;
+                const double up_x = shift_up(v[k].y, prev), up_y = v[k].x;
+                v[k].y = fma_t<double>(c[k].y, up_y - v[k].y, v[k].y);
+                v[k].x = fma_t<double>(c[k].x, up_x - v[k].x, v[k].x);
             }
         }
+#pragma unroll
+        for (unsigned k = 0; k < SLOTS; ++k) buf_store2(rs, loff + k * 1024u, v[k]);
     }
 }
 
@@ -343,6 +484,31 @@ hipError_t launch_upwind(hipStream_t stream, T *x, u64 padded, u32 fields, u32 c
     const u32 per_block = kRowsPerBlock / w.wpr;
     const dim3 block(kThreads);
     const unsigned long long pd = padded;
+    if constexpr (sizeof(T) == 8) {
+        // f64 rows with 16-byte lane accesses: every row base 16-byte aligned
+        // and an even row length (a lane's pair never straddles a row's end)
+        if (fields >= kGroupFields && !outlet_sum && nodes % 2 == 0 && padded % 2 == 0 &&
+            (reinterpret_cast<uintptr_t>(x) & 15u) == 0 && (reinterpret_cast<uintptr_t>(courant) & 15u) == 0) {
+            const u32 groups = (fields + kGroupFields - 1) / kGroupFields;
+            const u64 units = (u64)groups * channels;
+            const u32 wpr2 = nodes <= 2048 ? 1u : (nodes <= 4096 ? 2u : 4u);
+            const u32 per2 = kRowsPerBlock / wpr2;
+            const dim3 ggrid((u32)((units + per2 - 1) / per2));
+            if (nodes <= 1024)
+                hipLaunchKernelGGL((simk_upwind_group2_kernel<8, 1, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                                   fields, channels, nodes, courant, substeps, inflow);
+            else if (wpr2 == 1)
+                hipLaunchKernelGGL((simk_upwind_group2_kernel<16, 1, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                                   fields, channels, nodes, courant, substeps, inflow);
+            else if (wpr2 == 2)
+                hipLaunchKernelGGL((simk_upwind_group2_kernel<16, 2, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                                   fields, channels, nodes, courant, substeps, inflow);
+            else
+                hipLaunchKernelGGL((simk_upwind_group2_kernel<16, 4, kGroupFields>), ggrid, block, 0, stream, x, pd,
+                                   fields, channels, nodes, courant, substeps, inflow);
+            return hipGetLastError();
+        }
+    }
     if (fields >= kGroupFields) {
         // kGroupFields fields of a channel per wave group (coefficients loaded once)
         const u32 groups = (fields + kGroupFields - 1) / kGroupFields;

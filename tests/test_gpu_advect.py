@@ -22,13 +22,16 @@ def _dev(gm, arr):
                                                             (257, 2, 5, 3), (1000, 1, 9, 40), (8192, 1, 2, 4),
                                                             (2048, 2, 3, 4), (3001, 3, 5, 7), (5000, 1, 3, 6),
                                                             (6000, 2, 1, 3), (8192, 2, 10, 4), (5000, 3, 17, 6),
-                                                            (2048, 5, 8, 3), (700, 3, 23, 5)])
+                                                            (2048, 5, 8, 3), (700, 3, 23, 5), (3001, 3, 9, 7),
+                                                            (999, 2, 8, 5)])
 def test_upwind_f64_matches_oracle(gm, nodes, channels, fields, substeps):
     """Row lengths over every wave shape: 16 or 32 slots per wave, 1, 2 or 4
     waves per row, ragged last parts, empty last parts (5,000 nodes in 4
     waves) and row counts that leave a workgroup's last row inactive; with 8
     or more fields, wave groups of 8 fields of a channel, including groups
-    whose last fields are past the end."""
+    whose last fields are past the end: with even row lengths the groups use
+    16-byte lane accesses (two nodes per lane and slot), with odd ones 8-byte
+    accesses."""
     rng = np.random.default_rng(nodes + channels)
     padded = (channels * nodes + 63) // 64 * 64
     x = rng.random((fields, padded)) * 10.0 ** rng.uniform(-12, -2, (fields, 1))

@@ -31,12 +31,14 @@ def _set(gm, blocks):
     return meth, ctl
 
 
-@pytest.mark.parametrize("pair", [0, 0xFFFFFFFF], ids=["paired-slots", "single-slots"])
-def test_three_mechanisms_one_launch(gm, pair):
+@pytest.mark.parametrize("pair,carry", [(0, 0), (0xFFFFFFFF, 0), (0, 1)],
+                         ids=["paired-slots", "single-slots", "stage-carry"])
+def test_three_mechanisms_one_launch(gm, pair, carry):
     """Ragged cell counts (not multiples of 64), two operator-split calls;
-    workspace slots paired (the default) or single."""
+    workspace slots paired (the default) or single; the stage carry off (the
+    default) or on."""
     try:
-        gm.set_gen_options(pair_slots=pair)
+        gm.set_gen_options(pair_slots=pair, stage_carry=carry)
         blocks = _blocks((200, 333, 130))
         meth, ctl = _set(gm, blocks)
         work = [z.copy() for _, z in blocks]
